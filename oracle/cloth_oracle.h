@@ -1,0 +1,70 @@
+/* CPU oracle: a single-threaded FP64 C restatement of the reference's
+ * implicit cloth step with self-contact (nobuyuki83/cloth_sim_self_collision).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py load this library, as the checker.  The product
+ * (cloth_sim_self_collision_amd) never links, loads or calls it.
+ *
+ * The API mirrors oracle/ref_harness.cpp (prefix orc_ instead of ref_) so a
+ * test can run the same call against the restatement and against the compiled
+ * reference and compare the results bit for bit.
+ *
+ * params[12]: lambda, mu, stiff_bend, gx, gy, gz, mass_point, stiff_contact,
+ *             clearance, dt, pcg_tol, pcg_maxit
+ * obst[8]:    kind (0 none, 1 plane, 2 sphere, 3 inverted sphere), then
+ *             plane (nx, ny, nz, d) or sphere (cx, cy, cz, r)
+ * contact lists: 5 ints per element: is_fv, i0, i1, i2, i3 (canonical key)
+ */
+#ifndef CLOTH_ORACLE_H
+#define CLOTH_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void* orc_create(int np, const double* xyz0, const double* xyz_init, int nt, const int* tri,
+                 int nq, const int* quad, const int* bc, const double* params, const double* obst);
+void orc_destroy(void* h);
+int orc_nnz(void* h);
+int orc_nnode(void* h);
+int orc_root(void* h);
+int orc_nedge(void* h);
+void orc_get_pattern(void* h, int* rowptr, int* colind, int* diaind);
+void orc_get_bvh(void* h, int* nodes);
+void orc_get_edge(void* h, int* index, int* array);
+void orc_assemble(void* h, const double* xyz, const double* uvw, double* dia, double* crs,
+                  double* b);
+void orc_ilu(void* h, const double* dia, const double* crs, double* fdia, double* fcrs);
+void orc_pcg(void* h, const double* dia, const double* crs, const double* fdia, const double* fcrs,
+             double* r, double* x, double tol, int maxit, double* conv, int* iters);
+void orc_ilu_solve(void* h, const double* fdia, const double* fcrs, double* vec);
+void orc_matvec(void* h, const double* dia, const double* crs, double alpha, const double* x,
+                double beta, double* y);
+void orc_step_internal_tol(void* h, double* xyz, double* uvw, double tol, int maxit, double* conv,
+                           int* iters);
+void orc_refit_prx(void* h, const double* xyz, double delta, double* bb);
+void orc_refit_ccd(void* h, const double* xyz, const double* uvw, double dt, double* bb);
+int orc_prox_set(void* h, const double* xyz, double delta, int* out, int cap);
+int orc_ccd_set(void* h, const double* xyz, const double* uvw, double dt, double delta, int* out,
+                int cap);
+int orc_contact_resolve(void* h, const double* xyz1, double* uvw);
+void orc_update_velocity(void* h, double* uvw, const double* xyz);
+void orc_step(void* h, double* xyz, double* uvw, int* stats, double* conv);
+
+void orc_cst(const double* C, const double* c, double lam, double mu, double* W, double* dW,
+             double* ddW);
+void orc_bend(const double* C, const double* c, double stiff, double* dW, double* ddW);
+void orc_contact_elem(const double* c, double k, double clearance, const double* obst, double* W,
+                      double* dW, double* ddW);
+void orc_psd22(const double* s2, double* s3);
+double orc_coplanar(const double* s, const double* e);
+double orc_dist_fv(const double* p, double* w);
+double orc_dist_ee(const double* p, double* w);
+
+/* dot-product summation used inside PCG: 0 = sequential left to right (the
+ * reference), 1 = blocked pairwise (a perturbation used by sensitivity tests) */
+void orc_set_dot_mode(void* h, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
