@@ -1,0 +1,614 @@
+// C-ABI of the cloth step (include/clothsim.h): handle lifetime, uploads,
+// and the host orchestration of StepTime (self_contact_sparse/main.cpp:213-263).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/clothsim.h"
+#include "cs_contact.hpp"
+#include "cs_device.hpp"
+#include "cs_setup.hpp"
+#include "cs_solve.hpp"
+
+namespace cs {
+
+namespace {
+thread_local std::string g_err;
+}
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+// One device allocation carved into aligned sub-arrays.
+class Arena {
+ public:
+  template <typename T>
+  void reserve(T** slot, size_t count) {
+    Item it;
+    it.slot = (void**)slot;
+    it.offset = total_;
+    items_.push_back(it);
+    total_ += ((count * sizeof(T) + 255) / 256) * 256;
+    if (count == 0) total_ += 256;
+  }
+  int commit() {
+    if (total_ == 0) return kOk;
+    CS_HIP(hipMalloc(&base_, total_));
+    for (auto& it : items_) *it.slot = (char*)base_ + it.offset;
+    return kOk;
+  }
+  void release() {
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+  }
+  size_t bytes() const { return total_; }
+
+ private:
+  struct Item {
+    void** slot;
+    size_t offset;
+  };
+  std::vector<Item> items_;
+  size_t total_ = 0;
+  void* base_ = nullptr;
+};
+
+struct Cloth {
+  MeshSetup S;
+  Params P;
+  DevCloth d;
+  Arena arena;
+  hipStream_t stream = nullptr;
+  int* hflags = nullptr;  // pinned
+  double* hscal = nullptr;
+  int* ising = nullptr;   // per-row singular-pivot flags
+  ContactState contact;
+  bool have_x1 = false;
+  int poll = 8;
+};
+
+namespace {
+
+template <typename T>
+int upload(T* dst, const std::vector<T>& v, hipStream_t s) {
+  if (!v.empty()) CS_HIP(hipMemcpyAsync(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+  return kOk;
+}
+
+Params to_params(const cs_params* p) {
+  Params P;
+  P.lam = p->lambda;
+  P.mu = p->myu;
+  P.stiff_bend = p->stiff_bend;
+  for (int k = 0; k < 3; k++) P.grav[k] = p->gravity[k];
+  P.mass = p->mass_point;
+  P.k_contact = p->stiff_contact;
+  P.clearance = p->contact_clearance;
+  P.dt = p->dt;
+  P.pcg_tol = p->pcg_tol;
+  P.pcg_maxit = p->pcg_maxit;
+  P.obstacle_kind = p->obstacle_kind;
+  for (int k = 0; k < 4; k++) P.obstacle[k] = p->obstacle[k];
+  return P;
+}
+
+int check_params(const cs_params* p) {
+  if (!p) {
+    set_error("params is NULL");
+    return kBadArg;
+  }
+  if (!(p->dt > 0) || !(p->mass_point > 0) || p->pcg_maxit < 0 || !(p->pcg_tol >= 0)) {
+    set_error("params need dt > 0, mass_point > 0, pcg_maxit >= 0, pcg_tol >= 0");
+    return kBadArg;
+  }
+  if (p->obstacle_kind < 0 || p->obstacle_kind > 3) {
+    set_error("unknown obstacle kind");
+    return kBadArg;
+  }
+  return kOk;
+}
+
+int build(Cloth* c, const cs_mesh* m) {
+  MeshSetup& S = c->S;
+  std::string err = build_setup(S, m->np, m->nt, m->nq, m->tri, m->quad, m->xyz);
+  if (!err.empty()) {
+    set_error(err);
+    return kBadArg;
+  }
+  const int np = S.np, nt = S.nt, nq = S.nq, nnz = S.nnz, nn = S.nnode;
+  DevCloth& d = c->d;
+  std::memset(&d, 0, sizeof(d));
+  d.np = np;
+  d.nt = nt;
+  d.nq = nq;
+  d.nnz = nnz;
+  d.nnode = nn;
+  d.nlf = S.nlf;
+  d.nlb = S.nlb;
+  Arena& A = c->arena;
+  int *tri, *quad, *bc, *rowptr, *colind, *diaind, *rowof, *lfp, *lfr, *lbp, *lbr;
+  int *csp, *cs_, *dsp, *ds, *gsp, *gs, *rp, *rg, *node, *pre, *pree, *bypre, *leafof, *rord;
+  double* xyz0;
+  A.reserve(&tri, 3 * (size_t)nt);
+  A.reserve(&quad, 4 * (size_t)nq);
+  A.reserve(&bc, np);
+  A.reserve(&xyz0, 3 * (size_t)np);
+  A.reserve(&rowptr, np + 1);
+  A.reserve(&colind, nnz);
+  A.reserve(&diaind, np);
+  A.reserve(&rowof, nnz);
+  A.reserve(&lfp, S.lf_ptr.size());
+  A.reserve(&lfr, S.lf_rows.size());
+  A.reserve(&lbp, S.lb_ptr.size());
+  A.reserve(&lbr, S.lb_rows.size());
+  A.reserve(&csp, S.crs_src_ptr.size());
+  A.reserve(&cs_, S.crs_src.size());
+  A.reserve(&dsp, S.dia_src_ptr.size());
+  A.reserve(&ds, S.dia_src.size());
+  A.reserve(&gsp, S.grad_src_ptr.size());
+  A.reserve(&gs, S.grad_src.size());
+  A.reserve(&rp, S.ring_ptr.size());
+  A.reserve(&rg, S.ring.size());
+  A.reserve(&node, 3 * (size_t)nn);
+  A.reserve(&pre, nn);
+  A.reserve(&pree, nn);
+  A.reserve(&bypre, nn);
+  A.reserve(&leafof, nt);
+  A.reserve(&rord, nn);
+  A.reserve(&d.x, 3 * (size_t)np);
+  A.reserve(&d.v, 3 * (size_t)np);
+  A.reserve(&d.x1, 3 * (size_t)np);
+  A.reserve(&d.dia, 9 * (size_t)np);
+  A.reserve(&d.crs, 9 * (size_t)nnz);
+  A.reserve(&d.b, 3 * (size_t)np);
+  A.reserve(&d.fdia, 9 * (size_t)np);
+  A.reserve(&d.fcrs, 9 * (size_t)nnz);
+  A.reserve(&d.etri, 90 * (size_t)nt);
+  A.reserve(&d.kq, 16 * (size_t)nq);
+  A.reserve(&d.eq, 12 * (size_t)nq);
+  A.reserve(&d.r, 3 * (size_t)np);
+  A.reserve(&d.dx, 3 * (size_t)np);
+  A.reserve(&d.z, 3 * (size_t)np);
+  A.reserve(&d.ap, 3 * (size_t)np);
+  A.reserve(&d.p[0], 3 * (size_t)np);
+  A.reserve(&d.p[1], 3 * (size_t)np);
+  A.reserve(&d.prod, 3 * (size_t)np);
+  A.reserve(&d.scal, kNumScal);
+  A.reserve(&d.flags, kNumFlag);
+  A.reserve(&d.bb, 6 * (size_t)nn);
+  A.reserve(&c->ising, np);
+  double* host_pd;
+  A.reserve(&host_pd, 4 * (size_t)np);
+  if (int st = A.commit()) return st;
+  d.tri = tri;
+  d.quad = quad;
+  d.bc = bc;
+  d.xyz0 = xyz0;
+  d.rowptr = rowptr;
+  d.colind = colind;
+  d.diaind = diaind;
+  d.rowof = rowof;
+  d.lf_ptr = lfp;
+  d.lf_rows = lfr;
+  d.lb_ptr = lbp;
+  d.lb_rows = lbr;
+  d.crs_src_ptr = csp;
+  d.crs_src = cs_;
+  d.dia_src_ptr = dsp;
+  d.dia_src = ds;
+  d.grad_src_ptr = gsp;
+  d.grad_src = gs;
+  d.ring_ptr = rp;
+  d.ring = rg;
+  d.node = node;
+  d.pre = pre;
+  d.pre_end = pree;
+  d.by_pre = bypre;
+  d.leaf_of = leafof;
+  d.refit_order = rord;
+  hipStream_t s = c->stream;
+  CS_HIP(hipMemcpyAsync(tri, m->tri, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, s));
+  if (nq) CS_HIP(hipMemcpyAsync(quad, m->quad, sizeof(int) * 4 * nq, hipMemcpyHostToDevice, s));
+  std::vector<int> bcv(np, 0);
+  if (m->bc)
+    for (int i = 0; i < np; i++) bcv[i] = m->bc[i] != 0;
+  if (int st = upload(bc, bcv, s)) return st;
+  CS_HIP(hipMemcpyAsync(xyz0, m->xyz0, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s));
+  if (int st = upload(rowptr, S.rowptr, s)) return st;
+  if (int st = upload(colind, S.colind, s)) return st;
+  if (int st = upload(diaind, S.diaind, s)) return st;
+  if (int st = upload(rowof, S.rowof, s)) return st;
+  if (int st = upload(lfp, S.lf_ptr, s)) return st;
+  if (int st = upload(lfr, S.lf_rows, s)) return st;
+  if (int st = upload(lbp, S.lb_ptr, s)) return st;
+  if (int st = upload(lbr, S.lb_rows, s)) return st;
+  if (int st = upload(csp, S.crs_src_ptr, s)) return st;
+  if (int st = upload(cs_, S.crs_src, s)) return st;
+  if (int st = upload(dsp, S.dia_src_ptr, s)) return st;
+  if (int st = upload(ds, S.dia_src, s)) return st;
+  if (int st = upload(gsp, S.grad_src_ptr, s)) return st;
+  if (int st = upload(gs, S.grad_src, s)) return st;
+  if (int st = upload(rp, S.ring_ptr, s)) return st;
+  if (int st = upload(rg, S.ring, s)) return st;
+  if (int st = upload(node, S.node, s)) return st;
+  if (int st = upload(pre, S.pre, s)) return st;
+  if (int st = upload(pree, S.pre_end, s)) return st;
+  if (int st = upload(bypre, S.by_pre, s)) return st;
+  if (int st = upload(leafof, S.leaf_of, s)) return st;
+  if (int st = upload(rord, S.refit_order, s)) return st;
+  CS_HIP(hipMemcpyAsync(d.x, m->xyz, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s));
+  if (m->uvw)
+    CS_HIP(hipMemcpyAsync(d.v, m->uvw, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s));
+  else
+    CS_HIP(hipMemsetAsync(d.v, 0, sizeof(double) * 3 * np, s));
+  CS_HIP(hipMemsetAsync(d.flags, 0, sizeof(int) * kNumFlag, s));
+  CS_HIP(hipMemsetAsync(d.scal, 0, sizeof(double) * kNumScal, s));
+  c->contact.host_pd_buf = host_pd;
+  if (int st = launch_bend_setup(d, c->P, s)) return st;
+  if (int st = contact_init(c->contact, S, d, s)) return st;
+  CS_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+Cloth* H(cs_cloth* p) { return reinterpret_cast<Cloth*>(p); }
+const Cloth* H(const cs_cloth* p) { return reinterpret_cast<const Cloth*>(p); }
+
+int sync(Cloth* c) {
+  CS_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+// StepTime_InternalDynamicsILU on the resident state
+int internal_step(Cloth* c, double* conv, int* iters) {
+  DevCloth& d = c->d;
+  hipStream_t s = c->stream;
+  CS_HIP(hipMemcpyAsync(d.x1, d.x, sizeof(double) * 3 * d.np, hipMemcpyDeviceToDevice, s));
+  c->have_x1 = true;
+  if (int st = launch_assemble(d, c->P, s)) return st;
+  if (int st = launch_ilu(d, c->ising, s)) return st;
+  if (int st = run_pcg(d, c->P.pcg_tol, c->P.pcg_maxit, c->hflags, s, c->poll)) return st;
+  if (int st = launch_finish_internal(d, c->P, s)) return st;
+  CS_HIP(hipMemcpyAsync(c->hflags, d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
+  CS_HIP(hipMemcpyAsync(c->hscal, d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
+  CS_HIP(hipStreamSynchronize(s));
+  if (conv) *conv = c->hscal[kConv];
+  if (iters) *iters = c->hflags[kDone] ? c->hflags[kIters] : c->P.pcg_maxit;
+  return kOk;
+}
+
+}  // namespace
+}  // namespace cs
+
+using namespace cs;
+
+extern "C" {
+
+const char* cs_last_error(void) { return last_error(); }
+
+int cs_device_count(int* n) {
+  int k = 0;
+  hipError_t e = hipGetDeviceCount(&k);
+  if (e != hipSuccess) k = 0;
+  if (n) *n = k;
+  return k > 0 ? kOk : kNoDevice;
+}
+
+int cs_cloth_create(const cs_mesh* mesh, const cs_params* params, cs_cloth** out) {
+  if (!mesh || !out || !mesh->xyz0 || !mesh->xyz || !mesh->tri || (mesh->nq > 0 && !mesh->quad)) {
+    set_error("cs_cloth_create: missing mesh arrays");
+    return kBadArg;
+  }
+  if (int st = check_params(params)) return st;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device");
+    return kNoDevice;
+  }
+  Cloth* c = new (std::nothrow) Cloth();
+  if (!c) {
+    set_error("host allocation failed");
+    return kOutOfMemory;
+  }
+  c->P = to_params(params);
+  int st = kOk;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&c->hflags, sizeof(int) * kNumFlag) != hipSuccess ||
+      hipHostMalloc((void**)&c->hscal, sizeof(double) * kNumScal) != hipSuccess) {
+    set_error("stream or pinned buffer creation failed");
+    st = kHipError;
+  }
+  if (st == kOk) st = build(c, mesh);
+  if (st != kOk) {
+    cs_cloth_destroy(reinterpret_cast<cs_cloth*>(c));
+    return st;
+  }
+  *out = reinterpret_cast<cs_cloth*>(c);
+  return kOk;
+}
+
+int cs_cloth_destroy(cs_cloth* p) {
+  if (!p) return kOk;
+  Cloth* c = H(p);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  contact_release(c->contact);
+  c->arena.release();
+  if (c->hflags) (void)hipHostFree(c->hflags);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return kOk;
+}
+
+int cs_cloth_set_params(cs_cloth* p, const cs_params* params) {
+  if (!p) return kBadArg;
+  if (int st = check_params(params)) return st;
+  Cloth* c = H(p);
+  const double old_bend = c->P.stiff_bend;
+  c->P = to_params(params);
+  if (c->P.stiff_bend != old_bend) {
+    if (int st = launch_bend_setup(c->d, c->P, c->stream)) return st;
+    return sync(c);
+  }
+  return kOk;
+}
+
+int cs_cloth_sizes(const cs_cloth* p, int* np, int* nt, int* nq, int* nnz, int* nnode) {
+  if (!p) return kBadArg;
+  const Cloth* c = H(p);
+  if (np) *np = c->S.np;
+  if (nt) *nt = c->S.nt;
+  if (nq) *nq = c->S.nq;
+  if (nnz) *nnz = c->S.nnz;
+  if (nnode) *nnode = c->S.nnode;
+  return kOk;
+}
+
+int cs_set_state(cs_cloth* p, const double* xyz, const double* uvw) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  const size_t n = sizeof(double) * 3 * c->S.np;
+  if (xyz) CS_HIP(hipMemcpyAsync(c->d.x, xyz, n, hipMemcpyHostToDevice, c->stream));
+  if (uvw) CS_HIP(hipMemcpyAsync(c->d.v, uvw, n, hipMemcpyHostToDevice, c->stream));
+  return sync(c);
+}
+
+int cs_get_state(cs_cloth* p, double* xyz, double* uvw) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  const size_t n = sizeof(double) * 3 * c->S.np;
+  if (xyz) CS_HIP(hipMemcpyAsync(xyz, c->d.x, n, hipMemcpyDeviceToHost, c->stream));
+  if (uvw) CS_HIP(hipMemcpyAsync(uvw, c->d.v, n, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+int cs_state_device(cs_cloth* p, double** xyz, double** uvw) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  if (xyz) *xyz = c->d.x;
+  if (uvw) *uvw = c->d.v;
+  return kOk;
+}
+
+int cs_set_step_start(cs_cloth* p, const double* xyz1) {
+  if (!p || !xyz1) return kBadArg;
+  Cloth* c = H(p);
+  CS_HIP(hipMemcpyAsync(c->d.x1, xyz1, sizeof(double) * 3 * c->S.np, hipMemcpyHostToDevice,
+                        c->stream));
+  c->have_x1 = true;
+  return sync(c);
+}
+
+int cs_set_obstacle_values(cs_cloth* p, const double* pd, const double* n) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  if (!pd || !n) {
+    c->d.host_pd = nullptr;
+    c->d.host_n = nullptr;
+    return kOk;
+  }
+  const int np = c->S.np;
+  double* buf = c->contact.host_pd_buf;
+  CS_HIP(hipMemcpyAsync(buf, pd, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+  CS_HIP(hipMemcpyAsync(buf + np, n, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));
+  c->d.host_pd = buf;
+  c->d.host_n = buf + np;
+  return sync(c);
+}
+
+int cs_step_internal(cs_cloth* p, double* conv_ratio, int* iteration) {
+  if (!p) return kBadArg;
+  return internal_step(H(p), conv_ratio, iteration);
+}
+
+int cs_contact_resolve(cs_cloth* p, int* impulse_applied, cs_step_stats* stats) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  if (!c->have_x1) {
+    set_error("cs_contact_resolve needs the start-of-step positions of cs_step_internal");
+    return kBadArg;
+  }
+  int applied = 0;
+  if (int st = contact_resolve(c->contact, c->S, c->d, c->P, c->stream, &applied, stats)) return st;
+  if (impulse_applied) *impulse_applied = applied;
+  return kOk;
+}
+
+int cs_post_contact_update(cs_cloth* p) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  if (int st = post_contact_update(c->d, c->P, c->stream)) return st;
+  return sync(c);
+}
+
+int cs_step(cs_cloth* p, int n, cs_step_stats* stats) {
+  if (!p || n < 0) return kBadArg;
+  Cloth* c = H(p);
+  for (int k = 0; k < n; k++) {
+    cs_step_stats st_local;
+    std::memset(&st_local, 0, sizeof(st_local));
+    double conv = 0;
+    int iters = 0;
+    if (int st = internal_step(c, &conv, &iters)) return st;
+    int applied = 0;
+    if (int st = contact_resolve(c->contact, c->S, c->d, c->P, c->stream, &applied, &st_local))
+      return st;
+    if (applied) {
+      if (int st = post_contact_update(c->d, c->P, c->stream)) return st;
+    }
+    st_local.pcg_conv = conv;
+    st_local.pcg_iters = iters;
+    st_local.ilu_singular = c->hflags[kNsing];
+    st_local.impulse_applied = applied;
+    if (stats) *stats = st_local;
+  }
+  return sync(c);
+}
+
+int cs_get_pattern(const cs_cloth* p, int* rowptr, int* colind, int* diaind) {
+  if (!p) return kBadArg;
+  const Cloth* c = H(p);
+  if (rowptr) std::memcpy(rowptr, c->S.rowptr.data(), sizeof(int) * (c->S.np + 1));
+  if (colind) std::memcpy(colind, c->S.colind.data(), sizeof(int) * c->S.nnz);
+  if (diaind) std::memcpy(diaind, c->S.diaind.data(), sizeof(int) * c->S.np);
+  return kOk;
+}
+
+int cs_assemble(cs_cloth* p, const double* xyz, const double* uvw) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  const size_t n = sizeof(double) * 3 * c->S.np;
+  if (xyz) CS_HIP(hipMemcpyAsync(c->d.x, xyz, n, hipMemcpyHostToDevice, c->stream));
+  if (uvw) CS_HIP(hipMemcpyAsync(c->d.v, uvw, n, hipMemcpyHostToDevice, c->stream));
+  if (int st = launch_assemble(c->d, c->P, c->stream)) return st;
+  return sync(c);
+}
+
+int cs_get_system(cs_cloth* p, double* dia, double* crs, double* b) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  hipStream_t s = c->stream;
+  if (dia) CS_HIP(hipMemcpyAsync(dia, c->d.dia, sizeof(double) * 9 * c->S.np, hipMemcpyDeviceToHost, s));
+  if (crs && c->S.nnz)
+    CS_HIP(hipMemcpyAsync(crs, c->d.crs, sizeof(double) * 9 * c->S.nnz, hipMemcpyDeviceToHost, s));
+  if (b) CS_HIP(hipMemcpyAsync(b, c->d.b, sizeof(double) * 3 * c->S.np, hipMemcpyDeviceToHost, s));
+  return sync(c);
+}
+
+int cs_set_system(cs_cloth* p, const double* dia, const double* crs, const double* b) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  hipStream_t s = c->stream;
+  if (dia) CS_HIP(hipMemcpyAsync(c->d.dia, dia, sizeof(double) * 9 * c->S.np, hipMemcpyHostToDevice, s));
+  if (crs && c->S.nnz)
+    CS_HIP(hipMemcpyAsync(c->d.crs, crs, sizeof(double) * 9 * c->S.nnz, hipMemcpyHostToDevice, s));
+  if (b) CS_HIP(hipMemcpyAsync(c->d.b, b, sizeof(double) * 3 * c->S.np, hipMemcpyHostToDevice, s));
+  return sync(c);
+}
+
+int cs_ilu_factor(cs_cloth* p, int* n_singular) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  CS_HIP(hipMemsetAsync(c->d.flags, 0, sizeof(int) * kNumFlag, c->stream));
+  if (int st = launch_ilu(c->d, c->ising, c->stream)) return st;
+  CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, c->stream));
+  if (int st = sync(c)) return st;
+  if (n_singular) *n_singular = c->hflags[kNsing];
+  return kOk;
+}
+
+int cs_get_factor(cs_cloth* p, double* fdia, double* fcrs) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  hipStream_t s = c->stream;
+  if (fdia) CS_HIP(hipMemcpyAsync(fdia, c->d.fdia, sizeof(double) * 9 * c->S.np, hipMemcpyDeviceToHost, s));
+  if (fcrs && c->S.nnz)
+    CS_HIP(hipMemcpyAsync(fcrs, c->d.fcrs, sizeof(double) * 9 * c->S.nnz, hipMemcpyDeviceToHost, s));
+  return sync(c);
+}
+
+int cs_set_factor(cs_cloth* p, const double* fdia, const double* fcrs) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  hipStream_t s = c->stream;
+  if (fdia) CS_HIP(hipMemcpyAsync(c->d.fdia, fdia, sizeof(double) * 9 * c->S.np, hipMemcpyHostToDevice, s));
+  if (fcrs && c->S.nnz)
+    CS_HIP(hipMemcpyAsync(c->d.fcrs, fcrs, sizeof(double) * 9 * c->S.nnz, hipMemcpyHostToDevice, s));
+  return sync(c);
+}
+
+int cs_ilu_solve(cs_cloth* p, double* vec) {
+  if (!p || !vec) return kBadArg;
+  Cloth* c = H(p);
+  hipStream_t s = c->stream;
+  const size_t n = sizeof(double) * 3 * c->S.np;
+  CS_HIP(hipMemsetAsync(c->d.flags, 0, sizeof(int) * kNumFlag, s));
+  CS_HIP(hipMemcpyAsync(c->d.r, vec, n, hipMemcpyHostToDevice, s));
+  if (int st = launch_ilu_apply(c->d, c->d.r, c->d.z, s)) return st;
+  CS_HIP(hipMemcpyAsync(vec, c->d.z, n, hipMemcpyDeviceToHost, s));
+  return sync(c);
+}
+
+int cs_matvec(cs_cloth* p, double alpha, const double* x, double beta, double* y) {
+  if (!p || !x || !y) return kBadArg;
+  Cloth* c = H(p);
+  hipStream_t s = c->stream;
+  const size_t n = sizeof(double) * 3 * c->S.np;
+  CS_HIP(hipMemcpyAsync(c->d.r, x, n, hipMemcpyHostToDevice, s));
+  CS_HIP(hipMemcpyAsync(c->d.z, y, n, hipMemcpyHostToDevice, s));
+  if (int st = launch_matvec(c->d, alpha, c->d.r, beta, c->d.z, s)) return st;
+  CS_HIP(hipMemcpyAsync(y, c->d.z, n, hipMemcpyDeviceToHost, s));
+  return sync(c);
+}
+
+int cs_pcg_solve(cs_cloth* p, const double* b, double tol, int maxit, double* x, double* r,
+                 double* conv, int* iters) {
+  if (!p || !b || maxit < 0) return kBadArg;
+  Cloth* c = H(p);
+  hipStream_t s = c->stream;
+  const size_t n = sizeof(double) * 3 * c->S.np;
+  CS_HIP(hipMemcpyAsync(c->d.b, b, n, hipMemcpyHostToDevice, s));
+  if (int st = run_pcg(c->d, tol, maxit, c->hflags, s, c->poll)) return st;
+  if (x) CS_HIP(hipMemcpyAsync(x, c->d.dx, n, hipMemcpyDeviceToHost, s));
+  if (r) CS_HIP(hipMemcpyAsync(r, c->d.r, n, hipMemcpyDeviceToHost, s));
+  CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
+  CS_HIP(hipMemcpyAsync(c->hscal, c->d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
+  if (int st = sync(c)) return st;
+  if (conv) *conv = c->hscal[kConv];
+  if (iters) *iters = c->hflags[kDone] ? c->hflags[kIters] : maxit;
+  return kOk;
+}
+
+int cs_get_bvh(const cs_cloth* p, int* nodes) {
+  if (!p || !nodes) return kBadArg;
+  const Cloth* c = H(p);
+  std::memcpy(nodes, c->S.node.data(), sizeof(int) * 3 * c->S.nnode);
+  return kOk;
+}
+
+int cs_bvh_refit_prx(cs_cloth* p, const double* xyz, double delta, double* bb) {
+  if (!p || !xyz) return kBadArg;
+  Cloth* c = H(p);
+  return contact_refit_host(c->contact, c->S, c->d, c->stream, xyz, nullptr, delta, 0.0, bb);
+}
+
+int cs_bvh_refit_ccd(cs_cloth* p, const double* xyz, const double* uvw, double dt, double* bb) {
+  if (!p || !xyz || !uvw) return kBadArg;
+  Cloth* c = H(p);
+  return contact_refit_host(c->contact, c->S, c->d, c->stream, xyz, uvw, 0.0, dt, bb);
+}
+
+int cs_query_proximity(cs_cloth* p, const double* xyz, double delta, int* out, int cap, int* n) {
+  if (!p || !xyz || cap < 0) return kBadArg;
+  Cloth* c = H(p);
+  return contact_query_host(c->contact, c->S, c->d, c->stream, xyz, nullptr, delta, 0.0, out, cap, n);
+}
+
+int cs_query_ccd(cs_cloth* p, const double* xyz, const double* uvw, double dt, double delta,
+                 int* out, int cap, int* n) {
+  if (!p || !xyz || !uvw || cap < 0) return kBadArg;
+  Cloth* c = H(p);
+  return contact_query_host(c->contact, c->S, c->d, c->stream, xyz, uvw, delta, dt, out, cap, n);
+}
+
+}  // extern "C"

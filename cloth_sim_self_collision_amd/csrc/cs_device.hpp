@@ -1,0 +1,127 @@
+// Device-resident state of one cloth and the host orchestration around it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cs_setup.hpp"
+
+namespace cs {
+
+// status codes of the C-ABI (include/clothsim.h)
+enum Status : int {
+  kOk = 0,
+  kBadArg = 1,
+  kHipError = 2,
+  kOutOfMemory = 3,
+  kSingularPivot = 4,
+  kNotConverged = 5,
+  kNonFinite = 6,
+  kCapacity = 7,
+  kNoDevice = 8,
+};
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define CS_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      ::cs::set_error(std::string(#call) + ": " + hipGetErrorString(e_));                 \
+      return ::cs::kHipError;                                                             \
+    }                                                                                     \
+  } while (0)
+
+struct Params {
+  double lam, mu, stiff_bend, grav[3], mass, k_contact, clearance, dt;
+  double pcg_tol;
+  int pcg_maxit;
+  int obstacle_kind;
+  double obstacle[4];
+};
+
+// device scalar slots of the linear solve
+enum Scal : int {
+  kRz = 0,      // (r, M^-1 r) of the current iteration
+  kInvRr0 = 1,  // 1 / (r0, r0)
+  kAlpha = 2,
+  kBeta = 3,
+  kConv = 4,    // achieved residual ratio
+  kTol2 = 5,    // tolerance squared
+  kLastDot = 6,
+  kNumScal = 16
+};
+enum Flag : int {
+  kDone = 0,    // the solve has converged (or had a zero right-hand side)
+  kIters = 1,   // iteration count reported like Solve_PCG
+  kNsing = 2,   // ILU pivots with |det| <= 1e-30
+  kFirstSing11 = 3,  // row of the 11th singular pivot, np if none
+  kNumFlag = 16
+};
+
+// Plain pointers to every device array of a cloth; passed to kernels by value.
+struct DevCloth {
+  int np, nt, nq, nnz, nnode;
+  int nlf, nlb;
+  // mesh and setup
+  const int* tri;
+  const int* quad;
+  const int* bc;
+  const double* xyz0;
+  const int* rowptr;
+  const int* colind;
+  const int* diaind;
+  const int* rowof;
+  const int* lf_ptr;
+  const int* lf_rows;
+  const int* lb_ptr;
+  const int* lb_rows;
+  const int* crs_src_ptr;
+  const int* crs_src;
+  const int* dia_src_ptr;
+  const int* dia_src;
+  const int* grad_src_ptr;
+  const int* grad_src;
+  const int* ring_ptr;
+  const int* ring;
+  const int* node;
+  const int* pre;
+  const int* pre_end;
+  const int* by_pre;
+  const int* leaf_of;
+  const int* refit_order;
+  // state
+  double* x;   // positions (3 np)
+  double* v;   // velocities (3 np)
+  double* x1;  // positions at the start of the step
+  // linear system
+  double* dia;
+  double* crs;
+  double* b;
+  double* fdia;
+  double* fcrs;
+  double* etri;  // per triangle: 81 Hessian values then 9 gradient values
+  double* kq;    // per bending stencil: 16 Hessian scalars (constant in time)
+  double* eq;    // per bending stencil: 12 gradient values
+  const double* host_pd;  // optional per-vertex obstacle depth (np) from a host callback
+  const double* host_n;   // and normal (3 np); null -> analytic obstacle
+  // PCG
+  double* r;
+  double* dx;
+  double* z;
+  double* ap;
+  double* p[2];
+  double* prod;  // element-wise products feeding the next reduction
+  double* scal;
+  int* flags;
+  // BVH boxes (6 per node)
+  double* bb;
+};
+
+struct Cloth;  // host handle (cs_api.cpp)
+
+}  // namespace cs

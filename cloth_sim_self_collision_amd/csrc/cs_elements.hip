@@ -1,0 +1,148 @@
+// Element routines over arrays (cloth_internal_physics.h) and a device
+// arithmetic self-check, exposed through the C-ABI for tests and callers
+// that evaluate single elements.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/clothsim.h"
+#include "cs_device.hpp"
+#include "cs_math.hpp"
+
+namespace cs {
+
+__global__ void k_elem_cst(int n, const double* C, const double* x, double lam, double mu,
+                           double* dW, double* ddW) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  CstState s;
+  cst_state(C + 9 * (size_t)e, x + 9 * (size_t)e, lam, mu, s);
+  for (int a = 0; a < 3; a++)
+    for (int i = 0; i < 3; i++) dW[9 * (size_t)e + a * 3 + i] = cst_grad(s, a, i);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) cst_hess_block(s, a, b, ddW + 81 * (size_t)e + (a * 3 + b) * 9);
+}
+
+// Hessian laid out [4][4][3][3] like the reference (w_a w_b scale on the diagonal)
+__global__ void k_elem_bend(int n, const double* C, const double* x, double stiff, double* dW,
+                            double* ddW) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double w[4], s, kab[16];
+  bend_weights(C + 12 * (size_t)e, stiff, w, &s);
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) kab[a * 4 + b] = w[a] * w[b] * s;
+  double* H = ddW + 144 * (size_t)e;
+  for (int q = 0; q < 144; q++) H[q] = 0;
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++)
+      for (int i = 0; i < 3; i++) H[((a * 4 + b) * 3 + i) * 3 + i] = kab[a * 4 + b];
+  for (int a = 0; a < 4; a++)
+    for (int i = 0; i < 3; i++) dW[12 * (size_t)e + a * 3 + i] = bend_grad(kab, x + 12 * (size_t)e, a, i);
+}
+
+__global__ void k_elem_contact(int n, const double* p, double k, double clearance, int kind,
+                               double o0, double o1, double o2, double o3, double* dW,
+                               double* ddW) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double q[4] = {o0, o1, o2, o3};
+  double pd, nrm[3];
+  obstacle_depth(kind, q, p + 3 * (size_t)e, &pd, nrm);
+  contact_penalty(pd, nrm, k, clearance, dW + 3 * (size_t)e, ddW + 9 * (size_t)e);
+}
+
+__global__ void k_divsqrt(int n, const double* a, const double* b, double* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  out[2 * e] = a[e] / b[e];
+  out[2 * e + 1] = sqrt(a[e]);
+}
+
+namespace {
+
+// device round trip for the element entry points
+struct Scratch {
+  std::vector<void*> ptrs;
+  ~Scratch() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int get(T** p, size_t n) {
+    CS_HIP(hipMalloc((void**)p, sizeof(T) * (n ? n : 1)));
+    ptrs.push_back(*p);
+    return kOk;
+  }
+};
+
+unsigned blocks(int n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+}  // namespace cs
+
+using namespace cs;
+
+extern "C" int cs_element_cst(int n, const double* C, const double* c, double lambda, double myu,
+                              double* dW, double* ddW) {
+  if (n < 0 || (n > 0 && (!C || !c || !dW || !ddW))) return kBadArg;
+  if (n == 0) return kOk;
+  Scratch s;
+  double *dC, *dc, *dg, *dh;
+  if (s.get(&dC, 9L * n) || s.get(&dc, 9L * n) || s.get(&dg, 9L * n) || s.get(&dh, 81L * n))
+    return kHipError;
+  CS_HIP(hipMemcpy(dC, C, sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+  CS_HIP(hipMemcpy(dc, c, sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_elem_cst, dim3(blocks(n)), dim3(256), 0, 0, n, dC, dc, lambda, myu, dg, dh);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpy(dW, dg, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));
+  CS_HIP(hipMemcpy(ddW, dh, sizeof(double) * 81 * n, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+extern "C" int cs_element_bend(int n, const double* C, const double* c, double stiff, double* dW,
+                               double* ddW) {
+  if (n < 0 || (n > 0 && (!C || !c || !dW || !ddW))) return kBadArg;
+  if (n == 0) return kOk;
+  Scratch s;
+  double *dC, *dc, *dg, *dh;
+  if (s.get(&dC, 12L * n) || s.get(&dc, 12L * n) || s.get(&dg, 12L * n) || s.get(&dh, 144L * n))
+    return kHipError;
+  CS_HIP(hipMemcpy(dC, C, sizeof(double) * 12 * n, hipMemcpyHostToDevice));
+  CS_HIP(hipMemcpy(dc, c, sizeof(double) * 12 * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_elem_bend, dim3(blocks(n)), dim3(256), 0, 0, n, dC, dc, stiff, dg, dh);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpy(dW, dg, sizeof(double) * 12 * n, hipMemcpyDeviceToHost));
+  CS_HIP(hipMemcpy(ddW, dh, sizeof(double) * 144 * n, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+extern "C" int cs_element_contact(int n, const double* p, double stiff, double clearance,
+                                  int kind, const double* obstacle, double* dW, double* ddW) {
+  if (n < 0 || (n > 0 && (!p || !obstacle || !dW || !ddW)) || kind < 0 || kind > 3)
+    return kBadArg;
+  if (n == 0) return kOk;
+  Scratch s;
+  double *dp, *dg, *dh;
+  if (s.get(&dp, 3L * n) || s.get(&dg, 3L * n) || s.get(&dh, 9L * n)) return kHipError;
+  CS_HIP(hipMemcpy(dp, p, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_elem_contact, dim3(blocks(n)), dim3(256), 0, 0, n, dp, stiff, clearance,
+                     kind, obstacle[0], obstacle[1], obstacle[2], obstacle[3], dg, dh);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpy(dW, dg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  CS_HIP(hipMemcpy(ddW, dh, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+extern "C" int cs_selftest_divsqrt(int n, const double* a, const double* b, double* out) {
+  if (n < 0 || (n > 0 && (!a || !b || !out))) return kBadArg;
+  if (n == 0) return kOk;
+  Scratch s;
+  double *da, *db, *dout;
+  if (s.get(&da, n) || s.get(&db, n) || s.get(&dout, 2L * n)) return kHipError;
+  CS_HIP(hipMemcpy(da, a, sizeof(double) * n, hipMemcpyHostToDevice));
+  CS_HIP(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_divsqrt, dim3(blocks(n)), dim3(256), 0, 0, n, da, db, dout);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpy(out, dout, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+  return kOk;
+}

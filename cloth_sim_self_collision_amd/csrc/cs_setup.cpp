@@ -1,0 +1,399 @@
+// Host-side, per-mesh setup of the cloth step (see cs_setup.hpp).
+#include "cs_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace cs {
+namespace {
+
+// vertex -> incident elements, in element order
+void incidence(const int* elem, int ne, int nn, int np, std::vector<int>& ptr,
+               std::vector<int>& idx) {
+  ptr.assign(np + 1, 0);
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < nn; k++) ptr[elem[e * nn + k] + 1]++;
+  for (int i = 0; i < np; i++) ptr[i + 1] += ptr[i];
+  idx.resize(ptr[np]);
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < nn; k++) idx[fill[elem[e * nn + k]]++] = e;
+}
+
+// vertices sharing an element with each vertex, self excluded, in discovery
+// order over the incident elements (jagged_array.h:158-207, is_dia=false)
+void neighbours(const int* elem, int ne, int nn, int np, std::vector<int>& ptr,
+                std::vector<int>& out) {
+  std::vector<int> vptr, vidx;
+  incidence(elem, ne, nn, np, vptr, vidx);
+  std::vector<int> seen(np, -1);
+  ptr.assign(np + 1, 0);
+  out.clear();
+  for (int v = 0; v < np; v++) {
+    seen[v] = v;
+    for (int k = vptr[v]; k < vptr[v + 1]; k++) {
+      const int e = vidx[k];
+      for (int a = 0; a < nn; a++) {
+        const int w = elem[e * nn + a];
+        if (seen[w] != v) {
+          seen[w] = v;
+          out.push_back(w);
+        }
+      }
+    }
+    ptr[v + 1] = (int)out.size();
+  }
+}
+
+// ---- top-down BVH whose node numbering follows the reference's creation
+// order: children are created in pairs, child 0 is the flood-filled side
+// (bvh_aabb.cpp:109-313)
+
+struct Centroid {
+  double x, y, z;
+};
+
+struct BvhBuilder {
+  const int* tri;
+  const double* xyz;
+  std::vector<int> adj;    // 3 per triangle, -1 on boundary
+  std::vector<int> owner;  // node currently owning each triangle
+  std::vector<int>& node;
+  std::vector<int> stack;
+
+  BvhBuilder(std::vector<int>& n) : node(n) {}
+
+  Centroid centroid(int t) const {
+    const int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+    Centroid g;
+    g.x = (xyz[3 * a] + xyz[3 * b] + xyz[3 * c]) / 3.0;
+    g.y = (xyz[3 * a + 1] + xyz[3 * b + 1] + xyz[3 * c + 1]) / 3.0;
+    g.z = (xyz[3 * a + 2] + xyz[3 * b + 2] + xyz[3 * c + 2]) / 3.0;
+    return g;
+  }
+  static double proj(const Centroid& g, const double o[3], const double d[3]) {
+    return (g.x - o[0]) * d[0] + (g.y - o[1]) * d[1] + (g.z - o[2]) * d[2];
+  }
+
+  // first triangle (in list order) off the splitting plane; orients d towards it
+  int pick_seed(const std::vector<int>& list, const double o[3], double d[3]) const {
+    for (int t : list) {
+      const Centroid g = centroid(t);
+      if (std::fabs(proj(g, o, d)) < 1.0e-10) continue;
+      if (proj(g, o, d) < 0) {
+        d[0] *= -1;
+        d[1] *= -1;
+        d[2] *= -1;
+      }
+      return t;
+    }
+    return -1;
+  }
+
+  bool split(int parent, const std::vector<int>& list) {
+    const double eps = 1.0e-10;
+    // box of the centroids, grown point by point with margin eps (aabb.h:102-122)
+    double lo[3], hi[3];
+    for (size_t i = 0; i < list.size(); i++) {
+      const Centroid g = centroid(list[i]);
+      const double p[3] = {g.x, g.y, g.z};
+      for (int k = 0; k < 3; k++) {
+        if (i == 0) {
+          lo[k] = p[k] - eps;
+          hi[k] = p[k] + eps;
+        } else {
+          lo[k] = (lo[k] < p[k] - eps) ? lo[k] : p[k] - eps;
+          hi[k] = (hi[k] > p[k] + eps) ? hi[k] : p[k] + eps;
+        }
+      }
+    }
+    const double lx = hi[0] - lo[0], ly = hi[1] - lo[1], lz = hi[2] - lo[2];
+    double d[3] = {0, 0, 0};  // strictly longest axis; stays zero on a tie
+    if (lx > ly && lx > lz) d[0] = 1;
+    if (ly > lz && ly > lx) d[1] = 1;
+    if (lz > lx && lz > ly) d[2] = 1;
+    double o[3] = {(lo[0] + hi[0]) * 0.5, (lo[1] + hi[1]) * 0.5, (lo[2] + hi[2]) * 0.5};
+    int seed = pick_seed(list, o, d);
+    if (seed < 0) {
+      // principal axis of the centroid scatter: 10 power iterations from (1,1,1)
+      double m[3] = {0, 0, 0};
+      for (int t : list) {
+        const Centroid g = centroid(t);
+        m[0] += g.x;
+        m[1] += g.y;
+        m[2] += g.z;
+      }
+      const double n = (double)list.size();
+      if (!(std::fabs(n) < 1.e-16)) {
+        m[0] /= n;
+        m[1] /= n;
+        m[2] /= n;
+      }
+      o[0] = m[0];
+      o[1] = m[1];
+      o[2] = m[2];
+      double cov[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int t : list) {
+        const Centroid g = centroid(t);
+        const double r[3] = {g.x - o[0], g.y - o[1], g.z - o[2]};
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) cov[i][j] += r[i] * r[j];
+      }
+      d[0] = d[1] = d[2] = 1;
+      for (int it = 0; it < 10; it++) {
+        double t[3];
+        for (int i = 0; i < 3; i++) t[i] = cov[i][0] * d[0] + cov[i][1] * d[1] + cov[i][2] * d[2];
+        const double len = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        for (int i = 0; i < 3; i++) d[i] = t[i] / len;
+      }
+      seed = pick_seed(list, o, d);
+      if (seed < 0) return false;
+    }
+    const int c0 = (int)node.size() / 3;
+    const int c1 = c0 + 1;
+    node.insert(node.end(), {parent, 0, 0, parent, 0, 0});
+    node[3 * parent + 1] = c0;
+    node[3 * parent + 2] = c1;
+    // child 0: LIFO flood fill from the seed over adjacent triangles that are
+    // still owned by the parent and lie on the non-negative side
+    std::vector<int> l0;
+    owner[seed] = c0;
+    l0.push_back(seed);
+    stack.clear();
+    stack.push_back(seed);
+    while (!stack.empty()) {
+      const int t = stack.back();
+      stack.pop_back();
+      for (int e = 0; e < 3; e++) {
+        const int u = adj[3 * t + e];
+        if (u == -1 || owner[u] != parent) continue;
+        if (proj(centroid(u), o, d) < 0) continue;
+        stack.push_back(u);
+        owner[u] = c0;
+        l0.push_back(u);
+      }
+    }
+    std::vector<int> l1;
+    for (int t : list) {
+      if (owner[t] == c0) continue;
+      owner[t] = c1;
+      l1.push_back(t);
+    }
+    if (l1.empty()) return false;
+    if (l0.size() == 1) {
+      node[3 * c0 + 1] = l0[0];
+      node[3 * c0 + 2] = -1;
+    } else if (!split(c0, l0)) {
+      return false;
+    }
+    l0.clear();
+    l0.shrink_to_fit();
+    if (l1.size() == 1) {
+      node[3 * c1 + 1] = l1[0];
+      node[3 * c1 + 2] = -1;
+    } else if (!split(c1, l1)) {
+      return false;
+    }
+    return true;
+  }
+};
+
+// triangle adjacency across oppositely oriented shared edges, first match in
+// incidence order (bvh_aabb.cpp:109-138)
+void triangle_adjacency(const int* tri, int nt, int np, std::vector<int>& adj) {
+  static const int ev[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+  std::vector<int> vptr, vidx;
+  incidence(tri, nt, 3, np, vptr, vidx);
+  adj.assign(3 * nt, -1);
+  for (int t = 0; t < nt; t++)
+    for (int e = 0; e < 3; e++) {
+      const int a = tri[3 * t + ev[e][0]], b = tri[3 * t + ev[e][1]];
+      for (int k = vptr[a]; k < vptr[a + 1] && adj[3 * t + e] == -1; k++) {
+        const int u = vidx[k];
+        if (u == t) continue;
+        for (int f = 0; f < 3; f++)
+          if (tri[3 * u + ev[f][1]] == a && tri[3 * u + ev[f][0]] == b) {
+            adj[3 * t + e] = u;
+            break;
+          }
+      }
+    }
+}
+
+void levels_to_lists(const std::vector<int>& lvl, int nlev, std::vector<int>& ptr,
+                     std::vector<int>& rows, int* maxw) {
+  ptr.assign(nlev + 1, 0);
+  for (int l : lvl) ptr[l + 1]++;
+  for (int i = 0; i < nlev; i++) ptr[i + 1] += ptr[i];
+  rows.resize(lvl.size());
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < (int)lvl.size(); i++) rows[fill[lvl[i]]++] = i;
+  *maxw = 0;
+  for (int i = 0; i < nlev; i++) *maxw = std::max(*maxw, ptr[i + 1] - ptr[i]);
+}
+
+}  // namespace
+
+std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,
+                        const double* xyz_init) {
+  if (np <= 0 || nt <= 1 || nq < 0) return "mesh needs np > 0, nt > 1, nq >= 0";
+  for (int i = 0; i < 3 * nt; i++)
+    if (tri[i] < 0 || tri[i] >= np) return "triangle index out of range";
+  for (int i = 0; i < 4 * nq; i++)
+    if (quad[i] < 0 || quad[i] >= np) return "bending stencil index out of range";
+  for (int t = 0; t < nt; t++) {
+    const int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+    if (a == b || b == c || a == c) return "degenerate triangle (repeated vertex)";
+  }
+  s.np = np;
+  s.nt = nt;
+  s.nq = nq;
+  // pattern: union of the bending stencils, columns ascending
+  neighbours(quad, nq, 4, np, s.rowptr, s.colind);
+  for (int i = 0; i < np; i++) std::sort(s.colind.begin() + s.rowptr[i], s.colind.begin() + s.rowptr[i + 1]);
+  s.nnz = s.rowptr[np];
+  s.rowof.resize(s.nnz);
+  s.diaind.resize(np);
+  for (int i = 0; i < np; i++) {
+    s.diaind[i] = s.rowptr[i + 1];
+    for (int k = s.rowptr[i]; k < s.rowptr[i + 1]; k++) {
+      s.rowof[k] = i;
+      if (s.colind[k] > i && s.diaind[i] == s.rowptr[i + 1]) s.diaind[i] = k;
+    }
+  }
+  // level schedules
+  {
+    std::vector<int> lf(np, 0), lb(np, 0);
+    int nf = 0, nb = 0;
+    for (int i = 0; i < np; i++) {
+      int l = 0;
+      for (int k = s.rowptr[i]; k < s.diaind[i]; k++) l = std::max(l, lf[s.colind[k]] + 1);
+      lf[i] = l;
+      nf = std::max(nf, l + 1);
+    }
+    for (int i = np - 1; i >= 0; i--) {
+      int l = 0;
+      for (int k = s.diaind[i]; k < s.rowptr[i + 1]; k++) l = std::max(l, lb[s.colind[k]] + 1);
+      lb[i] = l;
+      nb = std::max(nb, l + 1);
+    }
+    s.nlf = nf;
+    s.nlb = nb;
+    levels_to_lists(lf, nf, s.lf_ptr, s.lf_rows, &s.max_lf);
+    levels_to_lists(lb, nb, s.lb_ptr, s.lb_rows, &s.max_lb);
+  }
+  // gather lists: sources of every Hessian block and gradient entry in the
+  // order the reference's element loops add them (solve_internal_sparse.h:19-76)
+  {
+    std::vector<std::vector<int>> crs(s.nnz), dia(np), grad(np);
+    auto find_col = [&](int i, int j) -> int {
+      const int* b = s.colind.data() + s.rowptr[i];
+      const int* e = s.colind.data() + s.rowptr[i + 1];
+      const int* f = std::lower_bound(b, e, j);
+      return (f != e && *f == j) ? (int)(f - s.colind.data()) : -1;
+    };
+    for (int t = 0; t < nt; t++)
+      for (int a = 0; a < 3; a++) {
+        const int i = tri[3 * t + a];
+        grad[i].push_back(t * 3 + a);
+        for (int b = 0; b < 3; b++) {
+          if (a == b) {
+            dia[i].push_back(t * 9 + a * 3 + b);
+          } else {
+            const int k = find_col(i, tri[3 * t + b]);
+            if (k >= 0) crs[k].push_back(t * 9 + a * 3 + b);
+          }
+        }
+      }
+    for (int q = 0; q < nq; q++)
+      for (int a = 0; a < 4; a++) {
+        const int i = quad[4 * q + a];
+        grad[i].push_back(-(q * 4 + a) - 1);
+        for (int b = 0; b < 4; b++) {
+          if (a == b) {
+            dia[i].push_back(-(q * 16 + a * 4 + b) - 1);
+          } else {
+            const int k = find_col(i, quad[4 * q + b]);
+            if (k >= 0) crs[k].push_back(-(q * 16 + a * 4 + b) - 1);
+          }
+        }
+      }
+    auto flatten = [](std::vector<std::vector<int>>& v, std::vector<int>& ptr,
+                      std::vector<int>& out) {
+      ptr.assign(v.size() + 1, 0);
+      for (size_t i = 0; i < v.size(); i++) ptr[i + 1] = ptr[i] + (int)v[i].size();
+      out.resize(ptr.back());
+      for (size_t i = 0; i < v.size(); i++)
+        std::copy(v[i].begin(), v[i].end(), out.begin() + ptr[i]);
+    };
+    flatten(crs, s.crs_src_ptr, s.crs_src);
+    flatten(dia, s.dia_src_ptr, s.dia_src);
+    flatten(grad, s.grad_src_ptr, s.grad_src);
+  }
+  neighbours(tri, nt, 3, np, s.ring_ptr, s.ring);
+  // BVH topology
+  {
+    s.node.clear();
+    s.node.insert(s.node.end(), {-1, 0, 0});
+    BvhBuilder B(s.node);
+    B.tri = tri;
+    B.xyz = xyz_init;
+    triangle_adjacency(tri, nt, np, B.adj);
+    B.owner.assign(nt, 0);
+    std::vector<int> all(nt);
+    for (int t = 0; t < nt; t++) all[t] = t;
+    if (!B.split(0, all)) return "BVH split failed (degenerate triangle cluster)";
+    s.nnode = (int)s.node.size() / 3;
+    s.root = 0;
+  }
+  // pre-order ranks, subtree extents, depths, leaf of each triangle
+  {
+    const int n = s.nnode;
+    s.pre.assign(n, 0);
+    s.pre_end.assign(n, 0);
+    s.by_pre.assign(n, 0);
+    s.depth.assign(n, 0);
+    s.leaf_of.assign(nt, -1);
+    std::vector<int> st;
+    st.push_back(s.root);
+    int r = 0;
+    while (!st.empty()) {
+      const int v = st.back();
+      st.pop_back();
+      s.pre[v] = r;
+      s.by_pre[r] = v;
+      r++;
+      const int c0 = s.node[3 * v + 1], c1 = s.node[3 * v + 2];
+      if (c1 == -1) {
+        s.leaf_of[c0] = v;
+      } else {
+        s.depth[c0] = s.depth[c1] = s.depth[v] + 1;
+        st.push_back(c1);
+        st.push_back(c0);
+      }
+    }
+    // subtree end: pre + size, sizes accumulated children-first (reverse pre-order)
+    std::vector<int> size(n, 1);
+    for (int k = n - 1; k >= 0; k--) {
+      const int v = s.by_pre[k];
+      if (s.node[3 * v + 2] != -1) size[v] = 1 + size[s.node[3 * v + 1]] + size[s.node[3 * v + 2]];
+    }
+    for (int v = 0; v < n; v++) s.pre_end[v] = s.pre[v] + size[v];
+    for (int t = 0; t < nt; t++)
+      if (s.leaf_of[t] < 0) return "triangle without a BVH leaf";
+    s.max_depth = 0;
+    for (int v = 0; v < n; v++) s.max_depth = std::max(s.max_depth, s.depth[v]);
+    // bottom-up refit order: deepest level first
+    s.refit_lvl_ptr.assign(s.max_depth + 2, 0);
+    for (int v = 0; v < n; v++) s.refit_lvl_ptr[s.max_depth - s.depth[v] + 1]++;
+    for (int l = 0; l <= s.max_depth; l++) s.refit_lvl_ptr[l + 1] += s.refit_lvl_ptr[l];
+    s.refit_order.resize(n);
+    std::vector<int> fill(s.refit_lvl_ptr.begin(), s.refit_lvl_ptr.end() - 1);
+    for (int v = 0; v < n; v++) s.refit_order[fill[s.max_depth - s.depth[v]]++] = v;
+  }
+  return "";
+}
+
+}  // namespace cs

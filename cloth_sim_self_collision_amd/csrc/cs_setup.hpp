@@ -1,0 +1,51 @@
+// Host-side, per-mesh setup of the cloth step: everything that depends only
+// on the mesh connectivity (and, for the BVH, on the initial positions) and is
+// built once before the device step loop runs.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cs {
+
+// source encodings of the gather lists
+//   Hessian block sources: >= 0 -> triangle block t*9 + a*3 + b
+//                          <  0 -> bending block -(q*16 + a*4 + b) - 1
+//   gradient sources:      >= 0 -> triangle corner t*3 + a
+//                          <  0 -> bending corner -(q*4 + a) - 1
+struct MeshSetup {
+  int np = 0, nt = 0, nq = 0, nnz = 0;
+  // 3x3-block CSR pattern without the diagonal: union of the bending stencils
+  // (self_contact_sparse/main.cpp:577-581)
+  std::vector<int> rowptr, colind, diaind, rowof;
+  // level schedules: forward substitution / ILU rows (dependencies through the
+  // strictly lower part) and backward substitution (strictly upper part)
+  std::vector<int> lf_ptr, lf_rows, lb_ptr, lb_rows;
+  int nlf = 0, nlb = 0, max_lf = 0, max_lb = 0;
+  // gather lists in element order (assembly without atomics, in the
+  // reference's summation order)
+  std::vector<int> crs_src_ptr, crs_src;
+  std::vector<int> dia_src_ptr, dia_src;
+  std::vector<int> grad_src_ptr, grad_src;
+  // vertex 1-ring through triangles (rigid impact zones)
+  std::vector<int> ring_ptr, ring;
+  // BVH: node i = (parent, child0, child1); child1 == -1 marks a leaf whose
+  // child0 is its triangle.  Node numbering is the reference's creation order.
+  int nnode = 0, root = 0;
+  std::vector<int> node;
+  std::vector<int> pre;       // pre-order rank of each node
+  std::vector<int> pre_end;   // one past the last pre-order rank of the subtree
+  std::vector<int> by_pre;    // node id at each pre-order rank
+  std::vector<int> leaf_of;   // leaf node of each triangle
+  std::vector<int> depth;     // depth of each node (root 0)
+  std::vector<int> refit_order;  // nodes ordered by decreasing depth (bottom-up refit)
+  std::vector<int> refit_lvl_ptr;  // level boundaries inside refit_order
+  int max_depth = 0;
+};
+
+// Returns an empty string on success, else a description of the problem.
+std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,
+                        const double* xyz_init);
+
+}  // namespace cs

@@ -1,0 +1,613 @@
+// Implicit internal solve of the cloth step on the device:
+//   element kernels -> gather assembly of the backward-Euler system
+//   -> block ILU(0) -> ILU-preconditioned CG
+// (solve_internal_sparse.h:255-326, ilu_sparse.cpp, matrix_square_sparse.cpp).
+//
+// Bit-exactness: every value is produced by one thread in the reference's
+// operation order.  Assembly gathers each block's contributions in element
+// order (no atomics); the ILU factor and the triangular sweeps run one row
+// per thread over a precomputed level schedule; the PCG reductions are summed
+// left to right exactly as the reference's InnerProduct.
+#include <hip/hip_runtime.h>
+
+#include "cs_device.hpp"
+#include "cs_math.hpp"
+#include "cs_solve.hpp"
+
+namespace cs {
+
+namespace {
+constexpr int kTB = 256;
+inline unsigned grid_for(long n, int tb = kTB) { return (unsigned)((n + tb - 1) / tb); }
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// element stage
+
+// constant bending Hessian scalars w_a w_b scale per stencil
+__global__ void k_bend_setup(DevCloth c, double stiff) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= c.nq) return;
+  double X[12];
+  for (int a = 0; a < 4; a++) {
+    const int v = c.quad[4 * q + a];
+    for (int k = 0; k < 3; k++) X[a * 3 + k] = c.xyz0[3 * v + k];
+  }
+  double w[4], s;
+  bend_weights(X, stiff, w, &s);
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) c.kq[q * 16 + a * 4 + b] = w[a] * w[b] * s;
+}
+
+// membrane element: 81 Hessian values (block (a,b) at (a*3+b)*9) + 9 gradient values
+__global__ void k_tri_elem(DevCloth c, double lam, double mu, int with_hess) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.nt) return;
+  double X[9], x[9];
+  for (int a = 0; a < 3; a++) {
+    const int v = c.tri[3 * t + a];
+    for (int k = 0; k < 3; k++) {
+      X[a * 3 + k] = c.xyz0[3 * v + k];
+      x[a * 3 + k] = c.x[3 * v + k];
+    }
+  }
+  CstState s;
+  cst_state(X, x, lam, mu, s);
+  double* o = c.etri + (size_t)t * 90;
+  if (with_hess) {
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) cst_hess_block(s, a, b, o + (a * 3 + b) * 9);
+  }
+  for (int a = 0; a < 3; a++)
+    for (int i = 0; i < 3; i++) o[81 + a * 3 + i] = cst_grad(s, a, i);
+}
+
+__global__ void k_quad_grad(DevCloth c) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= c.nq) return;
+  double x[12];
+  for (int a = 0; a < 4; a++) {
+    const int v = c.quad[4 * q + a];
+    for (int k = 0; k < 3; k++) x[a * 3 + k] = c.x[3 * v + k];
+  }
+  const double* kab = c.kq + (size_t)q * 16;
+  for (int a = 0; a < 4; a++)
+    for (int i = 0; i < 3; i++) c.eq[(size_t)q * 12 + a * 3 + i] = bend_grad(kab, x, a, i);
+}
+
+// ---------------------------------------------------------------------------
+// gather assembly
+
+__device__ __forceinline__ void add_hess_src(const DevCloth& c, int src, double acc[9]) {
+  if (src >= 0) {
+    const double* e = c.etri + (size_t)(src / 9) * 90 + (src % 9) * 9;
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] += e[q];
+  } else {
+    // bending block: kab on the diagonal, exact zeros elsewhere
+    const double kv = c.kq[-src - 1];
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] += (q == 0 || q == 4 || q == 8) ? kv : 0.0;
+  }
+}
+
+// off-diagonal blocks; rows and columns of fixed vertices are zeroed
+// (matrix_square_sparse.cpp:272-302)
+__global__ void k_asm_crs(DevCloth c) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= c.nnz) return;
+  const int i = c.rowof[k], j = c.colind[k];
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) acc[q] = 0.0;
+  if (c.bc[i] == 0 && c.bc[j] == 0) {
+    for (int s = c.crs_src_ptr[k]; s < c.crs_src_ptr[k + 1]; s++) add_hess_src(c, c.crs_src[s], acc);
+  }
+  double* o = c.crs + (size_t)k * 9;
+#pragma unroll
+  for (int q = 0; q < 9; q++) o[q] = acc[q];
+}
+
+// diagonal blocks + right-hand side (solve_internal_sparse.h:285-313)
+__global__ void k_asm_dia_rhs(DevCloth c, Params P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.np) return;
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) acc[q] = 0.0;
+  for (int s = c.dia_src_ptr[i]; s < c.dia_src_ptr[i + 1]; s++) add_hess_src(c, c.dia_src[s], acc);
+  double pd, n[3];
+  if (c.host_pd) {
+    pd = c.host_pd[i];
+    n[0] = c.host_n[3 * i];
+    n[1] = c.host_n[3 * i + 1];
+    n[2] = c.host_n[3 * i + 2];
+  } else {
+    obstacle_depth(P.obstacle_kind, P.obstacle, c.x + 3 * i, &pd, n);
+  }
+  double cg[3], ch[9];
+  contact_penalty(pd, n, P.k_contact, P.clearance, cg, ch);
+#pragma unroll
+  for (int q = 0; q < 9; q++) acc[q] += ch[q];
+  const double md = P.mass / (P.dt * P.dt);
+  acc[0] += md;
+  acc[4] += md;
+  acc[8] += md;
+  const bool fixed = c.bc[i] != 0;
+  double* o = c.dia + (size_t)i * 9;
+#pragma unroll
+  for (int q = 0; q < 9; q++) o[q] = fixed ? ((q == 0 || q == 4 || q == 8) ? 1.0 : 0.0) : acc[q];
+  double g[3] = {0.0, 0.0, 0.0};
+  for (int s = c.grad_src_ptr[i]; s < c.grad_src_ptr[i + 1]; s++) {
+    const int src = c.grad_src[s];
+    const double* e = (src >= 0) ? c.etri + (size_t)(src / 3) * 90 + 81 + (src % 3) * 3
+                                 : c.eq + (size_t)(-src - 1) * 3;
+    g[0] += e[0];
+    g[1] += e[1];
+    g[2] += e[2];
+  }
+  for (int k = 0; k < 3; k++) g[k] += cg[k];
+  for (int k = 0; k < 3; k++) g[k] -= P.mass * P.grav[k];
+  for (int k = 0; k < 3; k++)
+    c.b[3 * i + k] = fixed ? 0.0 : -g[k] + P.mass * c.v[3 * i + k] / P.dt;
+}
+
+// ---------------------------------------------------------------------------
+// block ILU(0), IKJ, one row per thread over the lower-part level schedule
+// (ilu_sparse.cpp:387-401, 507-569).  Rows after `stop` keep the matrix
+// values, row `stop` keeps its unscaled upper part: the reference returns
+// right after its 11th singular pivot.
+
+__device__ __forceinline__ int find_in_row(const int* colind, int b, int e, int j) {
+  while (b < e) {
+    const int m = (b + e) >> 1;
+    const int v = colind[m];
+    if (v == j) return m;
+    if (v < j)
+      b = m + 1;
+    else
+      e = m;
+  }
+  return -1;
+}
+
+__device__ void ilu_row(const DevCloth& c, int i, int stop, int* ising) {
+  const int rb = c.rowptr[i], re = c.rowptr[i + 1], di = c.diaind[i];
+  double* F = c.fcrs;
+  for (int ik = rb; ik < di; ik++) {
+    const int kr = c.colind[ik];
+    const double* l = F + (size_t)ik * 9;
+    for (int kj = c.diaind[kr]; kj < c.rowptr[kr + 1]; kj++) {
+      const int j = c.colind[kj];
+      const double* u = F + (size_t)kj * 9;
+      double* a;
+      if (j != i) {
+        const int pos = find_in_row(c.colind, rb, re, j);
+        if (pos < 0) continue;
+        a = F + (size_t)pos * 9;
+      } else {
+        a = c.fdia + (size_t)i * 9;
+      }
+      for (int r = 0; r < 3; r++) {
+        const double l0 = l[r * 3 + 0], l1 = l[r * 3 + 1], l2 = l[r * 3 + 2];
+        a[r * 3 + 0] -= l0 * u[0] + l1 * u[3] + l2 * u[6];
+        a[r * 3 + 1] -= l0 * u[1] + l1 * u[4] + l2 * u[7];
+        a[r * 3 + 2] -= l0 * u[2] + l1 * u[5] + l2 * u[8];
+      }
+    }
+  }
+  double* d = c.fdia + (size_t)i * 9;
+  const double det = +d[0] * d[4] * d[8] + d[3] * d[7] * d[2] + d[6] * d[1] * d[5] -
+                     d[0] * d[7] * d[5] - d[6] * d[4] * d[2] - d[3] * d[1] * d[8];
+  if (fabs(det) > 1.0e-30) {
+    const double id = 1.0 / det;
+    double t[9];
+    for (int q = 0; q < 9; q++) t[q] = d[q];
+    d[0] = id * (t[4] * t[8] - t[5] * t[7]);
+    d[1] = id * (t[2] * t[7] - t[1] * t[8]);
+    d[2] = id * (t[1] * t[5] - t[2] * t[4]);
+    d[3] = id * (t[5] * t[6] - t[3] * t[8]);
+    d[4] = id * (t[0] * t[8] - t[2] * t[6]);
+    d[5] = id * (t[2] * t[3] - t[0] * t[5]);
+    d[6] = id * (t[3] * t[7] - t[4] * t[6]);
+    d[7] = id * (t[1] * t[6] - t[0] * t[7]);
+    d[8] = id * (t[0] * t[4] - t[1] * t[3]);
+  } else if (ising) {
+    ising[i] = 1;
+    atomicAdd(&c.flags[kNsing], 1);
+  }
+  if (i == stop) return;
+  for (int ij = di; ij < re; ij++) {
+    double* u = F + (size_t)ij * 9;
+    double t[9];
+    for (int q = 0; q < 9; q++) t[q] = u[q];
+    for (int r = 0; r < 3; r++) {
+      u[r * 3 + 0] = d[r * 3 + 0] * t[0] + d[r * 3 + 1] * t[3] + d[r * 3 + 2] * t[6];
+      u[r * 3 + 1] = d[r * 3 + 0] * t[1] + d[r * 3 + 1] * t[4] + d[r * 3 + 2] * t[7];
+      u[r * 3 + 2] = d[r * 3 + 0] * t[2] + d[r * 3 + 1] * t[5] + d[r * 3 + 2] * t[8];
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_ilu_factor(DevCloth c, int* ising) {
+  for (int l = 0; l < c.nlf; l++) {
+    const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
+    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x) ilu_row(c, c.lf_rows[t], c.np, ising);
+    __syncthreads();
+  }
+}
+
+// re-run of the factorisation when more than 10 pivots were singular: stop at
+// the 11th singular row in row order, leave later rows untouched
+__global__ __launch_bounds__(1024) void k_ilu_redo(DevCloth c, int* ising) {
+  if (c.flags[kNsing] <= 10) return;
+  __shared__ int stop;
+  if (threadIdx.x == 0) {
+    int cnt = 0, s = c.np;
+    for (int i = 0; i < c.np; i++)
+      if (ising[i] && ++cnt == 11) {
+        s = i;
+        break;
+      }
+    stop = s;
+    c.flags[kFirstSing11] = s;
+  }
+  __syncthreads();
+  const int s = stop;
+  for (long q = threadIdx.x; q < 9L * c.np; q += blockDim.x) c.fdia[q] = c.dia[q];
+  for (long q = threadIdx.x; q < 9L * c.nnz; q += blockDim.x) c.fcrs[q] = c.crs[q];
+  __syncthreads();
+  for (int l = 0; l < c.nlf; l++) {
+    const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
+    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x) {
+      const int i = c.lf_rows[t];
+      if (i <= s) ilu_row(c, i, s, nullptr);
+    }
+    __syncthreads();
+  }
+}
+
+// forward then backward substitution, dst = (LU)^-1 src; optionally
+// prod = src .* dst for the next reduction (ilu_sparse.cpp:185-215, 295-324)
+__global__ __launch_bounds__(1024) void k_ilu_apply(DevCloth c, const double* src, double* dst,
+                                                    double* prod) {
+  if (c.flags[kDone]) return;
+  for (int l = 0; l < c.nlf; l++) {
+    const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
+    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x) {
+      const int i = c.lf_rows[t];
+      double t0 = src[3 * i], t1 = src[3 * i + 1], t2 = src[3 * i + 2];
+      for (int k = c.rowptr[i]; k < c.diaind[i]; k++) {
+        const int j = c.colind[k];
+        const double* a = c.fcrs + (size_t)k * 9;
+        const double y0 = dst[3 * j], y1 = dst[3 * j + 1], y2 = dst[3 * j + 2];
+        t0 -= a[0] * y0 + a[1] * y1 + a[2] * y2;
+        t1 -= a[3] * y0 + a[4] * y1 + a[5] * y2;
+        t2 -= a[6] * y0 + a[7] * y1 + a[8] * y2;
+      }
+      const double* d = c.fdia + (size_t)i * 9;
+      dst[3 * i] = d[0] * t0 + d[1] * t1 + d[2] * t2;
+      dst[3 * i + 1] = d[3] * t0 + d[4] * t1 + d[5] * t2;
+      dst[3 * i + 2] = d[6] * t0 + d[7] * t1 + d[8] * t2;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < c.nlb; l++) {
+    const int b = c.lb_ptr[l], e = c.lb_ptr[l + 1];
+    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x) {
+      const int i = c.lb_rows[t];
+      double t0 = dst[3 * i], t1 = dst[3 * i + 1], t2 = dst[3 * i + 2];
+      for (int k = c.diaind[i]; k < c.rowptr[i + 1]; k++) {
+        const int j = c.colind[k];
+        const double* a = c.fcrs + (size_t)k * 9;
+        const double y0 = dst[3 * j], y1 = dst[3 * j + 1], y2 = dst[3 * j + 2];
+        t0 -= a[0] * y0 + a[1] * y1 + a[2] * y2;
+        t1 -= a[3] * y0 + a[4] * y1 + a[5] * y2;
+        t2 -= a[6] * y0 + a[7] * y1 + a[8] * y2;
+      }
+      dst[3 * i] = t0;
+      dst[3 * i + 1] = t1;
+      dst[3 * i + 2] = t2;
+    }
+    __syncthreads();
+  }
+  if (prod)
+    for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src[e] * dst[e];
+}
+
+// y = alpha A x + beta y over the block-CSR + diagonal (matrix_square_sparse.cpp:218-242)
+__global__ void k_matvec(int np, const int* rowptr, const int* colind, const double* dia,
+                         const double* crs, double alpha, const double* x, double beta,
+                         double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double y0 = y[3 * i] * beta, y1 = y[3 * i + 1] * beta, y2 = y[3 * i + 2] * beta;
+  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+    const int j = colind[k];
+    const double* a = crs + (size_t)k * 9;
+    const double x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+    y0 += alpha * (a[0] * x0 + a[1] * x1 + a[2] * x2);
+    y1 += alpha * (a[3] * x0 + a[4] * x1 + a[5] * x2);
+    y2 += alpha * (a[6] * x0 + a[7] * x1 + a[8] * x2);
+  }
+  const double* a = dia + (size_t)i * 9;
+  const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+  y0 += alpha * (a[0] * x0 + a[1] * x1 + a[2] * x2);
+  y1 += alpha * (a[3] * x0 + a[4] * x1 + a[5] * x2);
+  y2 += alpha * (a[6] * x0 + a[7] * x1 + a[8] * x2);
+  y[3 * i] = y0;
+  y[3 * i + 1] = y1;
+  y[3 * i + 2] = y2;
+}
+
+// PCG search direction and product in one pass:
+//   p_new = z (first iteration) or z + beta p_old   (ilu_sparse.cpp:663, 702)
+//   ap    = 0*z + 1.0*(A p_new)  -- the reference writes A p into the buffer
+//           that holds z (ilu_sparse.cpp:670-672), scaling it by beta = 0
+//   prod  = p_new .* ap
+__global__ void k_spmv_dir(DevCloth c, int first, const double* __restrict__ pold,
+                           double* __restrict__ pnew) {
+  if (c.flags[kDone]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.np) return;
+  const double beta = c.scal[kBeta];
+  const double* z = c.z;
+  double y0 = z[3 * i] * 0.0, y1 = z[3 * i + 1] * 0.0, y2 = z[3 * i + 2] * 0.0;
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; k++) {
+    const int j = c.colind[k];
+    const double* a = c.crs + (size_t)k * 9;
+    double x0, x1, x2;
+    if (first) {
+      x0 = z[3 * j];
+      x1 = z[3 * j + 1];
+      x2 = z[3 * j + 2];
+    } else {
+      x0 = z[3 * j] + beta * pold[3 * j];
+      x1 = z[3 * j + 1] + beta * pold[3 * j + 1];
+      x2 = z[3 * j + 2] + beta * pold[3 * j + 2];
+    }
+    y0 += 1.0 * (a[0] * x0 + a[1] * x1 + a[2] * x2);
+    y1 += 1.0 * (a[3] * x0 + a[4] * x1 + a[5] * x2);
+    y2 += 1.0 * (a[6] * x0 + a[7] * x1 + a[8] * x2);
+  }
+  double x0, x1, x2;
+  if (first) {
+    x0 = z[3 * i];
+    x1 = z[3 * i + 1];
+    x2 = z[3 * i + 2];
+  } else {
+    x0 = z[3 * i] + beta * pold[3 * i];
+    x1 = z[3 * i + 1] + beta * pold[3 * i + 1];
+    x2 = z[3 * i + 2] + beta * pold[3 * i + 2];
+  }
+  const double* a = c.dia + (size_t)i * 9;
+  y0 += 1.0 * (a[0] * x0 + a[1] * x1 + a[2] * x2);
+  y1 += 1.0 * (a[3] * x0 + a[4] * x1 + a[5] * x2);
+  y2 += 1.0 * (a[6] * x0 + a[7] * x1 + a[8] * x2);
+  c.ap[3 * i] = y0;
+  c.ap[3 * i + 1] = y1;
+  c.ap[3 * i + 2] = y2;
+  pnew[3 * i] = x0;
+  pnew[3 * i + 1] = x1;
+  pnew[3 * i + 2] = x2;
+  c.prod[3 * i] = x0 * y0;
+  c.prod[3 * i + 1] = x1 * y1;
+  c.prod[3 * i + 2] = x2 * y2;
+}
+
+// r -= alpha A p; x += alpha p; prod = r .* r  (matrix_square_sparse.cpp:327-336)
+__global__ void k_update_rx(DevCloth c, const double* __restrict__ p) {
+  if (c.flags[kDone]) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * c.np) return;
+  const double alpha = c.scal[kAlpha];
+  const double r = c.r[e] + -alpha * c.ap[e];
+  c.r[e] = r;
+  c.dx[e] = c.dx[e] + alpha * p[e];
+  c.prod[e] = r * r;
+}
+
+__global__ void k_pcg_init(DevCloth c, double tol2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    for (int q = 0; q < kNumFlag; q++) c.flags[q] = 0;
+    c.flags[kFirstSing11] = c.np;
+    c.scal[kTol2] = tol2;
+    c.scal[kBeta] = 0.0;
+  }
+  if (e >= 3 * c.np) return;
+  const double b = c.b[e];
+  c.r[e] = b;
+  c.dx[e] = 0;
+  c.prod[e] = b * b;
+}
+
+// ---------------------------------------------------------------------------
+// reductions: the reference sums left to right (matrix_square_sparse.cpp:313-323).
+// The workgroup stages the products through LDS; thread 0 accumulates them in
+// index order, so the result is bit-identical to the sequential loop.
+
+__device__ double ordered_sum(const double* t, int n) {
+  __shared__ double buf[4096];
+  double s = 0.0;
+  for (int base = 0; base < n; base += 4096) {
+    const int m = min(4096, n - base);
+    for (int k = threadIdx.x; k < m; k += blockDim.x) buf[k] = t[base + k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int k = 0;
+      for (; k + 8 <= m; k += 8) {
+        const double a0 = buf[k], a1 = buf[k + 1], a2 = buf[k + 2], a3 = buf[k + 3];
+        const double a4 = buf[k + 4], a5 = buf[k + 5], a6 = buf[k + 6], a7 = buf[k + 7];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
+        s += a4;
+        s += a5;
+        s += a6;
+        s += a7;
+      }
+      for (; k < m; k++) s += buf[k];
+    }
+    __syncthreads();
+  }
+  return s;
+}
+
+// what the PCG does with a finished reduction (ilu_sparse.cpp:648-708)
+enum DotMode : int { kDotRr0 = 0, kDotRz0 = 1, kDotPap = 2, kDotRr = 3, kDotRz = 4, kDotFinal = 5 };
+
+__global__ __launch_bounds__(256) void k_dot(DevCloth c, int mode, int iter) {
+  if (mode != kDotRr0 && c.flags[kDone]) return;
+  const double s = ordered_sum(c.prod, 3 * c.np);
+  if (threadIdx.x != 0) return;
+  c.scal[kLastDot] = s;
+  switch (mode) {
+    case kDotRr0:
+      if (s < 1.0e-30) {
+        c.scal[kConv] = 0.0;
+        c.flags[kIters] = 0;
+        c.flags[kDone] = 1;
+      } else {
+        c.scal[kInvRr0] = 1.0 / s;
+      }
+      break;
+    case kDotRz0:
+      c.scal[kRz] = s;
+      break;
+    case kDotPap:
+      c.scal[kAlpha] = c.scal[kRz] / s;
+      break;
+    case kDotRr: {
+      const double q = s * c.scal[kInvRr0];
+      if (q < c.scal[kTol2]) {
+        c.scal[kConv] = sqrt(q);
+        c.flags[kIters] = iter;
+        c.flags[kDone] = 1;
+      }
+      break;
+    }
+    case kDotRz: {
+      c.scal[kBeta] = s / c.scal[kRz];
+      c.scal[kRz] = s;
+      break;
+    }
+    case kDotFinal:
+      c.scal[kConv] = sqrt(s * c.scal[kInvRr0]);
+      c.flags[kIters] = iter;
+      break;
+  }
+}
+
+__global__ void k_square(DevCloth c, const double* v) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * c.np) return;
+  c.prod[e] = v[e] * v[e];
+}
+
+// x += dx, v = dx / dt (solve_internal_sparse.h:323-325)
+__global__ void k_finish_internal(DevCloth c, double dt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * c.np) return;
+  const double d = c.dx[e];
+  c.x[e] = c.x[e] + d;
+  c.v[e] = d / dt;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+
+int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s) {
+  if (c.nq > 0) hipLaunchKernelGGL(k_bend_setup, dim3(grid_for(c.nq)), dim3(kTB), 0, s, c, P.stiff_bend);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s) {
+  hipLaunchKernelGGL(k_tri_elem, dim3(grid_for(c.nt)), dim3(kTB), 0, s, c, P.lam, P.mu, 1);
+  if (c.nq > 0) hipLaunchKernelGGL(k_quad_grad, dim3(grid_for(c.nq)), dim3(kTB), 0, s, c);
+  if (c.nnz > 0) hipLaunchKernelGGL(k_asm_crs, dim3(grid_for(c.nnz)), dim3(kTB), 0, s, c);
+  hipLaunchKernelGGL(k_asm_dia_rhs, dim3(grid_for(c.np)), dim3(kTB), 0, s, c, P);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_ilu(const DevCloth& c, int* ising, hipStream_t s) {
+  CS_HIP(hipMemcpyAsync(c.fdia, c.dia, sizeof(double) * 9 * c.np, hipMemcpyDeviceToDevice, s));
+  if (c.nnz > 0)
+    CS_HIP(hipMemcpyAsync(c.fcrs, c.crs, sizeof(double) * 9 * (size_t)c.nnz,
+                          hipMemcpyDeviceToDevice, s));
+  CS_HIP(hipMemsetAsync(ising, 0, sizeof(int) * c.np, s));
+  hipLaunchKernelGGL(k_ilu_factor, dim3(1), dim3(1024), 0, s, c, ising);
+  hipLaunchKernelGGL(k_ilu_redo, dim3(1), dim3(1024), 0, s, c, ising);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_matvec(const DevCloth& c, double alpha, const double* x, double beta, double* y,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_matvec, dim3(grid_for(c.np)), dim3(kTB), 0, s, c.np, c.rowptr, c.colind,
+                     c.dia, c.crs, alpha, x, beta, y);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, src, dst, (double*)nullptr);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+// Solve_PCG on the assembled system (c.b) with the factor in fdia/fcrs.
+// Result in c.dx; conv/iters in the device scalars.  `host_flags` is pinned
+// memory used to poll for convergence every `poll` iterations.
+int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream_t s, int poll) {
+  const long n3 = 3L * c.np;
+  hipLaunchKernelGGL(k_pcg_init, dim3(grid_for(n3)), dim3(kTB), 0, s, c, tol * tol);
+  hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr0, 0);
+  hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
+  hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz0, 0);
+  CS_HIP(hipGetLastError());
+  bool done = false;
+  int it = 0;
+  for (; it < maxit && !done; it++) {
+    double* pnew = c.p[it & 1];
+    const double* pold = c.p[(it + 1) & 1];
+    hipLaunchKernelGGL(k_spmv_dir, dim3(grid_for(c.np)), dim3(kTB), 0, s, c, it == 0 ? 1 : 0,
+                       pold, pnew);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotPap, it);
+    hipLaunchKernelGGL(k_update_rx, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)pnew);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr, it);
+    hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz, it);
+    CS_HIP(hipGetLastError());
+    if ((it + 1) % poll == 0 || it + 1 == maxit) {
+      CS_HIP(hipMemcpyAsync(host_flags, c.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
+      CS_HIP(hipStreamSynchronize(s));
+      done = host_flags[kDone] != 0;
+    }
+  }
+  if (!done) {
+    // cap reached: report the achieved ratio (ilu_sparse.cpp:706-707)
+    hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)c.r);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotFinal, maxit);
+    CS_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+
+int launch_gradients(const DevCloth& c, const Params& P, hipStream_t s) {
+  hipLaunchKernelGGL(k_tri_elem, dim3(grid_for(c.nt)), dim3(kTB), 0, s, c, P.lam, P.mu, 0);
+  if (c.nq > 0) hipLaunchKernelGGL(k_quad_grad, dim3(grid_for(c.nq)), dim3(kTB), 0, s, c);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_finish_internal(const DevCloth& c, const Params& P, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish_internal, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, c, P.dt);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace cs

@@ -1,0 +1,281 @@
+"""Python front-end of one device-resident cloth.
+
+``DeviceCloth`` mirrors the reference's step loop (StepTime,
+self_contact_sparse/main.cpp:213-263) and the stage functions it calls:
+
+=============================================  ===================================
+reference                                      DeviceCloth
+=============================================  ===================================
+StepTime_InternalDynamicsILU                   step_internal_dynamics_ilu()
+GetIntermidiateVelocityContactResolved         intermediate_velocity_contact_resolved()
+x = x1 + v dt; UpdateIntermidiateVelocity      update_intermediate_velocity()
+StepTime                                       step()
+CMatrixSquareSparse::MatVec                    matvec()
+CPreconditionerILU::SetValueILU+DoILUDecomp    ilu_factor()
+CPreconditionerILU::Solve                      ilu_solve()
+Solve_PCG                                      pcg_solve()
+BuildBoundingBoxChild_Prx / _CCD               bvh_refit_prx() / bvh_refit_ccd()
+GetContactElement_Proximity / _CCD             contacts_proximity() / contacts_ccd()
+=============================================  ===================================
+
+All compute runs in libclothsim.so on the GPU; this module only moves arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dp, f64, i32, ip
+from .scene import Cloth, apply_sweep
+
+
+def make_params(cloth: Cloth) -> _lib.Params:
+    m = cloth.material
+    p = _lib.Params()
+    p.lam = m.lam
+    p.myu = m.mu
+    p.stiff_bend = m.stiff_bend
+    for k in range(3):
+        p.gravity[k] = m.gravity[k]
+    p.mass_point = cloth.mass_point
+    p.stiff_contact = m.stiff_contact
+    p.contact_clearance = cloth.clearance
+    p.dt = cloth.dt
+    p.pcg_tol = cloth.pcg_tol
+    p.pcg_maxit = cloth.pcg_maxit
+    p.obstacle_kind = cloth.obstacle.kind
+    for k, v in enumerate(cloth.obstacle.params[:4]):
+        p.obstacle[k] = v
+    return p
+
+
+class DeviceCloth:
+    """One cloth resident on the GPU."""
+
+    def __init__(self, cloth: Cloth):
+        lib = _lib.load()
+        self.cloth = cloth
+        self._arrays = [f64(cloth.xyz0), f64(cloth.xyz), f64(cloth.uvw), i32(cloth.tri),
+                        i32(cloth.quad), i32(cloth.bc)]
+        a = self._arrays
+        mesh = _lib.Mesh(cloth.np_, cloth.nt, cloth.nq, dp(a[0]), dp(a[1]), dp(a[2]), ip(a[3]),
+                         ip(a[4]), ip(a[5]))
+        self._params = make_params(cloth)
+        h = C.c_void_p()
+        check(lib.cs_cloth_create(C.byref(mesh), C.byref(self._params), C.byref(h)))
+        self.h = h
+        sz = [C.c_int() for _ in range(5)]
+        check(lib.cs_cloth_sizes(self.h, *[C.byref(s) for s in sz]))
+        self.np_, self.nt, self.nq, self.nnz, self.nnode = [s.value for s in sz]
+        self.time = 0.0
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().cs_cloth_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state
+    def set_state(self, xyz=None, uvw=None):
+        x = f64(xyz) if xyz is not None else None
+        v = f64(uvw) if uvw is not None else None
+        check(_lib.load().cs_set_state(self.h, dp(x) if x is not None else None,
+                                       dp(v) if v is not None else None))
+
+    def get_state(self):
+        x = np.zeros(3 * self.np_)
+        v = np.zeros(3 * self.np_)
+        check(_lib.load().cs_get_state(self.h, dp(x), dp(v)))
+        return x, v
+
+    def set_step_start(self, xyz1):
+        check(_lib.load().cs_set_step_start(self.h, dp(f64(xyz1))))
+
+    def set_params(self, cloth: Optional[Cloth] = None):
+        self._params = make_params(cloth or self.cloth)
+        check(_lib.load().cs_cloth_set_params(self.h, C.byref(self._params)))
+
+    def set_obstacle_values(self, pd=None, n=None):
+        if pd is None:
+            check(_lib.load().cs_set_obstacle_values(self.h, None, None))
+        else:
+            check(_lib.load().cs_set_obstacle_values(self.h, dp(f64(pd)), dp(f64(n))))
+
+    # -- the three stages of StepTime
+    def step_internal_dynamics_ilu(self):
+        conv = C.c_double()
+        it = C.c_int()
+        check(_lib.load().cs_step_internal(self.h, C.byref(conv), C.byref(it)))
+        return conv.value, it.value
+
+    def intermediate_velocity_contact_resolved(self):
+        applied = C.c_int()
+        st = _lib.StepStats()
+        check(_lib.load().cs_contact_resolve(self.h, C.byref(applied), C.byref(st)))
+        return bool(applied.value), st.as_dict()
+
+    def update_intermediate_velocity(self):
+        check(_lib.load().cs_post_contact_update(self.h))
+
+    def step(self, n: int = 1) -> dict:
+        """n StepTime iterations; applies the cloth's kinematic sweep (if any)
+        before each step.  Returns the stats of the last step."""
+        st = _lib.StepStats()
+        lib = _lib.load()
+        if self.cloth.sweep is None:
+            check(lib.cs_step(self.h, n, C.byref(st)))
+            self.time += n * self.cloth.dt
+            return st.as_dict()
+        for _ in range(n):
+            x, v = self.get_state()
+            apply_sweep(self.cloth, x, self.time + self.cloth.dt)
+            self.set_state(x, None)
+            check(lib.cs_step(self.h, 1, C.byref(st)))
+            self.time += self.cloth.dt
+        return st.as_dict()
+
+    # -- lower-level operations
+    def pattern(self):
+        rp = np.zeros(self.np_ + 1, np.int32)
+        ci = np.zeros(max(self.nnz, 1), np.int32)
+        di = np.zeros(self.np_, np.int32)
+        check(_lib.load().cs_get_pattern(self.h, ip(rp), ip(ci), ip(di)))
+        return rp, ci[: self.nnz], di
+
+    def assemble(self, xyz, uvw):
+        check(_lib.load().cs_assemble(self.h, dp(f64(xyz)), dp(f64(uvw))))
+        return self.get_system()
+
+    def get_system(self):
+        dia = np.zeros(9 * self.np_)
+        crs = np.zeros(9 * max(self.nnz, 1))
+        b = np.zeros(3 * self.np_)
+        check(_lib.load().cs_get_system(self.h, dp(dia), dp(crs), dp(b)))
+        return dia, crs[: 9 * self.nnz], b
+
+    def set_system(self, dia=None, crs=None, b=None):
+        args = [dp(f64(a)) if a is not None and len(a) else None for a in (dia, crs, b)]
+        check(_lib.load().cs_set_system(self.h, *args))
+
+    def ilu_factor(self):
+        ns = C.c_int()
+        check(_lib.load().cs_ilu_factor(self.h, C.byref(ns)))
+        return ns.value
+
+    def get_factor(self):
+        fd = np.zeros(9 * self.np_)
+        fc = np.zeros(9 * max(self.nnz, 1))
+        check(_lib.load().cs_get_factor(self.h, dp(fd), dp(fc)))
+        return fd, fc[: 9 * self.nnz]
+
+    def set_factor(self, fdia, fcrs):
+        check(_lib.load().cs_set_factor(self.h, dp(f64(fdia)),
+                                        dp(f64(fcrs)) if len(fcrs) else None))
+
+    def ilu_solve(self, vec):
+        v = np.array(vec, dtype=np.float64)
+        check(_lib.load().cs_ilu_solve(self.h, dp(v)))
+        return v
+
+    def matvec(self, alpha, x, beta, y):
+        yy = np.array(y, dtype=np.float64)
+        check(_lib.load().cs_matvec(self.h, alpha, dp(f64(x)), beta, dp(yy)))
+        return yy
+
+    def pcg_solve(self, b, tol, maxit):
+        x = np.zeros(3 * self.np_)
+        r = np.zeros(3 * self.np_)
+        conv = C.c_double()
+        it = C.c_int()
+        check(_lib.load().cs_pcg_solve(self.h, dp(f64(b)), tol, maxit, dp(x), dp(r),
+                                       C.byref(conv), C.byref(it)))
+        return x, r, conv.value, it.value
+
+    def bvh(self):
+        nodes = np.zeros(3 * self.nnode, np.int32)
+        check(_lib.load().cs_get_bvh(self.h, ip(nodes)))
+        return nodes.reshape(-1, 3)
+
+    def bvh_refit_prx(self, xyz, delta):
+        bb = np.zeros(6 * self.nnode)
+        check(_lib.load().cs_bvh_refit_prx(self.h, dp(f64(xyz)), delta, dp(bb)))
+        return bb.reshape(-1, 6)
+
+    def bvh_refit_ccd(self, xyz, uvw, dt):
+        bb = np.zeros(6 * self.nnode)
+        check(_lib.load().cs_bvh_refit_ccd(self.h, dp(f64(xyz)), dp(f64(uvw)), dt, dp(bb)))
+        return bb.reshape(-1, 6)
+
+    def _contacts(self, fn, *args):
+        cap = 1 << 16
+        while True:
+            out = np.zeros(5 * cap, np.int32)
+            n = C.c_int()
+            check(fn(self.h, *args, ip(out), cap, C.byref(n)))
+            if n.value <= cap:
+                return out[: 5 * n.value].reshape(-1, 5)
+            cap = n.value
+
+    def contacts_proximity(self, xyz, delta):
+        x = f64(xyz)
+        return self._contacts(_lib.load().cs_query_proximity, dp(x), delta)
+
+    def contacts_ccd(self, xyz, uvw, dt, delta):
+        x = f64(xyz)
+        v = f64(uvw)
+        return self._contacts(_lib.load().cs_query_ccd, dp(x), dp(v), dt, delta)
+
+
+def element_cst(C_, c_, lam, mu):
+    """WdWddW_CST on the device for arrays of triangles: (dW[n,9], ddW[n,81])."""
+    C_ = f64(C_).reshape(-1)
+    c_ = f64(c_).reshape(-1)
+    n = len(C_) // 9
+    dW = np.zeros(9 * n)
+    ddW = np.zeros(81 * n)
+    check(_lib.load().cs_element_cst(n, dp(C_), dp(c_), lam, mu, dp(dW), dp(ddW)))
+    return dW.reshape(n, 9), ddW.reshape(n, 81)
+
+
+def element_bend(C_, c_, stiff):
+    C_ = f64(C_).reshape(-1)
+    c_ = f64(c_).reshape(-1)
+    n = len(C_) // 12
+    dW = np.zeros(12 * n)
+    ddW = np.zeros(144 * n)
+    check(_lib.load().cs_element_bend(n, dp(C_), dp(c_), stiff, dp(dW), dp(ddW)))
+    return dW.reshape(n, 12), ddW.reshape(n, 144)
+
+
+def element_contact(p, stiff, clearance, obstacle):
+    p = f64(p).reshape(-1)
+    n = len(p) // 3
+    o = f64(list(obstacle.params[:4]) + [0.0] * (4 - len(obstacle.params[:4])))
+    dW = np.zeros(3 * n)
+    ddW = np.zeros(9 * n)
+    check(_lib.load().cs_element_contact(n, dp(p), stiff, clearance, obstacle.kind, dp(o),
+                                         dp(dW), dp(ddW)))
+    return dW.reshape(n, 3), ddW.reshape(n, 9)
+
+
+def selftest_divsqrt(a, b):
+    a = f64(a)
+    b = f64(b)
+    out = np.zeros(2 * len(a))
+    check(_lib.load().cs_selftest_divsqrt(len(a), dp(a), dp(b), dp(out)))
+    return out[0::2], out[1::2]

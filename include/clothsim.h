@@ -1,0 +1,178 @@
+/* clothsim: MI355X (gfx950) implementation of the implicit cloth step with
+ * self-contact of nobuyuki83/cloth_sim_self_collision, behind a C ABI.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Arrays follow the reference layouts:
+ *    positions/velocities are AoS double[3*np]; triangles int[3*nt];
+ *    bending stencils int[4*nq] laid out [opp0, opp1, e0, e1]; boundary flags
+ *    int[np] (0 free, 1 fixed); the matrix is 3x3-block CSR without the
+ *    diagonal plus a separate diagonal (CMatrixSquareSparse,
+ *    matrix_square_sparse.h:14-48) -- row offsets int[np+1], column indices
+ *    int[nnz] ascending per row, values double[9*nnz] and double[9*np],
+ *    blocks row-major.
+ *  - Every call returns a status (CS_OK = 0); cs_last_error() describes the
+ *    most recent failure of the calling thread.  No C++ exception crosses
+ *    the ABI.
+ *  - One HIP stream per cloth; calls on distinct cloths may run from
+ *    different threads, calls on one cloth must not overlap.
+ *  - Host-pointer entry points copy in/out and synchronise; *_device entry
+ *    points take device pointers and stay on the cloth's stream.
+ *  - FP64 throughout; results are bit-identical to the reference's x86-64
+ *    build on the same inputs (contact sets, matrices, factors, positions).
+ */
+#ifndef CLOTHSIM_H
+#define CLOTHSIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_OK 0
+#define CS_ERR_BAD_ARG 1
+#define CS_ERR_HIP 2
+#define CS_ERR_OOM 3
+#define CS_ERR_SINGULAR 4      /* informational: ILU met singular pivots */
+#define CS_ERR_NOT_CONVERGED 5 /* informational: PCG hit its iteration cap */
+#define CS_ERR_NONFINITE 6
+#define CS_ERR_CAPACITY 7
+#define CS_ERR_NO_DEVICE 8
+
+/* obstacle kinds: penetrationDepth(pd, n, p) with pd > 0 inside and n the
+ * push-out direction (cloth_internal_physics.h:251-255) */
+#define CS_OBST_NONE 0
+#define CS_OBST_PLANE 1      /* params (nx, ny, nz, d): pd = d - n.p        */
+#define CS_OBST_SPHERE 2     /* params (cx, cy, cz, r): pd = r - |p - c|    */
+#define CS_OBST_SPHERE_IN 3  /* sign-flipped sphere of self_contact_sparse/main.cpp:117-133 */
+
+typedef struct cs_cloth cs_cloth;
+
+typedef struct {
+  int np, nt, nq;
+  const double* xyz0;    /* rest positions, 3*np */
+  const double* xyz;     /* initial positions (also seed the BVH topology), 3*np */
+  const double* uvw;     /* initial velocities, 3*np (NULL = zero) */
+  const int* tri;        /* 3*nt */
+  const int* quad;       /* 4*nq */
+  const int* bc;         /* np, NULL = all free */
+} cs_mesh;
+
+typedef struct {
+  double lambda, myu;         /* Lame parameters */
+  double stiff_bend;
+  double gravity[3];
+  double mass_point;          /* per-vertex mass (area*density/np in the demos) */
+  double stiff_contact;       /* obstacle penalty and self-contact stiffness */
+  double contact_clearance;
+  double dt;
+  double pcg_tol;             /* the reference hard-codes 1e-4 ... */
+  int pcg_maxit;              /* ... and 100 (solve_internal_sparse.h:318-319) */
+  int obstacle_kind;
+  double obstacle[4];
+} cs_params;
+
+typedef struct {
+  double pcg_conv;        /* achieved residual ratio */
+  int pcg_iters;          /* as Solve_PCG reports it */
+  int ilu_singular;       /* pivots with |det| <= 1e-30 */
+  int impulse_applied;    /* the contact pass changed velocities */
+  int n_prox;             /* proximity contacts */
+  int n_ccd_pass;         /* CCD passes that found contacts (<= 5) */
+  int n_ccd[5];           /* contacts per CCD pass */
+  int n_riz_pass;         /* impact-zone passes run (<= 100) */
+  int n_riz_vertices;     /* vertices in impact zones at the end */
+} cs_step_stats;
+
+const char* cs_last_error(void);
+int cs_device_count(int* n);
+
+/* Create a cloth: uploads the mesh and builds the pattern (union of the
+ * bending stencils, self_contact_sparse/main.cpp:577-581), the ILU(0)
+ * diagonal index (ilu_sparse.cpp:114-134), level schedules, assembly gather
+ * lists, the BVH topology (MakeBVHTopology_TopDown, bvh_aabb.cpp:294-313) and
+ * the vertex 1-ring (main.cpp:575). */
+int cs_cloth_create(const cs_mesh* mesh, const cs_params* params, cs_cloth** out);
+int cs_cloth_destroy(cs_cloth* c);
+int cs_cloth_set_params(cs_cloth* c, const cs_params* params);
+int cs_cloth_sizes(const cs_cloth* c, int* np, int* nt, int* nq, int* nnz, int* nnode);
+
+/* state (host arrays of 3*np doubles) */
+int cs_set_state(cs_cloth* c, const double* xyz, const double* uvw);
+int cs_get_state(cs_cloth* c, double* xyz, double* uvw);
+/* device pointers of the resident state (valid for the cloth's lifetime) */
+int cs_state_device(cs_cloth* c, double** xyz, double** uvw);
+/* positions at the start of the step, read by cs_contact_resolve
+ * (cs_step_internal stores them itself) */
+int cs_set_step_start(cs_cloth* c, const double* xyz1);
+/* per-vertex obstacle values from a host penetrationDepth callback: pd[np],
+ * n[3*np]; NULL restores the analytic obstacle of the parameters */
+int cs_set_obstacle_values(cs_cloth* c, const double* pd, const double* n);
+
+/* ---- the three calls of StepTime (self_contact_sparse/main.cpp:213-263) ---- */
+
+/* StepTime_InternalDynamicsILU (solve_internal_sparse.h:255-326): assemble,
+ * factor, PCG with the cloth's tolerance/cap, x += dx, v = dx/dt.  Also
+ * stores the start-of-step positions used by the contact pass. */
+int cs_step_internal(cs_cloth* c, double* conv_ratio, int* iteration);
+/* GetIntermidiateVelocityContactResolved (self_collision_cloth.h:21-34) on the
+ * stored start-of-step positions and the current velocities. */
+int cs_contact_resolve(cs_cloth* c, int* impulse_applied, cs_step_stats* stats);
+/* x = x_start + v dt followed by UpdateIntermidiateVelocity
+ * (solve_internal_sparse.h:329-369); StepTime does this only when an impulse
+ * was applied. */
+int cs_post_contact_update(cs_cloth* c);
+/* n full StepTime iterations; stats of the last step (may be NULL) */
+int cs_step(cs_cloth* c, int n, cs_step_stats* stats);
+
+/* ---- lower-level entry points (host arrays unless noted) ---- */
+int cs_get_pattern(const cs_cloth* c, int* rowptr, int* colind, int* diaind);
+/* assemble A and b at the given state without solving */
+int cs_assemble(cs_cloth* c, const double* xyz, const double* uvw);
+int cs_get_system(cs_cloth* c, double* dia, double* crs, double* b);
+int cs_set_system(cs_cloth* c, const double* dia, const double* crs, const double* b);
+/* SetValueILU + DoILUDecomp on the current system (ilu_sparse.cpp:352-624) */
+int cs_ilu_factor(cs_cloth* c, int* n_singular);
+int cs_get_factor(cs_cloth* c, double* fdia, double* fcrs);
+int cs_set_factor(cs_cloth* c, const double* fdia, const double* fcrs);
+/* CPreconditionerILU::Solve (forward + backward substitution), in place */
+int cs_ilu_solve(cs_cloth* c, double* vec);
+/* CMatrixSquareSparse::MatVec: y = alpha A x + beta y */
+int cs_matvec(cs_cloth* c, double alpha, const double* x, double beta, double* y);
+/* Solve_PCG (ilu_sparse.cpp:628-709) on the current system and factor with
+ * right-hand side b; x out, r (residual) out; conv/iters like the reference */
+int cs_pcg_solve(cs_cloth* c, const double* b, double tol, int maxit, double* x, double* r,
+                 double* conv, int* iters);
+
+/* BVH: node array 3 ints per node (parent, child0, child1; child1 == -1 marks
+ * a leaf holding triangle child0), numbered like the reference */
+int cs_get_bvh(const cs_cloth* c, int* nodes);
+/* BuildBoundingBoxChild_Prx / _CCD (bvh_aabb.cpp:23-106); boxes out as 6
+ * doubles per node (xmin, xmax, ymin, ymax, zmin, zmax) when bb != NULL */
+int cs_bvh_refit_prx(cs_cloth* c, const double* xyz, double delta, double* bb);
+int cs_bvh_refit_ccd(cs_cloth* c, const double* xyz, const double* uvw, double dt, double* bb);
+/* GetContactElement_Proximity / _CCD (bvh_aabb.cpp:441-557, 797-923) as the
+ * std::set<CContactElement> contents in order: 5 ints per contact
+ * (is_fv, i0, i1, i2, i3).  *n receives the count; at most cap are written. */
+int cs_query_proximity(cs_cloth* c, const double* xyz, double delta, int* out, int cap, int* n);
+int cs_query_ccd(cs_cloth* c, const double* xyz, const double* uvw, double dt, double delta,
+                 int* out, int cap, int* n);
+
+/* ---- element routines over arrays (cloth_internal_physics.h) ---- */
+/* WdWddW_CST for n triangles: C, c 9 doubles each; W[n], dW[9n], ddW[81n] */
+int cs_element_cst(int n, const double* C, const double* c, double lambda, double myu,
+                   double* dW, double* ddW);
+/* WdWddW_Bend for n stencils: C, c 12 doubles each; dW[12n], ddW[144n] */
+int cs_element_bend(int n, const double* C, const double* c, double stiff, double* dW,
+                    double* ddW);
+/* WdWddW_Contact for n points against an obstacle: dW[3n], ddW[9n] */
+int cs_element_contact(int n, const double* p, double stiff, double clearance, int kind,
+                       const double* obstacle, double* dW, double* ddW);
+
+/* device arithmetic check: out[2i] = a[i]/b[i], out[2i+1] = sqrt(a[i]) */
+int cs_selftest_divsqrt(int n, const double* a, const double* b, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLOTHSIM_H */

@@ -1,0 +1,149 @@
+"""GPU parity: every stage of the HIP step against the CPU oracle, bit for bit.
+
+The oracle (oracle/cloth_oracle.c) is itself pinned to the compiled
+reference (tests/test_oracle.py), so equality here is equality with the
+reference on the same inputs.  Float arrays are compared with
+np.array_equal (exact; +0.0 and -0.0 compare equal), contact sets as int
+arrays.
+"""
+import numpy as np
+import pytest
+
+from cloth_sim_self_collision_amd import scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sim_mod():
+    from cloth_sim_self_collision_amd import _lib, sim
+    if _lib.device_count() < 1:
+        pytest.skip("no HIP device")
+    return sim
+
+
+def _scenes():
+    return {
+        "drape16": scene.drape_over_sphere(16),
+        "drape32": scene.drape_over_sphere(32),
+        "refdemo0": scene.reference_demo(0),
+        "refdemo1": scene.reference_demo(1),
+        "fold24": scene.fold_sweep(24, size=0.3),
+    }
+
+
+def _trajectory(oracle_mod, cl, steps, keep):
+    """Oracle states (x, v, x_start) before each step index in `keep`."""
+    o = oracle_mod.Oracle(cl, "port")
+    x, v, t = cl.xyz.copy(), cl.uvw.copy(), 0.0
+    out = {}
+    for k in range(steps):
+        scene.apply_sweep(cl, x, t + cl.dt)
+        if k in keep:
+            out[k] = (x.copy(), v.copy())
+        x, v, *_ = o.step(x, v)
+        t += cl.dt
+    return o, out
+
+
+def test_device_division_and_sqrt_are_correctly_rounded(sim_mod):
+    rng = np.random.default_rng(1)
+    n = 1 << 16
+    a = rng.standard_normal(n) * np.exp2(rng.integers(-60, 60, n))
+    b = rng.standard_normal(n) * np.exp2(rng.integers(-60, 60, n))
+    a = np.concatenate([a, [1.0, 2.0, 3.0, 1e-310, 5e-324, 1e308, 0.0, 0.25]])
+    b = np.concatenate([b, [3.0, 7.0, 1e-300, 3.0, 1.0, 1e-10, 1.0, 3.0]])
+    q, s = sim_mod.selftest_divsqrt(a, b)
+    assert np.array_equal(q, a / b)
+    pos = a >= 0
+    assert np.array_equal(s[pos], np.sqrt(a[pos]))
+
+
+def test_element_routines_bitwise(sim_mod, oracle_mod):
+    rng = np.random.default_rng(2)
+    n = 512
+    C = rng.standard_normal((n, 9))
+    c = C + 0.1 * rng.standard_normal((n, 9))
+    dW, ddW = sim_mod.element_cst(C, c, 576.9, 384.6)
+    for e in range(0, n, 37):
+        _, g, h = oracle_mod.element_cst(C[e], c[e], 576.9, 384.6)
+        assert np.array_equal(dW[e], g) and np.array_equal(ddW[e], h)
+    Cq = rng.standard_normal((n, 12))
+    cq = Cq + 0.1 * rng.standard_normal((n, 12))
+    dWb, ddWb = sim_mod.element_bend(Cq, cq, 1e-3)
+    for e in range(0, n, 37):
+        g, h = oracle_mod.element_bend(Cq[e], cq[e], 1e-3)
+        assert np.array_equal(dWb[e], g) and np.array_equal(ddWb[e], h)
+    p = rng.standard_normal((n, 3)) * 0.4
+    for ob in (scene.sphere((0.0, 0.0, 0.0), 0.3), scene.plane((0.0, 0.0, 1.0), -0.1),
+               scene.Obstacle(scene.OBST_SPHERE_INV, (0.1, 0.0, 0.0, 0.35))):
+        g, h = sim_mod.element_contact(p, 1e3, 0.01, ob)
+        for e in range(0, n, 17):
+            _, g0, h0 = oracle_mod.element_contact(p[e], 1e3, 0.01, ob.as_array())
+            assert np.array_equal(g[e], g0) and np.array_equal(h[e], h0)
+
+
+@pytest.mark.parametrize("name", ["drape16", "refdemo0", "fold24"])
+def test_setup_products(sim_mod, oracle_mod, name):
+    cl = _scenes()[name]
+    o = oracle_mod.Oracle(cl, "port")
+    with sim_mod.DeviceCloth(cl) as d:
+        for a, b in zip(d.pattern(), o.pattern()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(d.bvh(), o.bvh())
+
+
+@pytest.mark.parametrize("name,steps,keep", [
+    ("drape32", 70, (0, 25, 50, 69)),
+    ("refdemo1", 30, (0, 29)),
+    ("fold24", 40, (10, 39)),
+])
+def test_stages_bitwise(sim_mod, oracle_mod, name, steps, keep):
+    cl = _scenes()[name]
+    o, states = _trajectory(oracle_mod, cl, steps, keep)
+    with sim_mod.DeviceCloth(cl) as d:
+        for k, (x, v) in states.items():
+            dia, crs, b = d.assemble(x, v)
+            odia, ocrs, ob = o.assemble(x, v)
+            assert np.array_equal(dia, odia) and np.array_equal(crs, ocrs) and np.array_equal(b, ob), k
+            ns = d.ilu_factor()
+            fd, fc = d.get_factor()
+            ofd, ofc = o.ilu(odia, ocrs)
+            assert ns == 0
+            assert np.array_equal(fd, ofd) and np.array_equal(fc, ofc), k
+            vec = np.sin(np.arange(3 * cl.np_) * 0.37)
+            assert np.array_equal(d.ilu_solve(vec), o.ilu_solve(ofd, ofc, vec)), k
+            y = np.cos(np.arange(3 * cl.np_) * 0.11)
+            assert np.array_equal(d.matvec(1.0, vec, 0.5, y), o.matvec(odia, ocrs, 1.0, vec, 0.5, y)), k
+            xs, rs, conv, it = d.pcg_solve(b, cl.pcg_tol, cl.pcg_maxit)
+            oxs, ors, oconv, oit = o.pcg(odia, ocrs, ofd, ofc, ob, cl.pcg_tol, cl.pcg_maxit)
+            assert it == oit and conv == oconv, k
+            assert np.array_equal(xs, oxs) and np.array_equal(rs, ors), k
+            px = d.contacts_proximity(x, cl.clearance)
+            assert np.array_equal(px, o.prox_set(x, cl.clearance)), k
+            vv = v * 3.0
+            cc = d.contacts_ccd(x, vv, cl.dt, cl.clearance)
+            assert np.array_equal(cc, o.ccd_set(x, vv, cl.dt, cl.clearance)), k
+            bbp = d.bvh_refit_prx(x, cl.clearance)
+            assert np.array_equal(bbp, o.refit_prx(x, cl.clearance)), k
+            bbc = d.bvh_refit_ccd(x, vv, cl.dt)
+            assert np.array_equal(bbc, o.refit_ccd(x, vv, cl.dt)), k
+
+
+@pytest.mark.parametrize("name,steps", [("drape32", 100), ("refdemo0", 60), ("refdemo1", 60),
+                                        ("fold24", 60)])
+def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps):
+    cl = _scenes()[name]
+    o = oracle_mod.Oracle(cl, "port")
+    x, v = cl.xyz.copy(), cl.uvw.copy()
+    with sim_mod.DeviceCloth(cl) as d:
+        for k in range(steps):
+            x_before = x.copy()
+            scene.apply_sweep(cl, x, d.time + cl.dt)
+            x, v, imp, it, conv = o.step(x, v)
+            st = d.step(1)
+            gx, gv = d.get_state()
+            assert st["pcg_iters"] == it and st["impulse_applied"] == imp, (k, st)
+            assert np.array_equal(gx, x) and np.array_equal(gv, v), (
+                k, np.abs(gx - x).max(), np.abs(gv - v).max(), st)
+            del x_before
