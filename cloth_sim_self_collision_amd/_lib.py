@@ -69,6 +69,11 @@ SIGNATURES = {
     "cs_contact_resolve": (C.c_int, [_V, _I, C.POINTER(StepStats)]),
     "cs_post_contact_update": (C.c_int, [_V]),
     "cs_step": (C.c_int, [_V, C.c_int, C.POINTER(StepStats)]),
+    "cs_set_sweep": (C.c_int, [_V, C.c_int, _I, _D, C.c_double]),
+    "cs_set_time": (C.c_int, [_V, C.c_double]),
+    "cs_get_time": (C.c_int, [_V, _D]),
+    "cs_profile_enable": (C.c_int, [_V, C.c_int]),
+    "cs_profile_read": (C.c_int, [_V, C.c_int, _D, C.POINTER(C.c_longlong)]),
     "cs_get_pattern": (C.c_int, [_V, _I, _I, _I]),
     "cs_assemble": (C.c_int, [_V, _D, _D]),
     "cs_get_system": (C.c_int, [_V, _D, _D, _D]),
@@ -88,6 +93,7 @@ SIGNATURES = {
     "cs_element_bend": (C.c_int, [C.c_int, _D, _D, C.c_double, _D, _D]),
     "cs_element_contact": (C.c_int, [C.c_int, _D, C.c_double, C.c_double, C.c_int, _D, _D, _D]),
     "cs_selftest_divsqrt": (C.c_int, [C.c_int, _D, _D, _D]),
+    "cs_inspect_setup": (C.c_int, [C.POINTER(Mesh), _I, _I, _I, _I, _I]),
 }
 
 _lib = None
@@ -136,3 +142,22 @@ def device_count() -> int:
     n = C.c_int(0)
     load().cs_device_count(C.byref(n))
     return n.value
+
+
+def inspect_setup(cloth) -> dict:
+    """Host-only setup of a cloth's mesh (no device): pattern and BVH."""
+    lib = load()
+    keep = [f64(cloth.xyz0), f64(cloth.xyz), f64(cloth.uvw), i32(cloth.tri), i32(cloth.quad),
+            i32(cloth.bc)]
+    mesh = Mesh(cloth.np_, cloth.nt, cloth.nq, dp(keep[0]), dp(keep[1]), dp(keep[2]),
+                ip(keep[3]), ip(keep[4]), ip(keep[5]))
+    sizes = np.zeros(7, np.int32)
+    check(lib.cs_inspect_setup(C.byref(mesh), ip(sizes), None, None, None, None))
+    np_, nt, nq, nnz, nnode, nlf, nlb = (int(v) for v in sizes)
+    rp = np.zeros(np_ + 1, np.int32)
+    ci = np.zeros(max(nnz, 1), np.int32)
+    di = np.zeros(np_, np.int32)
+    nodes = np.zeros(3 * nnode, np.int32)
+    check(lib.cs_inspect_setup(C.byref(mesh), ip(sizes), ip(rp), ip(ci), ip(di), ip(nodes)))
+    return {"rowptr": rp, "colind": ci[:nnz], "diaind": di, "nodes": nodes.reshape(-1, 3),
+            "levels_forward": nlf, "levels_backward": nlb}

@@ -66,6 +66,13 @@ struct Cloth {
   ContactState contact;
   bool have_x1 = false;
   int poll = 8;
+  Prof prof;
+  double time = 0.0;  // simulated time, advanced by dt per cs_step
+  // kinematic sweep: vertices placed at rest + delta * clamp((t+dt)/duration, 0, 1)
+  int* sweep_verts = nullptr;
+  int n_sweep = 0;
+  double sweep_delta[3] = {0, 0, 0};
+  double sweep_duration = 1.0;
 };
 
 namespace {
@@ -265,9 +272,10 @@ int internal_step(Cloth* c, double* conv, int* iters) {
   hipStream_t s = c->stream;
   CS_HIP(hipMemcpyAsync(d.x1, d.x, sizeof(double) * 3 * d.np, hipMemcpyDeviceToDevice, s));
   c->have_x1 = true;
-  if (int st = launch_assemble(d, c->P, s)) return st;
-  if (int st = launch_ilu(d, c->ising, s)) return st;
-  if (int st = run_pcg(d, c->P.pcg_tol, c->P.pcg_maxit, c->hflags, s, c->poll)) return st;
+  Prof* prof = c->prof.on ? &c->prof : nullptr;
+  if (int st = launch_assemble(d, c->P, s, prof)) return st;
+  if (int st = launch_ilu(d, c->ising, s, prof)) return st;
+  if (int st = run_pcg(d, c->P.pcg_tol, c->P.pcg_maxit, c->hflags, s, c->poll, prof)) return st;
   if (int st = launch_finish_internal(d, c->P, s)) return st;
   CS_HIP(hipMemcpyAsync(c->hflags, d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
   CS_HIP(hipMemcpyAsync(c->hscal, d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
@@ -332,6 +340,8 @@ int cs_cloth_destroy(cs_cloth* p) {
   Cloth* c = H(p);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   contact_release(c->contact);
+  c->prof.release();
+  if (c->sweep_verts) (void)hipFree(c->sweep_verts);
   c->arena.release();
   if (c->hflags) (void)hipHostFree(c->hflags);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -449,9 +459,16 @@ int cs_step(cs_cloth* p, int n, cs_step_stats* stats) {
     std::memset(&st_local, 0, sizeof(st_local));
     double conv = 0;
     int iters = 0;
+    if (c->n_sweep > 0) {
+      double sv = (c->time + c->P.dt) / c->sweep_duration;
+      sv = sv < 0.0 ? 0.0 : (sv > 1.0 ? 1.0 : sv);
+      const double off[3] = {c->sweep_delta[0] * sv, c->sweep_delta[1] * sv, c->sweep_delta[2] * sv};
+      if (int st = launch_apply_sweep(c->d, c->sweep_verts, c->n_sweep, off, c->stream)) return st;
+    }
     if (int st = internal_step(c, &conv, &iters)) return st;
     int applied = 0;
-    if (int st = contact_resolve(c->contact, c->S, c->d, c->P, c->stream, &applied, &st_local))
+    Prof* prof = c->prof.on ? &c->prof : nullptr;
+    if (int st = contact_resolve(c->contact, c->S, c->d, c->P, c->stream, &applied, &st_local, prof))
       return st;
     if (applied) {
       if (int st = post_contact_update(c->d, c->P, c->stream)) return st;
@@ -461,8 +478,60 @@ int cs_step(cs_cloth* p, int n, cs_step_stats* stats) {
     st_local.ilu_singular = c->hflags[kNsing];
     st_local.impulse_applied = applied;
     if (stats) *stats = st_local;
+    c->time += c->P.dt;
   }
   return sync(c);
+}
+
+int cs_set_sweep(cs_cloth* p, int n, const int* verts, const double* delta, double duration) {
+  if (!p || n < 0 || (n > 0 && (!verts || !delta)) || !(duration > 0)) return kBadArg;
+  Cloth* c = H(p);
+  for (int i = 0; i < n; i++)
+    if (verts[i] < 0 || verts[i] >= c->S.np) {
+      set_error("sweep vertex out of range");
+      return kBadArg;
+    }
+  if (c->sweep_verts) (void)hipFree(c->sweep_verts);
+  c->sweep_verts = nullptr;
+  c->n_sweep = 0;
+  if (n > 0) {
+    CS_HIP(hipMalloc((void**)&c->sweep_verts, sizeof(int) * n));
+    CS_HIP(hipMemcpy(c->sweep_verts, verts, sizeof(int) * n, hipMemcpyHostToDevice));
+  }
+  c->n_sweep = n;
+  for (int k = 0; k < 3; k++) c->sweep_delta[k] = n > 0 ? delta[k] : 0.0;
+  c->sweep_duration = duration;
+  return kOk;
+}
+
+int cs_set_time(cs_cloth* p, double t) {
+  if (!p) return kBadArg;
+  H(p)->time = t;
+  return kOk;
+}
+
+int cs_get_time(const cs_cloth* p, double* t) {
+  if (!p || !t) return kBadArg;
+  *t = H(p)->time;
+  return kOk;
+}
+
+int cs_profile_enable(cs_cloth* p, int on) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  c->prof.stream = c->stream;
+  c->prof.reset();
+  c->prof.on = on != 0;
+  return kOk;
+}
+
+int cs_profile_read(cs_cloth* p, int kind, double* total_ms, long long* launches) {
+  if (!p || kind < 0 || kind >= kNumProf) return kBadArg;
+  Cloth* c = H(p);
+  c->prof.harvest();
+  if (total_ms) *total_ms = c->prof.total_ms[kind];
+  if (launches) *launches = c->prof.count[kind];
+  return kOk;
 }
 
 int cs_get_pattern(const cs_cloth* p, int* rowptr, int* colind, int* diaind) {
@@ -576,6 +645,34 @@ int cs_pcg_solve(cs_cloth* p, const double* b, double tol, int maxit, double* x,
   if (int st = sync(c)) return st;
   if (conv) *conv = c->hscal[kConv];
   if (iters) *iters = c->hflags[kDone] ? c->hflags[kIters] : maxit;
+  return kOk;
+}
+
+int cs_inspect_setup(const cs_mesh* m, int* sizes, int* rowptr, int* colind, int* diaind,
+                     int* nodes) {
+  if (!m || !m->xyz || !m->tri || (m->nq > 0 && !m->quad)) {
+    set_error("cs_inspect_setup: missing mesh arrays");
+    return kBadArg;
+  }
+  MeshSetup S;
+  std::string err = build_setup(S, m->np, m->nt, m->nq, m->tri, m->quad, m->xyz);
+  if (!err.empty()) {
+    set_error(err);
+    return kBadArg;
+  }
+  if (sizes) {
+    sizes[0] = S.np;
+    sizes[1] = S.nt;
+    sizes[2] = S.nq;
+    sizes[3] = S.nnz;
+    sizes[4] = S.nnode;
+    sizes[5] = S.nlf;
+    sizes[6] = S.nlb;
+  }
+  if (rowptr) std::memcpy(rowptr, S.rowptr.data(), sizeof(int) * (S.np + 1));
+  if (colind) std::memcpy(colind, S.colind.data(), sizeof(int) * S.nnz);
+  if (diaind) std::memcpy(diaind, S.diaind.data(), sizeof(int) * S.np);
+  if (nodes) std::memcpy(nodes, S.node.data(), sizeof(int) * 3 * S.nnode);
   return kOk;
 }
 

@@ -930,8 +930,9 @@ void contact_release(ContactState& cs) {
 }
 
 int contact_resolve(ContactState& cs, const MeshSetup& S, const DevCloth& d, const Params& P,
-                    hipStream_t s, int* applied, cs_step_stats* stats) {
+                    hipStream_t s, int* applied, cs_step_stats* stats, Prof* prof) {
   (void)S;
+  ProfScope ps(prof, kProfContact);
   const double* X = d.x1;
   double* U = d.v;
   int g = 0;

@@ -55,7 +55,7 @@ int contact_init(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStr
 void contact_release(ContactState& cs);
 // full contact pass on (d.x1, d.v); stats may be NULL
 int contact_resolve(ContactState& cs, const MeshSetup& S, const DevCloth& d, const Params& P,
-                    hipStream_t s, int* applied, cs_step_stats* stats);
+                    hipStream_t s, int* applied, cs_step_stats* stats, Prof* prof = nullptr);
 // x = x1 + v dt, then UpdateIntermidiateVelocity
 int post_contact_update(const DevCloth& d, const Params& P, hipStream_t s);
 // host-array helpers of the C-ABI (uvw == NULL -> proximity)

@@ -122,6 +122,85 @@ struct DevCloth {
   double* bb;
 };
 
+// Optional per-kernel-class timing with HIP events recorded on the cloth's
+// stream around each launch (cs_profile_enable / cs_profile_read).
+enum ProfKind : int {
+  kProfSweep = 0,     // ILU forward+backward substitution
+  kProfReduce = 1,    // ordered PCG reductions
+  kProfSpmv = 2,      // search direction + A p
+  kProfAxpy = 3,      // r, x updates
+  kProfFactor = 4,    // ILU(0) factorisation
+  kProfAssemble = 5,  // element kernels + gather assembly
+  kProfContact = 6,   // refit, traversal, selection, impulses, zones
+  kProfOther = 7,
+  kNumProf = 8
+};
+
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<int> kinds;
+  hipStream_t stream = nullptr;
+  double total_ms[kNumProf] = {0};
+  long count[kNumProf] = {0};
+  // fold the recorded pairs into the totals (waits for the last event)
+  void harvest() {
+    if (used == 0) return;
+    (void)hipEventSynchronize(pool[used - 1]);
+    for (size_t k = 0; k < kinds.size(); k++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pool[2 * k], pool[2 * k + 1]) == hipSuccess) {
+        total_ms[kinds[k]] += ms;
+        count[kinds[k]] += 1;
+      }
+    }
+    used = 0;
+    kinds.clear();
+  }
+  void reset() {
+    harvest();
+    for (int k = 0; k < kNumProf; k++) {
+      total_ms[k] = 0;
+      count[k] = 0;
+    }
+  }
+  void release() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+    used = 0;
+    kinds.clear();
+  }
+  int begin(int kind) {
+    if (!on) return -1;
+    if (used + 2 > 4096) harvest();
+    if (used + 2 > pool.size()) {
+      for (int k = 0; k < 64; k++) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        pool.push_back(e);
+      }
+    }
+    const int id = (int)used;
+    used += 2;
+    kinds.push_back(kind);
+    (void)hipEventRecord(pool[id], stream);
+    return id;
+  }
+  void end(int id) {
+    if (id >= 0) (void)hipEventRecord(pool[id + 1], stream);
+  }
+};
+
+struct ProfScope {
+  Prof* p;
+  int id;
+  ProfScope(Prof* pr, int kind) : p(pr), id(pr ? pr->begin(kind) : -1) {}
+  ~ProfScope() {
+    if (p) p->end(id);
+  }
+};
+
 struct Cloth;  // host handle (cs_api.cpp)
 
 }  // namespace cs

@@ -515,8 +515,27 @@ __global__ void k_finish_internal(DevCloth c, double dt) {
   c.v[e] = d / dt;
 }
 
+// prescribed positions of swept vertices: x = rest + delta * s
+__global__ void k_apply_sweep(DevCloth c, const int* verts, int n, double o0, double o1, double o2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = verts[i];
+  c.x[3 * v] = c.xyz0[3 * v] + o0;
+  c.x[3 * v + 1] = c.xyz0[3 * v + 1] + o1;
+  c.x[3 * v + 2] = c.xyz0[3 * v + 2] + o2;
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
+
+int launch_apply_sweep(const DevCloth& c, const int* verts, int n, const double off[3],
+                       hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_apply_sweep, dim3(grid_for(n)), dim3(kTB), 0, s, c, verts, n, off[0],
+                       off[1], off[2]);
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
 
 int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s) {
   if (c.nq > 0) hipLaunchKernelGGL(k_bend_setup, dim3(grid_for(c.nq)), dim3(kTB), 0, s, c, P.stiff_bend);
@@ -524,7 +543,8 @@ int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s) {
   return kOk;
 }
 
-int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s) {
+int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s, Prof* prof) {
+  ProfScope ps(prof, kProfAssemble);
   hipLaunchKernelGGL(k_tri_elem, dim3(grid_for(c.nt)), dim3(kTB), 0, s, c, P.lam, P.mu, 1);
   if (c.nq > 0) hipLaunchKernelGGL(k_quad_grad, dim3(grid_for(c.nq)), dim3(kTB), 0, s, c);
   if (c.nnz > 0) hipLaunchKernelGGL(k_asm_crs, dim3(grid_for(c.nnz)), dim3(kTB), 0, s, c);
@@ -533,7 +553,8 @@ int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s) {
   return kOk;
 }
 
-int launch_ilu(const DevCloth& c, int* ising, hipStream_t s) {
+int launch_ilu(const DevCloth& c, int* ising, hipStream_t s, Prof* prof) {
+  ProfScope ps(prof, kProfFactor);
   CS_HIP(hipMemcpyAsync(c.fdia, c.dia, sizeof(double) * 9 * c.np, hipMemcpyDeviceToDevice, s));
   if (c.nnz > 0)
     CS_HIP(hipMemcpyAsync(c.fcrs, c.crs, sizeof(double) * 9 * (size_t)c.nnz,
@@ -562,25 +583,56 @@ int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStrea
 // Solve_PCG on the assembled system (c.b) with the factor in fdia/fcrs.
 // Result in c.dx; conv/iters in the device scalars.  `host_flags` is pinned
 // memory used to poll for convergence every `poll` iterations.
-int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream_t s, int poll) {
+int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream_t s, int poll,
+            Prof* prof) {
   const long n3 = 3L * c.np;
-  hipLaunchKernelGGL(k_pcg_init, dim3(grid_for(n3)), dim3(kTB), 0, s, c, tol * tol);
-  hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr0, 0);
-  hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
-  hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz0, 0);
+  {
+    ProfScope ps(prof, kProfOther);
+    hipLaunchKernelGGL(k_pcg_init, dim3(grid_for(n3)), dim3(kTB), 0, s, c, tol * tol);
+  }
+  {
+    ProfScope ps(prof, kProfReduce);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr0, 0);
+  }
+  {
+    ProfScope ps(prof, kProfSweep);
+    hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
+  }
+  {
+    ProfScope ps(prof, kProfReduce);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz0, 0);
+  }
   CS_HIP(hipGetLastError());
   bool done = false;
   int it = 0;
   for (; it < maxit && !done; it++) {
     double* pnew = c.p[it & 1];
     const double* pold = c.p[(it + 1) & 1];
-    hipLaunchKernelGGL(k_spmv_dir, dim3(grid_for(c.np)), dim3(kTB), 0, s, c, it == 0 ? 1 : 0,
-                       pold, pnew);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotPap, it);
-    hipLaunchKernelGGL(k_update_rx, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)pnew);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr, it);
-    hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz, it);
+    {
+      ProfScope ps(prof, kProfSpmv);
+      hipLaunchKernelGGL(k_spmv_dir, dim3(grid_for(c.np)), dim3(kTB), 0, s, c, it == 0 ? 1 : 0,
+                         pold, pnew);
+    }
+    {
+      ProfScope ps(prof, kProfReduce);
+      hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotPap, it);
+    }
+    {
+      ProfScope ps(prof, kProfAxpy);
+      hipLaunchKernelGGL(k_update_rx, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)pnew);
+    }
+    {
+      ProfScope ps(prof, kProfReduce);
+      hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr, it);
+    }
+    {
+      ProfScope ps(prof, kProfSweep);
+      hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
+    }
+    {
+      ProfScope ps(prof, kProfReduce);
+      hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz, it);
+    }
     CS_HIP(hipGetLastError());
     if ((it + 1) % poll == 0 || it + 1 == maxit) {
       CS_HIP(hipMemcpyAsync(host_flags, c.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
@@ -591,6 +643,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   if (!done) {
     // cap reached: report the achieved ratio (ilu_sparse.cpp:706-707)
     hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)c.r);
+    ProfScope ps(prof, kProfReduce);
     hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotFinal, maxit);
     CS_HIP(hipGetLastError());
   }

@@ -29,7 +29,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, dp, f64, i32, ip
-from .scene import Cloth, apply_sweep
+from .scene import Cloth
 
 
 def make_params(cloth: Cloth) -> _lib.Params:
@@ -70,7 +70,11 @@ class DeviceCloth:
         sz = [C.c_int() for _ in range(5)]
         check(lib.cs_cloth_sizes(self.h, *[C.byref(s) for s in sz]))
         self.np_, self.nt, self.nq, self.nnz, self.nnode = [s.value for s in sz]
-        self.time = 0.0
+        if cloth.sweep is not None:
+            verts = i32(cloth.sweep.vertices)
+            delta = f64(cloth.sweep.delta)
+            check(lib.cs_set_sweep(self.h, len(verts), ip(verts), dp(delta),
+                                   float(cloth.sweep.duration)))
 
     # -- lifetime
     def close(self):
@@ -132,22 +136,38 @@ class DeviceCloth:
     def update_intermediate_velocity(self):
         check(_lib.load().cs_post_contact_update(self.h))
 
+    @property
+    def time(self) -> float:
+        t = C.c_double()
+        check(_lib.load().cs_get_time(self.h, C.byref(t)))
+        return t.value
+
+    @time.setter
+    def time(self, t: float) -> None:
+        check(_lib.load().cs_set_time(self.h, float(t)))
+
     def step(self, n: int = 1) -> dict:
-        """n StepTime iterations; applies the cloth's kinematic sweep (if any)
-        before each step.  Returns the stats of the last step."""
+        """n StepTime iterations on the device (the kinematic sweep, if any, is
+        applied on the device before each step).  Returns the last step's stats."""
         st = _lib.StepStats()
-        lib = _lib.load()
-        if self.cloth.sweep is None:
-            check(lib.cs_step(self.h, n, C.byref(st)))
-            self.time += n * self.cloth.dt
-            return st.as_dict()
-        for _ in range(n):
-            x, v = self.get_state()
-            apply_sweep(self.cloth, x, self.time + self.cloth.dt)
-            self.set_state(x, None)
-            check(lib.cs_step(self.h, 1, C.byref(st)))
-            self.time += self.cloth.dt
+        check(_lib.load().cs_step(self.h, n, C.byref(st)))
         return st.as_dict()
+
+    # -- timing
+    PROFILE_KINDS = ("ilu_sweep", "pcg_reduce", "spmv", "vector_update", "ilu_factor",
+                     "assemble", "contact", "other")
+
+    def profile(self, on: bool = True) -> None:
+        check(_lib.load().cs_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        out = {}
+        for k, name in enumerate(self.PROFILE_KINDS):
+            ms = C.c_double()
+            n = C.c_longlong()
+            check(_lib.load().cs_profile_read(self.h, k, C.byref(ms), C.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
 
     # -- lower-level operations
     def pattern(self):

@@ -122,8 +122,22 @@ int cs_contact_resolve(cs_cloth* c, int* impulse_applied, cs_step_stats* stats);
  * (solve_internal_sparse.h:329-369); StepTime does this only when an impulse
  * was applied. */
 int cs_post_contact_update(cs_cloth* c);
-/* n full StepTime iterations; stats of the last step (may be NULL) */
+/* n full StepTime iterations; stats of the last step (may be NULL).  Before
+ * each step the swept vertices (cs_set_sweep) are placed at
+ * rest + delta * clamp((t + dt) / duration, 0, 1); t then advances by dt. */
 int cs_step(cs_cloth* c, int n, cs_step_stats* stats);
+/* kinematic boundary: n vertices (which should carry bc = 1) moved linearly by
+ * delta[3] over `duration` seconds, then held.  The reference has only fixed
+ * boundaries; this drives the fold workload. n = 0 clears it. */
+int cs_set_sweep(cs_cloth* c, int n, const int* verts, const double* delta, double duration);
+int cs_set_time(cs_cloth* c, double t);
+int cs_get_time(const cs_cloth* c, double* t);
+
+/* per-kernel-class timing with HIP events on the cloth's stream:
+ * 0 ILU sweeps, 1 PCG reductions, 2 SpMV, 3 vector updates, 4 ILU factor,
+ * 5 assembly, 6 contact pass, 7 other.  Enabling resets the totals. */
+int cs_profile_enable(cs_cloth* c, int on);
+int cs_profile_read(cs_cloth* c, int kind, double* total_ms, long long* launches);
 
 /* ---- lower-level entry points (host arrays unless noted) ---- */
 int cs_get_pattern(const cs_cloth* c, int* rowptr, int* colind, int* diaind);
@@ -168,6 +182,13 @@ int cs_element_bend(int n, const double* C, const double* c, double stiff, doubl
 /* WdWddW_Contact for n points against an obstacle: dW[3n], ddW[9n] */
 int cs_element_contact(int n, const double* p, double stiff, double clearance, int kind,
                        const double* obstacle, double* dW, double* ddW);
+
+/* host-only: the per-mesh setup of cs_cloth_create without a device.
+ * sizes[7] = np, nt, nq, nnz, nnode, forward levels, backward levels; any
+ * output array may be NULL (call once with NULLs to learn the sizes):
+ * rowptr[np+1], colind[nnz], diaind[np], nodes[3*nnode] */
+int cs_inspect_setup(const cs_mesh* mesh, int* sizes, int* rowptr, int* colind, int* diaind,
+                     int* nodes);
 
 /* device arithmetic check: out[2i] = a[i]/b[i], out[2i+1] = sqrt(a[i]) */
 int cs_selftest_divsqrt(int n, const double* a, const double* b, double* out);
