@@ -70,6 +70,7 @@ SIGNATURES = {
     "cs_post_contact_update": (C.c_int, [_V]),
     "cs_step": (C.c_int, [_V, C.c_int, C.POINTER(StepStats)]),
     "cs_set_sweep": (C.c_int, [_V, C.c_int, _I, _D, C.c_double]),
+    "cs_set_option": (C.c_int, [_V, C.c_int, C.c_int]),
     "cs_set_time": (C.c_int, [_V, C.c_double]),
     "cs_get_time": (C.c_int, [_V, _D]),
     "cs_profile_enable": (C.c_int, [_V, C.c_int]),
@@ -93,6 +94,7 @@ SIGNATURES = {
     "cs_element_bend": (C.c_int, [C.c_int, _D, _D, C.c_double, _D, _D]),
     "cs_element_contact": (C.c_int, [C.c_int, _D, C.c_double, C.c_double, C.c_int, _D, _D, _D]),
     "cs_selftest_divsqrt": (C.c_int, [C.c_int, _D, _D, _D]),
+    "cs_selftest_ordered_sum": (C.c_int, [C.c_int, _D, _D]),
     "cs_inspect_setup": (C.c_int, [C.POINTER(Mesh), _I, _I, _I, _I, _I]),
 }
 
@@ -151,13 +153,14 @@ def inspect_setup(cloth) -> dict:
             i32(cloth.bc)]
     mesh = Mesh(cloth.np_, cloth.nt, cloth.nq, dp(keep[0]), dp(keep[1]), dp(keep[2]),
                 ip(keep[3]), ip(keep[4]), ip(keep[5]))
-    sizes = np.zeros(7, np.int32)
+    sizes = np.zeros(10, np.int32)
     check(lib.cs_inspect_setup(C.byref(mesh), ip(sizes), None, None, None, None))
-    np_, nt, nq, nnz, nnode, nlf, nlb = (int(v) for v in sizes)
+    np_, nt, nq, nnz, nnode, nlf, nlb, kl, ku, ring = (int(v) for v in sizes)
     rp = np.zeros(np_ + 1, np.int32)
     ci = np.zeros(max(nnz, 1), np.int32)
     di = np.zeros(np_, np.int32)
     nodes = np.zeros(3 * nnode, np.int32)
     check(lib.cs_inspect_setup(C.byref(mesh), ip(sizes), ip(rp), ip(ci), ip(di), ip(nodes)))
     return {"rowptr": rp, "colind": ci[:nnz], "diaind": di, "nodes": nodes.reshape(-1, 3),
-            "levels_forward": nlf, "levels_backward": nlb}
+            "levels_forward": nlf, "levels_backward": nlb, "max_lower": kl, "max_upper": ku,
+            "ring_ok": bool(ring)}

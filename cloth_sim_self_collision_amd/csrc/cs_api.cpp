@@ -185,6 +185,15 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&d.flags, kNumFlag);
   A.reserve(&d.bb, 6 * (size_t)nn);
   A.reserve(&c->ising, np);
+  int *fcnt, *fcol, *bcnt, *bcol;
+  const size_t kl = S.kl > 0 ? (size_t)S.kl : 0, ku = S.ku > 0 ? (size_t)S.ku : 0;
+  A.reserve(&fcnt, kl ? np : 0);
+  A.reserve(&fcol, kl * np);
+  A.reserve(&bcnt, ku ? np : 0);
+  A.reserve(&bcol, ku * np);
+  A.reserve(&d.f_blk, kl * 9 * np);
+  A.reserve(&d.f_dinv, kl ? 9 * (size_t)np : 0);
+  A.reserve(&d.b_blk, ku * 9 * np);
   double* host_pd;
   A.reserve(&host_pd, 4 * (size_t)np);
   if (int st = A.commit()) return st;
@@ -214,6 +223,13 @@ int build(Cloth* c, const cs_mesh* m) {
   d.by_pre = bypre;
   d.leaf_of = leafof;
   d.refit_order = rord;
+  d.kl = S.kl;
+  d.ku = S.ku;
+  d.ring_ok = S.ring_ok ? 1 : 0;
+  d.f_cnt = fcnt;
+  d.f_col = fcol;
+  d.b_cnt = bcnt;
+  d.b_col = bcol;
   hipStream_t s = c->stream;
   CS_HIP(hipMemcpyAsync(tri, m->tri, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, s));
   if (nq) CS_HIP(hipMemcpyAsync(quad, m->quad, sizeof(int) * 4 * nq, hipMemcpyHostToDevice, s));
@@ -244,6 +260,12 @@ int build(Cloth* c, const cs_mesh* m) {
   if (int st = upload(bypre, S.by_pre, s)) return st;
   if (int st = upload(leafof, S.leaf_of, s)) return st;
   if (int st = upload(rord, S.refit_order, s)) return st;
+  if (S.kl > 0) {
+    if (int st = upload(fcnt, S.f_cnt, s)) return st;
+    if (int st = upload(fcol, S.f_col, s)) return st;
+    if (int st = upload(bcnt, S.b_cnt, s)) return st;
+    if (int st = upload(bcol, S.b_col, s)) return st;
+  }
   CS_HIP(hipMemcpyAsync(d.x, m->xyz, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s));
   if (m->uvw)
     CS_HIP(hipMemcpyAsync(d.v, m->uvw, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s));
@@ -504,6 +526,17 @@ int cs_set_sweep(cs_cloth* p, int n, const int* verts, const double* delta, doub
   return kOk;
 }
 
+int cs_set_option(cs_cloth* p, int option, int value) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  if (option == CS_OPT_SWEEP && value >= 0 && value <= 2) {
+    c->d.sweep_variant = value;
+    return kOk;
+  }
+  set_error("unknown option or value");
+  return kBadArg;
+}
+
 int cs_set_time(cs_cloth* p, double t) {
   if (!p) return kBadArg;
   H(p)->time = t;
@@ -603,6 +636,7 @@ int cs_set_factor(cs_cloth* p, const double* fdia, const double* fcrs) {
   if (fdia) CS_HIP(hipMemcpyAsync(c->d.fdia, fdia, sizeof(double) * 9 * c->S.np, hipMemcpyHostToDevice, s));
   if (fcrs && c->S.nnz)
     CS_HIP(hipMemcpyAsync(c->d.fcrs, fcrs, sizeof(double) * 9 * c->S.nnz, hipMemcpyHostToDevice, s));
+  if (int st = launch_pack_factor(c->d, s)) return st;
   return sync(c);
 }
 
@@ -668,6 +702,9 @@ int cs_inspect_setup(const cs_mesh* m, int* sizes, int* rowptr, int* colind, int
     sizes[4] = S.nnode;
     sizes[5] = S.nlf;
     sizes[6] = S.nlb;
+    sizes[7] = S.kl;
+    sizes[8] = S.ku;
+    sizes[9] = S.ring_ok ? 1 : 0;
   }
   if (rowptr) std::memcpy(rowptr, S.rowptr.data(), sizeof(int) * (S.np + 1));
   if (colind) std::memcpy(colind, S.colind.data(), sizeof(int) * S.nnz);

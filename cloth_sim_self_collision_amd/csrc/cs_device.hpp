@@ -120,6 +120,16 @@ struct DevCloth {
   int* flags;
   // BVH boxes (6 per node)
   double* bb;
+  // packed sweeps (null when a row has more than kMaxBlocksPerRow blocks)
+  int kl, ku, ring_ok;
+  int sweep_variant;  // 0 automatic, 1 packed without the LDS ring, 2 row-by-row
+  const int* f_cnt;
+  const int* f_col;
+  const int* b_cnt;
+  const int* b_col;
+  double* f_blk;   // [(k*9 + q) * np + slot]
+  double* f_dinv;  // [q * np + slot]
+  double* b_blk;
 };
 
 // Optional per-kernel-class timing with HIP events recorded on the cloth's

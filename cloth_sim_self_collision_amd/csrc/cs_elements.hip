@@ -8,6 +8,7 @@
 #include "../../include/clothsim.h"
 #include "cs_device.hpp"
 #include "cs_math.hpp"
+#include "cs_ordered_sum.hpp"
 
 namespace cs {
 
@@ -57,6 +58,11 @@ __global__ void k_divsqrt(int n, const double* a, const double* b, double* out) 
   if (e >= n) return;
   out[2 * e] = a[e] / b[e];
   out[2 * e + 1] = sqrt(a[e]);
+}
+
+__global__ __launch_bounds__(kSumThreads) void k_ordered_sum(const double* t, int n, double* out) {
+  const double s = ordered_sum_exact(t, n);
+  if (threadIdx.x == 0) out[0] = s;
 }
 
 namespace {
@@ -144,5 +150,17 @@ extern "C" int cs_selftest_divsqrt(int n, const double* a, const double* b, doub
   hipLaunchKernelGGL(k_divsqrt, dim3(blocks(n)), dim3(256), 0, 0, n, da, db, dout);
   CS_HIP(hipGetLastError());
   CS_HIP(hipMemcpy(out, dout, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+extern "C" int cs_selftest_ordered_sum(int n, const double* t, double* out) {
+  if (n < 0 || (n > 0 && !t) || !out) return kBadArg;
+  Scratch s;
+  double *dt, *dout;
+  if (s.get(&dt, n) || s.get(&dout, 1)) return kHipError;
+  if (n) CS_HIP(hipMemcpy(dt, t, sizeof(double) * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_ordered_sum, dim3(1), dim3(kSumThreads), 0, 0, dt, n, dout);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpy(out, dout, sizeof(double), hipMemcpyDeviceToHost));
   return kOk;
 }

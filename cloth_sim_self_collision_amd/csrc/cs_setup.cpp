@@ -284,6 +284,55 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
     levels_to_lists(lf, nf, s.lf_ptr, s.lf_rows, &s.max_lf);
     levels_to_lists(lb, nb, s.lb_ptr, s.lb_rows, &s.max_lb);
   }
+  // packed sweep layout and the ring-buffer safety check
+  {
+    s.kl = s.ku = 0;
+    for (int i = 0; i < np; i++) {
+      s.kl = std::max(s.kl, s.diaind[i] - s.rowptr[i]);
+      s.ku = std::max(s.ku, s.rowptr[i + 1] - s.diaind[i]);
+    }
+    if (s.kl <= kMaxBlocksPerRow && s.ku <= kMaxBlocksPerRow) {
+      s.f_cnt.assign(np, 0);
+      s.b_cnt.assign(np, 0);
+      s.f_col.assign((size_t)std::max(s.kl, 1) * np, 0);
+      s.b_col.assign((size_t)std::max(s.ku, 1) * np, 0);
+      for (int sl = 0; sl < np; sl++) {
+        const int i = s.lf_rows[sl];
+        s.f_cnt[sl] = s.diaind[i] - s.rowptr[i];
+        for (int k = 0; k < s.f_cnt[sl]; k++) s.f_col[(size_t)k * np + sl] = s.colind[s.rowptr[i] + k];
+        const int j = s.lb_rows[sl];
+        s.b_cnt[sl] = s.rowptr[j + 1] - s.diaind[j];
+        for (int k = 0; k < s.b_cnt[sl]; k++) s.b_col[(size_t)k * np + sl] = s.colind[s.diaind[j] + k];
+      }
+      // live interval of a row in a sweep: [level written, last level read];
+      // rows that share a ring slot must have disjoint intervals
+      auto check = [&](const std::vector<int>& lvl, bool lower) {
+        std::vector<int> last(np);
+        for (int i = 0; i < np; i++) last[i] = lvl[i];
+        for (int i = 0; i < np; i++) {
+          const int b = lower ? s.rowptr[i] : s.diaind[i];
+          const int e = lower ? s.diaind[i] : s.rowptr[i + 1];
+          for (int k = b; k < e; k++) last[s.colind[k]] = std::max(last[s.colind[k]], lvl[i]);
+        }
+        std::vector<std::vector<std::pair<int, int>>> slot(kRingRows);
+        for (int i = 0; i < np; i++) slot[i & (kRingRows - 1)].push_back({lvl[i], last[i]});
+        for (auto& v : slot) {
+          std::sort(v.begin(), v.end());
+          for (size_t q = 1; q < v.size(); q++)
+            if (v[q].first <= v[q - 1].second) return false;
+        }
+        return true;
+      };
+      std::vector<int> lf(np), lb(np);
+      for (int l = 0; l < s.nlf; l++)
+        for (int q = s.lf_ptr[l]; q < s.lf_ptr[l + 1]; q++) lf[s.lf_rows[q]] = l;
+      for (int l = 0; l < s.nlb; l++)
+        for (int q = s.lb_ptr[l]; q < s.lb_ptr[l + 1]; q++) lb[s.lb_rows[q]] = l;
+      s.ring_ok = check(lf, true) && check(lb, false);
+    } else {
+      s.kl = s.ku = 0;
+    }
+  }
   // gather lists: sources of every Hessian block and gradient entry in the
   // order the reference's element loops add them (solve_internal_sparse.h:19-76)
   {

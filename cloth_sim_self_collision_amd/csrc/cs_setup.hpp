@@ -23,6 +23,13 @@ struct MeshSetup {
   // strictly lower part) and backward substitution (strictly upper part)
   std::vector<int> lf_ptr, lf_rows, lb_ptr, lb_rows;
   int nlf = 0, nlb = 0, max_lf = 0, max_lb = 0;
+  // packed sweeps: rows in level order ("slots"), at most kl lower / ku upper
+  // blocks per row; column indices stored slot-minor: col[k*np + slot]
+  int kl = 0, ku = 0;
+  std::vector<int> f_cnt, f_col, b_cnt, b_col;
+  // the LDS ring of kRingRows rows can hold the sweep dependencies: no two rows
+  // sharing a ring slot are live at the same level
+  bool ring_ok = false;
   // gather lists in element order (assembly without atomics, in the
   // reference's summation order)
   std::vector<int> crs_src_ptr, crs_src;
@@ -43,6 +50,9 @@ struct MeshSetup {
   std::vector<int> refit_lvl_ptr;  // level boundaries inside refit_order
   int max_depth = 0;
 };
+
+constexpr int kRingRows = 4096;
+constexpr int kMaxBlocksPerRow = 8;
 
 // Returns an empty string on success, else a description of the problem.
 std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,

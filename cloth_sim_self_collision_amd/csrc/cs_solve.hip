@@ -12,6 +12,7 @@
 
 #include "cs_device.hpp"
 #include "cs_math.hpp"
+#include "cs_ordered_sum.hpp"
 #include "cs_solve.hpp"
 
 namespace cs {
@@ -315,6 +316,144 @@ __global__ __launch_bounds__(1024) void k_ilu_apply(DevCloth c, const double* sr
     for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src[e] * dst[e];
 }
 
+// ---------------------------------------------------------------------------
+// packed sweeps: the factor rearranged so that the rows of one level are
+// consecutive "slots" and every block component is contiguous over slots
+// (coalesced loads), the dependency values kept in an LDS ring of
+// kRingRows rows when the setup proved that no two rows sharing a ring slot
+// are live at the same time.  Per row the arithmetic is exactly that of
+// k_ilu_apply (same blocks in the same column order).
+
+__global__ void k_pack_factor(DevCloth c) {
+  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = c.np;
+  if (sl >= np) return;
+  {
+    const int i = c.lf_rows[sl];
+    const int cnt = c.f_cnt[sl];
+    const double* src = c.fcrs + (size_t)c.rowptr[i] * 9;
+    for (int k = 0; k < cnt; k++)
+      for (int q = 0; q < 9; q++) c.f_blk[(size_t)(k * 9 + q) * np + sl] = src[k * 9 + q];
+    for (int q = 0; q < 9; q++) c.f_dinv[(size_t)q * np + sl] = c.fdia[(size_t)i * 9 + q];
+  }
+  {
+    const int i = c.lb_rows[sl];
+    const int cnt = c.b_cnt[sl];
+    const double* src = c.fcrs + (size_t)c.diaind[i] * 9;
+    for (int k = 0; k < cnt; k++)
+      for (int q = 0; q < 9; q++) c.b_blk[(size_t)(k * 9 + q) * np + sl] = src[k * 9 + q];
+  }
+}
+
+template <bool kRing>
+__global__ __launch_bounds__(512) void k_ilu_apply_packed(DevCloth c, const double* __restrict__ src,
+                                                          double* dst, double* prod) {
+  if (c.flags[kDone]) return;
+  __shared__ double ring[3][kRingRows];
+  const int np = c.np;
+  constexpr int kMask = kRingRows - 1;
+  for (int l = 0; l < c.nlf; l++) {
+    const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
+    for (int sl = b + (int)threadIdx.x; sl < e; sl += blockDim.x) {
+      const int i = c.lf_rows[sl];
+      const int cnt = c.f_cnt[sl];
+      double t0 = src[3 * i], t1 = src[3 * i + 1], t2 = src[3 * i + 2];
+#pragma unroll
+      for (int k = 0; k < kMaxBlocksPerRow; k++) {
+        if (k < cnt) {
+          const int j = c.f_col[(size_t)k * np + sl];
+          double y0, y1, y2;
+          if (kRing) {
+            y0 = ring[0][j & kMask];
+            y1 = ring[1][j & kMask];
+            y2 = ring[2][j & kMask];
+          } else {
+            y0 = dst[3 * j];
+            y1 = dst[3 * j + 1];
+            y2 = dst[3 * j + 2];
+          }
+          const double* a = c.f_blk + (size_t)(k * 9) * np + sl;
+          const double a0 = a[0], a1 = a[(size_t)np], a2 = a[2 * (size_t)np];
+          const double a3 = a[3 * (size_t)np], a4 = a[4 * (size_t)np], a5 = a[5 * (size_t)np];
+          const double a6 = a[6 * (size_t)np], a7 = a[7 * (size_t)np], a8 = a[8 * (size_t)np];
+          t0 -= a0 * y0 + a1 * y1 + a2 * y2;
+          t1 -= a3 * y0 + a4 * y1 + a5 * y2;
+          t2 -= a6 * y0 + a7 * y1 + a8 * y2;
+        }
+      }
+      const double* d = c.f_dinv + sl;
+      const double d0 = d[0], d1 = d[(size_t)np], d2 = d[2 * (size_t)np];
+      const double d3 = d[3 * (size_t)np], d4 = d[4 * (size_t)np], d5 = d[5 * (size_t)np];
+      const double d6 = d[6 * (size_t)np], d7 = d[7 * (size_t)np], d8 = d[8 * (size_t)np];
+      const double z0 = d0 * t0 + d1 * t1 + d2 * t2;
+      const double z1 = d3 * t0 + d4 * t1 + d5 * t2;
+      const double z2 = d6 * t0 + d7 * t1 + d8 * t2;
+      if (kRing) {
+        ring[0][i & kMask] = z0;
+        ring[1][i & kMask] = z1;
+        ring[2][i & kMask] = z2;
+      }
+      dst[3 * i] = z0;
+      dst[3 * i + 1] = z1;
+      dst[3 * i + 2] = z2;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < c.nlb; l++) {
+    const int b = c.lb_ptr[l], e = c.lb_ptr[l + 1];
+    for (int sl = b + (int)threadIdx.x; sl < e; sl += blockDim.x) {
+      const int i = c.lb_rows[sl];
+      const int cnt = c.b_cnt[sl];
+      double t0 = dst[3 * i], t1 = dst[3 * i + 1], t2 = dst[3 * i + 2];
+#pragma unroll
+      for (int k = 0; k < kMaxBlocksPerRow; k++) {
+        if (k < cnt) {
+          const int j = c.b_col[(size_t)k * np + sl];
+          double y0, y1, y2;
+          if (kRing) {
+            y0 = ring[0][j & kMask];
+            y1 = ring[1][j & kMask];
+            y2 = ring[2][j & kMask];
+          } else {
+            y0 = dst[3 * j];
+            y1 = dst[3 * j + 1];
+            y2 = dst[3 * j + 2];
+          }
+          const double* a = c.b_blk + (size_t)(k * 9) * np + sl;
+          const double a0 = a[0], a1 = a[(size_t)np], a2 = a[2 * (size_t)np];
+          const double a3 = a[3 * (size_t)np], a4 = a[4 * (size_t)np], a5 = a[5 * (size_t)np];
+          const double a6 = a[6 * (size_t)np], a7 = a[7 * (size_t)np], a8 = a[8 * (size_t)np];
+          t0 -= a0 * y0 + a1 * y1 + a2 * y2;
+          t1 -= a3 * y0 + a4 * y1 + a5 * y2;
+          t2 -= a6 * y0 + a7 * y1 + a8 * y2;
+        }
+      }
+      if (kRing) {
+        ring[0][i & kMask] = t0;
+        ring[1][i & kMask] = t1;
+        ring[2][i & kMask] = t2;
+      }
+      dst[3 * i] = t0;
+      dst[3 * i + 1] = t1;
+      dst[3 * i + 2] = t2;
+    }
+    __syncthreads();
+  }
+  if (prod)
+    for (int e = threadIdx.x; e < 3 * np; e += blockDim.x) prod[e] = src[e] * dst[e];
+}
+
+// one forward+backward substitution launch, packed when available
+static void launch_sweep(const DevCloth& c, const double* src, double* dst, double* prod,
+                         hipStream_t s) {
+  if (c.kl > 0 && c.ring_ok && c.sweep_variant == 0)
+    hipLaunchKernelGGL(k_ilu_apply_packed<true>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
+  else if (c.kl > 0 && c.sweep_variant <= 1)
+    hipLaunchKernelGGL(k_ilu_apply_packed<false>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
+  else
+    hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, src, dst, prod);
+}
+
 // y = alpha A x + beta y over the block-CSR + diagonal (matrix_square_sparse.cpp:218-242)
 __global__ void k_matvec(int np, const int* rowptr, const int* colind, const double* dia,
                          const double* crs, double alpha, const double* x, double beta,
@@ -423,44 +562,15 @@ __global__ void k_pcg_init(DevCloth c, double tol2) {
 }
 
 // ---------------------------------------------------------------------------
-// reductions: the reference sums left to right (matrix_square_sparse.cpp:313-323).
-// The workgroup stages the products through LDS; thread 0 accumulates them in
-// index order, so the result is bit-identical to the sequential loop.
-
-__device__ double ordered_sum(const double* t, int n) {
-  __shared__ double buf[4096];
-  double s = 0.0;
-  for (int base = 0; base < n; base += 4096) {
-    const int m = min(4096, n - base);
-    for (int k = threadIdx.x; k < m; k += blockDim.x) buf[k] = t[base + k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int k = 0;
-      for (; k + 8 <= m; k += 8) {
-        const double a0 = buf[k], a1 = buf[k + 1], a2 = buf[k + 2], a3 = buf[k + 3];
-        const double a4 = buf[k + 4], a5 = buf[k + 5], a6 = buf[k + 6], a7 = buf[k + 7];
-        s += a0;
-        s += a1;
-        s += a2;
-        s += a3;
-        s += a4;
-        s += a5;
-        s += a6;
-        s += a7;
-      }
-      for (; k < m; k++) s += buf[k];
-    }
-    __syncthreads();
-  }
-  return s;
-}
+// reductions: the reference sums left to right (matrix_square_sparse.cpp:313-323);
+// ordered_sum_exact (cs_ordered_sum.hpp) reproduces that loop bit for bit.
 
 // what the PCG does with a finished reduction (ilu_sparse.cpp:648-708)
 enum DotMode : int { kDotRr0 = 0, kDotRz0 = 1, kDotPap = 2, kDotRr = 3, kDotRz = 4, kDotFinal = 5 };
 
-__global__ __launch_bounds__(256) void k_dot(DevCloth c, int mode, int iter) {
+__global__ __launch_bounds__(kSumThreads) void k_dot(DevCloth c, int mode, int iter) {
   if (mode != kDotRr0 && c.flags[kDone]) return;
-  const double s = ordered_sum(c.prod, 3 * c.np);
+  const double s = ordered_sum_exact(c.prod, 3 * c.np);
   if (threadIdx.x != 0) return;
   c.scal[kLastDot] = s;
   switch (mode) {
@@ -563,6 +673,12 @@ int launch_ilu(const DevCloth& c, int* ising, hipStream_t s, Prof* prof) {
   hipLaunchKernelGGL(k_ilu_factor, dim3(1), dim3(1024), 0, s, c, ising);
   hipLaunchKernelGGL(k_ilu_redo, dim3(1), dim3(1024), 0, s, c, ising);
   CS_HIP(hipGetLastError());
+  return launch_pack_factor(c, s);
+}
+
+int launch_pack_factor(const DevCloth& c, hipStream_t s) {
+  if (c.kl > 0) hipLaunchKernelGGL(k_pack_factor, dim3(grid_for(c.np)), dim3(kTB), 0, s, c);
+  CS_HIP(hipGetLastError());
   return kOk;
 }
 
@@ -575,7 +691,7 @@ int launch_matvec(const DevCloth& c, double alpha, const double* x, double beta,
 }
 
 int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStream_t s) {
-  hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, src, dst, (double*)nullptr);
+  launch_sweep(c, src, dst, nullptr, s);
   CS_HIP(hipGetLastError());
   return kOk;
 }
@@ -592,15 +708,15 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   }
   {
     ProfScope ps(prof, kProfReduce);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr0, 0);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRr0, 0);
   }
   {
     ProfScope ps(prof, kProfSweep);
-    hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
+    launch_sweep(c, c.r, c.z, c.prod, s);
   }
   {
     ProfScope ps(prof, kProfReduce);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz0, 0);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRz0, 0);
   }
   CS_HIP(hipGetLastError());
   bool done = false;
@@ -615,7 +731,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     }
     {
       ProfScope ps(prof, kProfReduce);
-      hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotPap, it);
+      hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotPap, it);
     }
     {
       ProfScope ps(prof, kProfAxpy);
@@ -623,15 +739,15 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     }
     {
       ProfScope ps(prof, kProfReduce);
-      hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRr, it);
+      hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRr, it);
     }
     {
       ProfScope ps(prof, kProfSweep);
-      hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, (const double*)c.r, c.z, c.prod);
+      launch_sweep(c, c.r, c.z, c.prod, s);
     }
     {
       ProfScope ps(prof, kProfReduce);
-      hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotRz, it);
+      hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRz, it);
     }
     CS_HIP(hipGetLastError());
     if ((it + 1) % poll == 0 || it + 1 == maxit) {
@@ -644,7 +760,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     // cap reached: report the achieved ratio (ilu_sparse.cpp:706-707)
     hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)c.r);
     ProfScope ps(prof, kProfReduce);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, c, (int)kDotFinal, maxit);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotFinal, maxit);
     CS_HIP(hipGetLastError());
   }
   return kOk;

@@ -10,6 +10,8 @@ int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s);
 int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s, Prof* prof = nullptr);
 // copy A into the factor storage and factorise in place
 int launch_ilu(const DevCloth& c, int* ising, hipStream_t s, Prof* prof = nullptr);
+// rearrange the factor for the packed sweeps (after any change of fdia/fcrs)
+int launch_pack_factor(const DevCloth& c, hipStream_t s);
 int launch_matvec(const DevCloth& c, double alpha, const double* x, double beta, double* y,
                   hipStream_t s);
 int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStream_t s);

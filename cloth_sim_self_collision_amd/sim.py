@@ -120,6 +120,9 @@ class DeviceCloth:
         else:
             check(_lib.load().cs_set_obstacle_values(self.h, dp(f64(pd)), dp(f64(n))))
 
+    def set_option(self, option: int, value: int):
+        check(_lib.load().cs_set_option(self.h, option, value))
+
     # -- the three stages of StepTime
     def step_internal_dynamics_ilu(self):
         conv = C.c_double()
@@ -299,3 +302,11 @@ def selftest_divsqrt(a, b):
     out = np.zeros(2 * len(a))
     check(_lib.load().cs_selftest_divsqrt(len(a), dp(a), dp(b), dp(out)))
     return out[0::2], out[1::2]
+
+
+def ordered_sum(t) -> float:
+    """The device's left-to-right reduction of t (the PCG dot-product sum)."""
+    t = f64(t)
+    out = np.zeros(1)
+    check(_lib.load().cs_selftest_ordered_sum(len(t), dp(t) if len(t) else None, dp(out)))
+    return float(out[0])

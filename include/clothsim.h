@@ -133,6 +133,12 @@ int cs_set_sweep(cs_cloth* c, int n, const int* verts, const double* delta, doub
 int cs_set_time(cs_cloth* c, double t);
 int cs_get_time(const cs_cloth* c, double* t);
 
+/* implementation choices (results are identical for every value):
+ * CS_OPT_SWEEP: 0 automatic, 1 packed factor without the LDS ring,
+ *               2 row-by-row CSR sweep */
+#define CS_OPT_SWEEP 1
+int cs_set_option(cs_cloth* c, int option, int value);
+
 /* per-kernel-class timing with HIP events on the cloth's stream:
  * 0 ILU sweeps, 1 PCG reductions, 2 SpMV, 3 vector updates, 4 ILU factor,
  * 5 assembly, 6 contact pass, 7 other.  Enabling resets the totals. */
@@ -184,7 +190,9 @@ int cs_element_contact(int n, const double* p, double stiff, double clearance, i
                        const double* obstacle, double* dW, double* ddW);
 
 /* host-only: the per-mesh setup of cs_cloth_create without a device.
- * sizes[7] = np, nt, nq, nnz, nnode, forward levels, backward levels; any
+ * sizes[10] = np, nt, nq, nnz, nnode, forward levels, backward levels, max
+ * lower / upper blocks per row of the packed sweeps (0 = not packed), 1 if
+ * the sweep dependencies fit the LDS ring; any
  * output array may be NULL (call once with NULLs to learn the sizes):
  * rowptr[np+1], colind[nnz], diaind[np], nodes[3*nnode] */
 int cs_inspect_setup(const cs_mesh* mesh, int* sizes, int* rowptr, int* colind, int* diaind,
@@ -192,6 +200,8 @@ int cs_inspect_setup(const cs_mesh* mesh, int* sizes, int* rowptr, int* colind, 
 
 /* device arithmetic check: out[2i] = a[i]/b[i], out[2i+1] = sqrt(a[i]) */
 int cs_selftest_divsqrt(int n, const double* a, const double* b, double* out);
+/* the PCG reduction on the device: *out = ((0.0 + t[0]) + t[1]) + ... */
+int cs_selftest_ordered_sum(int n, const double* t, double* out);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,42 @@ def test_device_division_and_sqrt_are_correctly_rounded(sim_mod):
     assert np.array_equal(s[pos], np.sqrt(a[pos]))
 
 
+def _sequential(t):
+    s = 0.0
+    for x in t:
+        s += x
+    return s
+
+
+def test_ordered_sum_is_the_sequential_sum(sim_mod):
+    """The device reduction used by the PCG equals the left-to-right loop
+    bit for bit on adversarial inputs: magnitudes over 2^+-60, exact halves
+    (rounding ties), runs of zeros, subnormals, cancellation, infinities."""
+    rng = np.random.default_rng(11)
+    cases = []
+    for n in (1, 17, 1000, 16384, 16385, 70000, 196608):
+        cases.append(rng.standard_normal(n))
+        cases.append(rng.standard_normal(n) ** 2 * np.exp2(rng.integers(-30, 30, n)))
+        cases.append(np.round(rng.standard_normal(n) * 8) / 4)
+        cases.append(np.concatenate([np.zeros(n // 2), rng.standard_normal(n - n // 2)]))
+        t = rng.standard_normal(n)
+        t[::7] *= -1e10
+        cases.append(t)
+        cases.append(np.exp2(rng.integers(-60, 60, n)).astype(float) * rng.choice([-1, 1], n))
+    cases.append(rng.standard_normal(3000) * np.exp2(rng.integers(-1074, -1000, 3000)))
+    t = rng.standard_normal(5000)
+    t[100] = np.inf
+    cases.append(t)
+    t = t.copy()
+    t[4000] = -np.inf
+    cases.append(t)
+    cases.append(np.zeros(0))
+    for t in cases:
+        got = sim_mod.ordered_sum(t)
+        want = _sequential(t)
+        assert got == want or (np.isnan(got) and np.isnan(want)), (len(t), got, want)
+
+
 def test_element_routines_bitwise(sim_mod, oracle_mod):
     rng = np.random.default_rng(2)
     n = 512
@@ -130,13 +166,15 @@ def test_stages_bitwise(sim_mod, oracle_mod, name, steps, keep):
             assert np.array_equal(bbc, o.refit_ccd(x, vv, cl.dt)), k
 
 
-@pytest.mark.parametrize("name,steps", [("drape32", 100), ("refdemo0", 60), ("refdemo1", 60),
-                                        ("fold24", 60)])
-def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps):
+@pytest.mark.parametrize("name,steps,variant", [("drape32", 100, 0), ("refdemo0", 60, 0),
+                                                ("refdemo1", 60, 0), ("fold24", 60, 0),
+                                                ("drape32", 60, 1), ("fold24", 40, 2)])
+def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant):
     cl = _scenes()[name]
     o = oracle_mod.Oracle(cl, "port")
     x, v = cl.xyz.copy(), cl.uvw.copy()
     with sim_mod.DeviceCloth(cl) as d:
+        d.set_option(1, variant)
         for k in range(steps):
             x_before = x.copy()
             scene.apply_sweep(cl, x, d.time + cl.dt)
