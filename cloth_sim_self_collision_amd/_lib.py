@@ -65,6 +65,7 @@ SIGNATURES = {
     "cs_state_device": (C.c_int, [_V, C.POINTER(_D), C.POINTER(_D)]),
     "cs_set_step_start": (C.c_int, [_V, _D]),
     "cs_set_obstacle_values": (C.c_int, [_V, _D, _D]),
+    "cs_update_velocity": (C.c_int, [_V]),
     "cs_step_internal": (C.c_int, [_V, _D, _I]),
     "cs_contact_resolve": (C.c_int, [_V, _I, C.POINTER(StepStats)]),
     "cs_post_contact_update": (C.c_int, [_V]),

@@ -133,6 +133,9 @@ int build(Cloth* c, const cs_mesh* m) {
   d.nnode = nn;
   d.nlf = S.nlf;
   d.nlb = S.nlb;
+  d.wmax = 0;
+  for (int l = 0; l < S.nlf; l++) d.wmax = std::max(d.wmax, S.lf_ptr[l + 1] - S.lf_ptr[l]);
+  for (int l = 0; l < S.nlb; l++) d.wmax = std::max(d.wmax, S.lb_ptr[l + 1] - S.lb_ptr[l]);
   Arena& A = c->arena;
   int *tri, *quad, *bc, *rowptr, *colind, *diaind, *rowof, *lfp, *lfr, *lbp, *lbr;
   int *csp, *cs_, *dsp, *ds, *gsp, *gs, *rp, *rg, *node, *pre, *pree, *bypre, *leafof, *rord;
@@ -181,6 +184,13 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&d.p[0], 3 * (size_t)np);
   A.reserve(&d.p[1], 3 * (size_t)np);
   A.reserve(&d.prod, 3 * (size_t)np);
+  const size_t ntile = (size_t)sum_tiles(3 * np);
+  A.reserve(&d.sum.tsum, ntile);
+  A.reserve(&d.sum.code, ntile);
+  A.reserve(&d.sum.tot, ntile);
+  A.reserve(&d.sum.pmin, ntile);
+  A.reserve(&d.sum.pmax, ntile);
+  A.reserve(&d.sum.ticket, 1);
   A.reserve(&d.scal, kNumScal);
   A.reserve(&d.flags, kNumFlag);
   A.reserve(&d.bb, 6 * (size_t)nn);
@@ -231,6 +241,7 @@ int build(Cloth* c, const cs_mesh* m) {
   d.b_cnt = bcnt;
   d.b_col = bcol;
   hipStream_t s = c->stream;
+  CS_HIP(hipMemsetAsync(d.sum.ticket, 0, sizeof(unsigned int), s));
   CS_HIP(hipMemcpyAsync(tri, m->tri, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, s));
   if (nq) CS_HIP(hipMemcpyAsync(quad, m->quad, sizeof(int) * 4 * nq, hipMemcpyHostToDevice, s));
   std::vector<int> bcv(np, 0);
@@ -473,9 +484,20 @@ int cs_post_contact_update(cs_cloth* p) {
   return sync(c);
 }
 
+int cs_update_velocity(cs_cloth* p) {
+  if (!p) return kBadArg;
+  Cloth* c = H(p);
+  if (int st = update_velocity(c->d, c->P, c->stream)) return st;
+  return sync(c);
+}
+
 int cs_step(cs_cloth* p, int n, cs_step_stats* stats) {
   if (!p || n < 0) return kBadArg;
   Cloth* c = H(p);
+  if (c->d.host_pd) {
+    set_error("cs_step: host obstacle values describe one state; use cs_step_internal");
+    return kBadArg;
+  }
   for (int k = 0; k < n; k++) {
     cs_step_stats st_local;
     std::memset(&st_local, 0, sizeof(st_local));
@@ -529,7 +551,7 @@ int cs_set_sweep(cs_cloth* p, int n, const int* verts, const double* delta, doub
 int cs_set_option(cs_cloth* p, int option, int value) {
   if (!p) return kBadArg;
   Cloth* c = H(p);
-  if (option == CS_OPT_SWEEP && value >= 0 && value <= 2) {
+  if (option == CS_OPT_SWEEP && value >= 0 && value <= 5) {
     c->d.sweep_variant = value;
     return kOk;
   }

@@ -977,12 +977,17 @@ int contact_resolve(ContactState& cs, const MeshSetup& S, const DevCloth& d, con
   return kOk;
 }
 
-int post_contact_update(const DevCloth& d, const Params& P, hipStream_t s) {
-  hipLaunchKernelGGL(k_advance, dim3(grid_for(3L * d.np)), dim3(kTB), 0, s, d, P.dt);
+int update_velocity(const DevCloth& d, const Params& P, hipStream_t s) {
   if (int st = launch_gradients(d, P, s)) return st;
   hipLaunchKernelGGL(k_velocity_update, dim3(grid_for(d.np)), dim3(kTB), 0, s, d, P);
   CS_HIP(hipGetLastError());
   return kOk;
+}
+
+int post_contact_update(const DevCloth& d, const Params& P, hipStream_t s) {
+  hipLaunchKernelGGL(k_advance, dim3(grid_for(3L * d.np)), dim3(kTB), 0, s, d, P.dt);
+  CS_HIP(hipGetLastError());
+  return update_velocity(d, P, s);
 }
 
 int contact_refit_host(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStream_t s,

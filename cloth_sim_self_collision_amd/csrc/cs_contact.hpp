@@ -57,6 +57,8 @@ void contact_release(ContactState& cs);
 int contact_resolve(ContactState& cs, const MeshSetup& S, const DevCloth& d, const Params& P,
                     hipStream_t s, int* applied, cs_step_stats* stats, Prof* prof = nullptr);
 // x = x1 + v dt, then UpdateIntermidiateVelocity
+// UpdateIntermidiateVelocity (solve_internal_sparse.h:329-369) at the current x
+int update_velocity(const DevCloth& d, const Params& P, hipStream_t s);
 int post_contact_update(const DevCloth& d, const Params& P, hipStream_t s);
 // host-array helpers of the C-ABI (uvw == NULL -> proximity)
 int contact_refit_host(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStream_t s,

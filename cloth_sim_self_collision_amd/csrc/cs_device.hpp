@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "cs_ordered_sum.hpp"
 #include "cs_setup.hpp"
 
 namespace cs {
@@ -116,13 +117,15 @@ struct DevCloth {
   double* ap;
   double* p[2];
   double* prod;  // element-wise products feeding the next reduction
+  SumScratch sum;  // tile summaries of the ordered reductions over prod
   double* scal;
   int* flags;
   // BVH boxes (6 per node)
   double* bb;
   // packed sweeps (null when a row has more than kMaxBlocksPerRow blocks)
   int kl, ku, ring_ok;
-  int sweep_variant;  // 0 automatic, 1 packed without the LDS ring, 2 row-by-row
+  int wmax;           // widest level of either sweep
+  int sweep_variant;  // 0 auto (pipelined), 1 packed + ring, 2 packed, 3 row-by-row, 4/5 pipelined 2/1 teams
   const int* f_cnt;
   const int* f_col;
   const int* b_cnt;

@@ -60,11 +60,6 @@ __global__ void k_divsqrt(int n, const double* a, const double* b, double* out) 
   out[2 * e + 1] = sqrt(a[e]);
 }
 
-__global__ __launch_bounds__(kSumThreads) void k_ordered_sum(const double* t, int n, double* out) {
-  const double s = ordered_sum_exact(t, n);
-  if (threadIdx.x == 0) out[0] = s;
-}
-
 namespace {
 
 // device round trip for the element entry points
@@ -157,9 +152,14 @@ extern "C" int cs_selftest_ordered_sum(int n, const double* t, double* out) {
   if (n < 0 || (n > 0 && !t) || !out) return kBadArg;
   Scratch s;
   double *dt, *dout;
-  if (s.get(&dt, n) || s.get(&dout, 1)) return kHipError;
+  SumScratch w;
+  const int nt = sum_tiles(n);
+  if (s.get(&dt, n) || s.get(&dout, 1) || s.get(&w.tsum, nt) || s.get(&w.code, nt) ||
+      s.get(&w.tot, nt) || s.get(&w.pmin, nt) || s.get(&w.pmax, nt) || s.get(&w.ticket, 1))
+    return kHipError;
+  CS_HIP(hipMemset(w.ticket, 0, sizeof(unsigned int)));
   if (n) CS_HIP(hipMemcpy(dt, t, sizeof(double) * n, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_ordered_sum, dim3(1), dim3(kSumThreads), 0, 0, dt, n, dout);
+  launch_ordered_sum(dt, n, w, nullptr, SumStore{dout}, 0);
   CS_HIP(hipGetLastError());
   CS_HIP(hipMemcpy(out, dout, sizeof(double), hipMemcpyDeviceToHost));
   return kOk;

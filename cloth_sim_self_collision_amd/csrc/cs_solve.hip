@@ -443,15 +443,178 @@ __global__ __launch_bounds__(512) void k_ilu_apply_packed(DevCloth c, const doub
     for (int e = threadIdx.x; e < 3 * np; e += blockDim.x) prod[e] = src[e] * dst[e];
 }
 
-// one forward+backward substitution launch, packed when available
+// ---------------------------------------------------------------------------
+// level-pipelined packed sweep.  The 512 threads form kTeams teams; team
+// (l mod kTeams) computes level l, one row per thread, and right after that
+// issues the loads of everything level l + kTeams needs that does not depend
+// on the sweep itself: the right-hand side, column indices, factor blocks and
+// inverted diagonal.  Those loads then have kTeams - 1 level times to arrive,
+// and the row index they use was itself loaded one round earlier, so no load
+// waits on another inside a round.  At compute time only the dependency
+// values are read, from the LDS ring.  Per row the arithmetic is exactly that
+// of k_ilu_apply (same blocks, same column order, same expression shape);
+// blocks past the row's count are loaded (the packing pads to the widest row)
+// but never enter the arithmetic.
+constexpr int kPipeThreads = 512;
+constexpr int kPipeLevels = 2048;  // level-pointer capacity kept in LDS
+
+struct PipeRow {
+  int cnt;
+  int i;  // -1: no row for this thread at this level
+  double t0, t1, t2;
+  int col[kMaxBlocksPerRow];
+  double a[kMaxBlocksPerRow * 9];
+  double d[9];
+};
+
+template <bool kFwd>
+__device__ __forceinline__ void pipe_load(const DevCloth& c, int K, int sl, int i,
+                                          const double* __restrict__ rhs, PipeRow& r) {
+  const int np = c.np;
+  r.i = i;
+  if (i < 0) return;
+  r.cnt = kFwd ? c.f_cnt[sl] : c.b_cnt[sl];
+  r.t0 = rhs[3 * i];
+  r.t1 = rhs[3 * i + 1];
+  r.t2 = rhs[3 * i + 2];
+  const int* col = kFwd ? c.f_col : c.b_col;
+  const double* blk = kFwd ? c.f_blk : c.b_blk;
+#pragma unroll
+  for (int k = 0; k < kMaxBlocksPerRow; k++) {
+    if (k < K) {
+      r.col[k] = col[(size_t)k * np + sl];
+#pragma unroll
+      for (int q = 0; q < 9; q++) r.a[k * 9 + q] = blk[(size_t)(k * 9 + q) * np + sl];
+    }
+  }
+  if (kFwd) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) r.d[q] = c.f_dinv[(size_t)q * np + sl];
+  }
+}
+
+template <bool kFwd>
+__device__ __forceinline__ void pipe_compute(int K, double (*ring)[kRingRows], double* dst,
+                                             PipeRow& r) {
+  constexpr int kMask = kRingRows - 1;
+  if (r.i < 0) return;
+  double y[kMaxBlocksPerRow][3];
+#pragma unroll
+  for (int k = 0; k < kMaxBlocksPerRow; k++) {
+    if (k < K) {
+      const int j = r.col[k] & kMask;
+      y[k][0] = ring[0][j];
+      y[k][1] = ring[1][j];
+      y[k][2] = ring[2][j];
+    }
+  }
+  double t0 = r.t0, t1 = r.t1, t2 = r.t2;
+#pragma unroll
+  for (int k = 0; k < kMaxBlocksPerRow; k++) {
+    if (k < K && k < r.cnt) {
+      const double* a = r.a + k * 9;
+      t0 -= a[0] * y[k][0] + a[1] * y[k][1] + a[2] * y[k][2];
+      t1 -= a[3] * y[k][0] + a[4] * y[k][1] + a[5] * y[k][2];
+      t2 -= a[6] * y[k][0] + a[7] * y[k][1] + a[8] * y[k][2];
+    }
+  }
+  double z0 = t0, z1 = t1, z2 = t2;
+  if (kFwd) {
+    const double* d = r.d;
+    z0 = d[0] * t0 + d[1] * t1 + d[2] * t2;
+    z1 = d[3] * t0 + d[4] * t1 + d[5] * t2;
+    z2 = d[6] * t0 + d[7] * t1 + d[8] * t2;
+  }
+  const int i = r.i;
+  ring[0][i & kMask] = z0;
+  ring[1][i & kMask] = z1;
+  ring[2][i & kMask] = z2;
+  dst[3 * i] = z0;
+  dst[3 * i + 1] = z1;
+  dst[3 * i + 2] = z2;
+}
+
+// Workgroup barrier that orders LDS only: the ring writes of a level are
+// visible after it, while the prefetch loads issued before it stay in
+// flight (a full __syncthreads() would wait for them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <bool kFwd, int kTeams>
+__device__ __forceinline__ void pipe_sweep(const DevCloth& c, const int* lptr, int nl,
+                                           const int* __restrict__ rows, const double* rhs,
+                                           double* dst, double (*ring)[kRingRows]) {
+  constexpr int kTeam = kPipeThreads / kTeams;
+  const int team = threadIdx.x / kTeam, lt = threadIdx.x - team * kTeam;
+  const int K = kFwd ? c.kl : c.ku;
+  auto row_at = [&](int l, int& sl) -> int {
+    if (l >= nl) return -1;
+    sl = lptr[l] + lt;
+    return sl < lptr[l + 1] ? rows[sl] : -1;
+  };
+  PipeRow r;
+  int sl = 0, sl_next = 0;
+  {
+    const int i = row_at(team, sl);
+    pipe_load<kFwd>(c, K, sl, i, rhs, r);
+  }
+  int i_next = row_at(team + kTeams, sl_next);
+  for (int l = 0; l < nl; l++) {
+    if (l % kTeams == team) {
+      pipe_compute<kFwd>(K, ring, dst, r);
+      pipe_load<kFwd>(c, K, sl_next, i_next, rhs, r);
+      i_next = row_at(l + 2 * kTeams, sl_next);
+    }
+    lds_barrier();
+  }
+}
+
+template <int kTeams>
+__global__ __launch_bounds__(kPipeThreads) void k_ilu_apply_piped(DevCloth c,
+                                                                  const double* __restrict__ src,
+                                                                  double* dst, double* prod) {
+  if (c.flags[kDone]) return;
+  __shared__ double ring[3][kRingRows];
+  __shared__ int lfp[kPipeLevels + 1];
+  __shared__ int lbp[kPipeLevels + 1];
+  for (int l = threadIdx.x; l <= c.nlf; l += blockDim.x) lfp[l] = c.lf_ptr[l];
+  for (int l = threadIdx.x; l <= c.nlb; l += blockDim.x) lbp[l] = c.lb_ptr[l];
+  __syncthreads();
+  pipe_sweep<true, kTeams>(c, lfp, c.nlf, c.lf_rows, src, dst, ring);
+  __syncthreads();  // the backward sweep reads the complete forward result from dst
+  pipe_sweep<false, kTeams>(c, lbp, c.nlb, c.lb_rows, dst, dst, ring);
+  __syncthreads();
+  if (prod)
+    for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src[e] * dst[e];
+}
+
+// one forward+backward substitution launch (variants give identical results)
 static void launch_sweep(const DevCloth& c, const double* src, double* dst, double* prod,
                          hipStream_t s) {
-  if (c.kl > 0 && c.ring_ok && c.sweep_variant == 0)
+  const bool packed = c.kl > 0;
+  const bool piped = packed && c.ring_ok && c.wmax <= kPipeThreads && c.nlf <= kPipeLevels &&
+                     c.nlb <= kPipeLevels;
+  if (c.sweep_variant == 4 && piped && c.wmax <= kPipeThreads / 2) {
+    hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+  } else if (c.sweep_variant == 5 && piped) {
+    hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+  } else if ((c.sweep_variant == 0 || c.sweep_variant >= 4) && piped) {
+    if (c.wmax <= kPipeThreads / 4)
+      hipLaunchKernelGGL(k_ilu_apply_piped<4>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+    else if (c.wmax <= kPipeThreads / 2)
+      hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+    else
+      hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+  } else if (packed && c.ring_ok && c.sweep_variant <= 1) {
     hipLaunchKernelGGL(k_ilu_apply_packed<true>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
-  else if (c.kl > 0 && c.sweep_variant <= 1)
+  } else if (packed && c.sweep_variant <= 2) {
     hipLaunchKernelGGL(k_ilu_apply_packed<false>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
-  else
+  } else {
     hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, src, dst, prod);
+  }
 }
 
 // y = alpha A x + beta y over the block-CSR + diagonal (matrix_square_sparse.cpp:218-242)
@@ -568,46 +731,56 @@ __global__ void k_pcg_init(DevCloth c, double tol2) {
 // what the PCG does with a finished reduction (ilu_sparse.cpp:648-708)
 enum DotMode : int { kDotRr0 = 0, kDotRz0 = 1, kDotPap = 2, kDotRr = 3, kDotRz = 4, kDotFinal = 5 };
 
-__global__ __launch_bounds__(kSumThreads) void k_dot(DevCloth c, int mode, int iter) {
-  if (mode != kDotRr0 && c.flags[kDone]) return;
-  const double s = ordered_sum_exact(c.prod, 3 * c.np);
-  if (threadIdx.x != 0) return;
-  c.scal[kLastDot] = s;
-  switch (mode) {
-    case kDotRr0:
-      if (s < 1.0e-30) {
-        c.scal[kConv] = 0.0;
-        c.flags[kIters] = 0;
-        c.flags[kDone] = 1;
-      } else {
-        c.scal[kInvRr0] = 1.0 / s;
+// PCG bookkeeping on the exact dot product (ilu_sparse.cpp:646-700)
+struct DotFinish {
+  DevCloth c;
+  int mode;
+  int iter;
+  __device__ void operator()(double s) const {
+    c.scal[kLastDot] = s;
+    switch (mode) {
+      case kDotRr0:
+        if (s < 1.0e-30) {
+          c.scal[kConv] = 0.0;
+          c.flags[kIters] = 0;
+          c.flags[kDone] = 1;
+        } else {
+          c.scal[kInvRr0] = 1.0 / s;
+        }
+        break;
+      case kDotRz0:
+        c.scal[kRz] = s;
+        break;
+      case kDotPap:
+        c.scal[kAlpha] = c.scal[kRz] / s;
+        break;
+      case kDotRr: {
+        const double q = s * c.scal[kInvRr0];
+        if (q < c.scal[kTol2]) {
+          c.scal[kConv] = sqrt(q);
+          c.flags[kIters] = iter;
+          c.flags[kDone] = 1;
+        }
+        break;
       }
-      break;
-    case kDotRz0:
-      c.scal[kRz] = s;
-      break;
-    case kDotPap:
-      c.scal[kAlpha] = c.scal[kRz] / s;
-      break;
-    case kDotRr: {
-      const double q = s * c.scal[kInvRr0];
-      if (q < c.scal[kTol2]) {
-        c.scal[kConv] = sqrt(q);
+      case kDotRz: {
+        c.scal[kBeta] = s / c.scal[kRz];
+        c.scal[kRz] = s;
+        break;
+      }
+      case kDotFinal:
+        c.scal[kConv] = sqrt(s * c.scal[kInvRr0]);
         c.flags[kIters] = iter;
-        c.flags[kDone] = 1;
-      }
-      break;
+        break;
     }
-    case kDotRz: {
-      c.scal[kBeta] = s / c.scal[kRz];
-      c.scal[kRz] = s;
-      break;
-    }
-    case kDotFinal:
-      c.scal[kConv] = sqrt(s * c.scal[kInvRr0]);
-      c.flags[kIters] = iter;
-      break;
   }
+};
+
+// exact left-to-right sum of prod, then the PCG bookkeeping of `mode`; every
+// mode but the first is skipped once the solve has converged
+static void launch_dot(const DevCloth& c, int mode, int iter, hipStream_t s) {
+  const int* skip = mode == kDotRr0 ? nullptr : c.flags + kDone;
+  launch_ordered_sum(c.prod, 3 * c.np, c.sum, skip, DotFinish{c, mode, iter}, s);
 }
 
 __global__ void k_square(DevCloth c, const double* v) {
@@ -708,7 +881,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   }
   {
     ProfScope ps(prof, kProfReduce);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRr0, 0);
+    launch_dot(c, kDotRr0, 0, s);
   }
   {
     ProfScope ps(prof, kProfSweep);
@@ -716,7 +889,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   }
   {
     ProfScope ps(prof, kProfReduce);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRz0, 0);
+    launch_dot(c, kDotRz0, 0, s);
   }
   CS_HIP(hipGetLastError());
   bool done = false;
@@ -731,7 +904,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     }
     {
       ProfScope ps(prof, kProfReduce);
-      hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotPap, it);
+      launch_dot(c, kDotPap, it, s);
     }
     {
       ProfScope ps(prof, kProfAxpy);
@@ -739,7 +912,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     }
     {
       ProfScope ps(prof, kProfReduce);
-      hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRr, it);
+      launch_dot(c, kDotRr, it, s);
     }
     {
       ProfScope ps(prof, kProfSweep);
@@ -747,7 +920,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     }
     {
       ProfScope ps(prof, kProfReduce);
-      hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotRz, it);
+      launch_dot(c, kDotRz, it, s);
     }
     CS_HIP(hipGetLastError());
     if ((it + 1) % poll == 0 || it + 1 == maxit) {
@@ -760,7 +933,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     // cap reached: report the achieved ratio (ilu_sparse.cpp:706-707)
     hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)c.r);
     ProfScope ps(prof, kProfReduce);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(kSumThreads), 0, s, c, (int)kDotFinal, maxit);
+    launch_dot(c, kDotFinal, maxit, s);
     CS_HIP(hipGetLastError());
   }
   return kOk;

@@ -137,7 +137,14 @@ class DeviceCloth:
         return bool(applied.value), st.as_dict()
 
     def update_intermediate_velocity(self):
+        """x = x_start + v dt, then UpdateIntermidiateVelocity."""
         check(_lib.load().cs_post_contact_update(self.h))
+
+    def update_velocity(self, uvw, xyz):
+        """UpdateIntermidiateVelocity alone at positions xyz; returns v."""
+        self.set_state(xyz, uvw)
+        check(_lib.load().cs_update_velocity(self.h))
+        return self.get_state()[1]
 
     @property
     def time(self) -> float:

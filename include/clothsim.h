@@ -106,7 +106,8 @@ int cs_state_device(cs_cloth* c, double** xyz, double** uvw);
  * (cs_step_internal stores them itself) */
 int cs_set_step_start(cs_cloth* c, const double* xyz1);
 /* per-vertex obstacle values from a host penetrationDepth callback: pd[np],
- * n[3*np]; NULL restores the analytic obstacle of the parameters */
+ * n[3*np], used by every assembly until cleared (cs_step refuses them, since
+ * they describe one state); NULL restores the analytic obstacle */
 int cs_set_obstacle_values(cs_cloth* c, const double* pd, const double* n);
 
 /* ---- the three calls of StepTime (self_contact_sparse/main.cpp:213-263) ---- */
@@ -122,6 +123,9 @@ int cs_contact_resolve(cs_cloth* c, int* impulse_applied, cs_step_stats* stats);
  * (solve_internal_sparse.h:329-369); StepTime does this only when an impulse
  * was applied. */
 int cs_post_contact_update(cs_cloth* c);
+/* UpdateIntermidiateVelocity alone (solve_internal_sparse.h:329-369): v -=
+ * dt/(2m) grad W(x) at the current positions (set them with cs_set_state) */
+int cs_update_velocity(cs_cloth* c);
 /* n full StepTime iterations; stats of the last step (may be NULL).  Before
  * each step the swept vertices (cs_set_sweep) are placed at
  * rest + delta * clamp((t + dt) / duration, 0, 1); t then advances by dt. */
@@ -134,8 +138,10 @@ int cs_set_time(cs_cloth* c, double t);
 int cs_get_time(const cs_cloth* c, double* t);
 
 /* implementation choices (results are identical for every value):
- * CS_OPT_SWEEP: 0 automatic, 1 packed factor without the LDS ring,
- *               2 row-by-row CSR sweep */
+ * CS_OPT_SWEEP: 0 automatic (level-pipelined packed sweep when the
+ *               mesh allows it), 1 packed factor with the LDS ring,
+ *               2 packed factor without the ring, 3 row-by-row CSR sweep,
+ *               4 / 5 the pipelined sweep with 2 / 1 thread teams */
 #define CS_OPT_SWEEP 1
 int cs_set_option(cs_cloth* c, int option, int value);
 

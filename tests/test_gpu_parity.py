@@ -81,12 +81,18 @@ def test_ordered_sum_is_the_sequential_sum(sim_mod):
         t[::7] *= -1e10
         cases.append(t)
         cases.append(np.exp2(rng.integers(-60, 60, n)).astype(float) * rng.choice([-1, 1], n))
+        # mixed signs decaying over 600 binades: dense binade changes early,
+        # then addends far below the running sum's ulp
+        cases.append(rng.standard_normal(n) * np.exp2(np.linspace(300, -300, n)))
     cases.append(rng.standard_normal(3000) * np.exp2(rng.integers(-1074, -1000, 3000)))
     t = rng.standard_normal(5000)
     t[100] = np.inf
     cases.append(t)
     t = t.copy()
     t[4000] = -np.inf
+    cases.append(t)
+    t = rng.standard_normal(9000)
+    t[3000] = np.nan
     cases.append(t)
     cases.append(np.zeros(0))
     for t in cases:
@@ -166,9 +172,29 @@ def test_stages_bitwise(sim_mod, oracle_mod, name, steps, keep):
             assert np.array_equal(bbc, o.refit_ccd(x, vv, cl.dt)), k
 
 
+def test_velocity_update_and_host_obstacle_values(sim_mod, oracle_mod):
+    cl = _scenes()["drape32"]
+    o, states = _trajectory(oracle_mod, cl, 40, (39,))
+    x, v = states[39]
+    with sim_mod.DeviceCloth(cl) as d:
+        assert np.array_equal(d.update_velocity(v, x), o.update_velocity(v, x))
+        dia0, crs0, b0 = d.assemble(x, v)
+        # penetrationDepth evaluated on the host, as the reference's callback
+        pd, n = cl.obstacle.depth(x)
+        d.set_obstacle_values(pd, n.ravel())
+        with pytest.raises(sim_mod._lib.ClothSimError):
+            d.step(1)  # host values describe one state only
+        dia1, crs1, b1 = d.assemble(x, v)
+        d.set_obstacle_values()
+        assert np.array_equal(dia0, dia1) and np.array_equal(crs0, crs1) and np.array_equal(b0, b1)
+        assert (pd + cl.clearance > 0).any()  # the obstacle term is active
+
+
 @pytest.mark.parametrize("name,steps,variant", [("drape32", 100, 0), ("refdemo0", 60, 0),
                                                 ("refdemo1", 60, 0), ("fold24", 60, 0),
-                                                ("drape32", 60, 1), ("fold24", 40, 2)])
+                                                ("drape32", 60, 1), ("fold24", 40, 2),
+                                                ("fold24", 40, 3), ("drape32", 40, 4),
+                                                ("fold24", 30, 5)])
 def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant):
     cl = _scenes()[name]
     o = oracle_mod.Oracle(cl, "port")
@@ -185,3 +211,36 @@ def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant):
             assert np.array_equal(gx, x) and np.array_equal(gv, v), (
                 k, np.abs(gx - x).max(), np.abs(gv - v).max(), st)
             del x_before
+
+
+def test_cpp_shim_step_loop_matches_reference(sim_mod, oracle_mod, tmp_path):
+    """The reference's StepTime written against include/clothsim_ref.hpp
+    (tests/shim/step_loop.cpp, a separate process) reproduces the oracle's
+    trajectory bit for bit, obstacle given as a host penetrationDepth callback."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "shim", "bin", "step_loop")
+    if not os.path.exists(exe):
+        pytest.fail("tests/shim/bin/step_loop is not built (make -C tests/shim)")
+    cl = _scenes()["drape16"]
+    steps = 30
+    m = cl.material
+    hdr = np.array([cl.np_, cl.nt, cl.nq, steps, cl.pcg_maxit], np.int32)
+    prm = np.array([cl.dt, m.lam, m.mu, m.stiff_bend, *m.gravity, cl.mass_point, m.stiff_contact,
+                    cl.clearance, cl.pcg_tol, *cl.obstacle.params], np.float64)
+    assert cl.obstacle.kind == scene.OBST_SPHERE and len(prm) == 15
+    src = tmp_path / "in.bin"
+    with open(src, "wb") as f:
+        for a in (hdr, prm, cl.xyz0, cl.xyz, cl.uvw):
+            f.write(np.ascontiguousarray(a).tobytes())
+        for a in (cl.tri, cl.quad, cl.bc):
+            f.write(np.ascontiguousarray(a, np.int32).tobytes())
+    out = tmp_path / "out.bin"
+    r = subprocess.run([exe, str(src), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = np.fromfile(out, np.float64)
+    o = oracle_mod.Oracle(cl, "port")
+    x, v = cl.xyz.copy(), cl.uvw.copy()
+    for _ in range(steps):
+        x, v, *_ = o.step(x, v)
+    assert np.array_equal(res[:3 * cl.np_], x) and np.array_equal(res[3 * cl.np_:], v)
