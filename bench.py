@@ -32,6 +32,10 @@ sys.path.insert(0, ROOT)
 from cloth_sim_self_collision_amd import scene  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc, each
+# scaled by what the counter reads for k_stream_copy of known size), keyed by
+# (workload, kernel class); the passes and the scaling are in profiles/
+TRAFFIC_BYTES: dict = {}
 
 
 def workload(name: str) -> scene.Cloth:
@@ -91,6 +95,10 @@ def main() -> int:
     ap.add_argument("--workload", default="fold256")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--maxit", type=int, default=None,
+                    help="PCG iteration cap override (short profiling runs only)")
+    ap.add_argument("--copy-mb", type=float, default=512.0,
+                    help="size of the streaming-copy probe that measures the copy peak")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -103,7 +111,12 @@ def main() -> int:
         print("no HIP device", file=sys.stderr)
         return 1
 
+    from cloth_sim_self_collision_amd.sim import bandwidth_probe
+    n_copy = int(args.copy_mb * 2**20 / 8) // 2 * 2
+    copy_gbs = bandwidth_probe(n_copy, 10) if n_copy > 0 else None
     cloth = workload(args.workload)
+    if args.maxit is not None:
+        cloth.pcg_maxit = args.maxit
     d = DeviceCloth(cloth)
     # warmup; the first step is also compared against the CPU reference
     first_x = None
@@ -140,9 +153,12 @@ def main() -> int:
         avg_s = ms / n * 1e-3
         ach = b / avg_s / 1e9 if b > 0 else None
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": None,
+                "unit": "GB/s", "frac": (ach / HBM_PEAK_GBS) if ach else None,
+                "traffic": TRAFFIC_BYTES.get((cloth.name, dom)),
                 "bytes_per_launch": b, "avg_launch_us": avg_s * 1e6, "launches": n,
-                "share_of_gpu_time": ms / sum(v[0] for v in kinds.values())}
+                "share_of_gpu_time": ms / sum(v[0] for v in kinds.values()),
+                "stream_copy_gbs": copy_gbs,
+                "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None}
     cpu, err = (None, None) if args.no_cpu else cpu_baseline(workload(args.workload), first_x)
     out = {
         "metric": "vertex-steps/s + p99 step latency, served vs plain GPU, at <=1e-5 m pos err",

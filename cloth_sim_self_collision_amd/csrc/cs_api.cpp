@@ -240,6 +240,7 @@ int build(Cloth* c, const cs_mesh* m) {
   d.f_col = fcol;
   d.b_cnt = bcnt;
   d.b_col = bcol;
+  d.ising = c->ising;
   hipStream_t s = c->stream;
   CS_HIP(hipMemsetAsync(d.sum.ticket, 0, sizeof(unsigned int), s));
   CS_HIP(hipMemcpyAsync(tri, m->tri, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, s));
@@ -307,7 +308,7 @@ int internal_step(Cloth* c, double* conv, int* iters) {
   c->have_x1 = true;
   Prof* prof = c->prof.on ? &c->prof : nullptr;
   if (int st = launch_assemble(d, c->P, s, prof)) return st;
-  if (int st = launch_ilu(d, c->ising, s, prof)) return st;
+  if (int st = launch_ilu(d, s, prof)) return st;
   if (int st = run_pcg(d, c->P.pcg_tol, c->P.pcg_maxit, c->hflags, s, c->poll, prof)) return st;
   if (int st = launch_finish_internal(d, c->P, s)) return st;
   CS_HIP(hipMemcpyAsync(c->hflags, d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
@@ -634,7 +635,7 @@ int cs_ilu_factor(cs_cloth* p, int* n_singular) {
   if (!p) return kBadArg;
   Cloth* c = H(p);
   CS_HIP(hipMemsetAsync(c->d.flags, 0, sizeof(int) * kNumFlag, c->stream));
-  if (int st = launch_ilu(c->d, c->ising, c->stream)) return st;
+  if (int st = launch_ilu(c->d, c->stream)) return st;
   CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, c->stream));
   if (int st = sync(c)) return st;
   if (n_singular) *n_singular = c->hflags[kNsing];
@@ -768,3 +769,174 @@ int cs_query_ccd(cs_cloth* p, const double* xyz, const double* uvw, double dt, d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches: many cloths stepped by shared launches
+
+namespace cs {
+
+struct Batch {
+  std::vector<Cloth*> cl;
+  DevCloth* dtab = nullptr;  // device copies of the cloths' DevCloth / Params
+  Params* ptab = nullptr;
+  int* dgather = nullptr;
+  int* hgather = nullptr;  // pinned
+  hipStream_t stream = nullptr;
+  BatchView view{};
+  double tol = 0.0;
+  int maxit = 0;
+  int poll = 8;
+  std::vector<DevCloth> htab;
+  std::vector<Params> hptab;
+};
+
+namespace {
+
+Batch* HB(cs_batch* p) { return reinterpret_cast<Batch*>(p); }
+
+void batch_free(Batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->dtab) (void)hipFree(b->dtab);
+  if (b->ptab) (void)hipFree(b->ptab);
+  if (b->dgather) (void)hipFree(b->dgather);
+  if (b->hgather) (void)hipHostFree(b->hgather);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// the member cloths' current device views and parameters into the tables
+int batch_refresh(Batch* b) {
+  for (size_t i = 0; i < b->cl.size(); i++) {
+    b->htab[i] = b->cl[i]->d;
+    b->hptab[i] = b->cl[i]->P;
+  }
+  const size_t n = b->cl.size();
+  CS_HIP(hipMemcpyAsync(b->dtab, b->htab.data(), sizeof(DevCloth) * n, hipMemcpyHostToDevice,
+                        b->stream));
+  CS_HIP(hipMemcpyAsync(b->ptab, b->hptab.data(), sizeof(Params) * n, hipMemcpyHostToDevice,
+                        b->stream));
+  return kOk;
+}
+
+}  // namespace
+}  // namespace cs
+
+extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
+  using namespace cs;
+  if (n < 1 || !cloths || !out) return kBadArg;
+  *out = nullptr;
+  Batch* b = new Batch();
+  for (int i = 0; i < n; i++) {
+    if (!cloths[i]) {
+      delete b;
+      set_error("cs_batch_create: NULL cloth");
+      return kBadArg;
+    }
+    Cloth* c = H(cloths[i]);
+    for (Cloth* o : b->cl)
+      if (o == c) {
+        delete b;
+        set_error("cs_batch_create: a cloth appears twice");
+        return kBadArg;
+      }
+    b->cl.push_back(c);
+  }
+  b->tol = b->cl[0]->P.pcg_tol;
+  b->maxit = b->cl[0]->P.pcg_maxit;
+  BatchView& v = b->view;
+  v.n = n;
+  v.piped = true;
+  for (Cloth* c : b->cl) {
+    if (c->P.pcg_tol != b->tol || c->P.pcg_maxit != b->maxit) {
+      delete b;
+      set_error("cs_batch_create: the cloths must share the PCG tolerance and iteration cap");
+      return kBadArg;
+    }
+    const DevCloth& d = c->d;
+    v.max_np = std::max(v.max_np, d.np);
+    v.max_nt = std::max(v.max_nt, d.nt);
+    v.max_nq = std::max(v.max_nq, d.nq);
+    v.max_nnz = std::max(v.max_nnz, d.nnz);
+    v.max_wmax = std::max(v.max_wmax, d.wmax);
+    v.piped = v.piped && d.kl > 0 && d.ring_ok && d.wmax <= 512 && d.nlf <= 2048 && d.nlb <= 2048;
+  }
+  b->htab.resize(n);
+  b->hptab.resize(n);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&b->dtab, sizeof(DevCloth) * n) != hipSuccess ||
+      hipMalloc((void**)&b->ptab, sizeof(Params) * n) != hipSuccess ||
+      hipMalloc((void**)&b->dgather, sizeof(int) * n) != hipSuccess ||
+      hipHostMalloc((void**)&b->hgather, sizeof(int) * n) != hipSuccess) {
+    batch_free(b);
+    set_error("cs_batch_create: device allocation failed");
+    return kOutOfMemory;
+  }
+  v.tab = b->dtab;
+  v.ptab = b->ptab;
+  *out = reinterpret_cast<cs_batch*>(b);
+  return kOk;
+}
+
+extern "C" int cs_batch_destroy(cs_batch* p) {
+  if (!p) return cs::kOk;
+  cs::batch_free(cs::HB(p));
+  return cs::kOk;
+}
+
+extern "C" int cs_batch_size(cs_batch* p, int* n) {
+  if (!p || !n) return cs::kBadArg;
+  *n = (int)cs::HB(p)->cl.size();
+  return cs::kOk;
+}
+
+extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
+  using namespace cs;
+  if (!p || nsteps < 0) return kBadArg;
+  Batch* b = HB(p);
+  for (Cloth* c : b->cl)
+    if (c->d.host_pd) {
+      set_error("cs_batch_step: host obstacle values describe one state");
+      return kBadArg;
+    }
+  hipStream_t s = b->stream;
+  // member cloths may have pending work on their own streams
+  for (Cloth* c : b->cl) CS_HIP(hipStreamSynchronize(c->stream));
+  if (int st = batch_refresh(b)) return st;
+  const size_t n = b->cl.size();
+  for (int k = 0; k < nsteps; k++) {
+    for (Cloth* c : b->cl)
+      if (c->n_sweep > 0) {
+        double sv = (c->time + c->P.dt) / c->sweep_duration;
+        sv = sv < 0.0 ? 0.0 : (sv > 1.0 ? 1.0 : sv);
+        const double off[3] = {c->sweep_delta[0] * sv, c->sweep_delta[1] * sv,
+                               c->sweep_delta[2] * sv};
+        if (int st = launch_apply_sweep(c->d, c->sweep_verts, c->n_sweep, off, s)) return st;
+      }
+    if (int st = batch_internal_step(b->view, b->tol, b->maxit, b->poll, b->dgather, b->hgather, s))
+      return st;
+    for (Cloth* c : b->cl) {
+      CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
+      CS_HIP(hipMemcpyAsync(c->hscal, c->d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
+    }
+    CS_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; i++) {
+      Cloth* c = b->cl[i];
+      c->have_x1 = true;
+      cs_step_stats st_local;
+      std::memset(&st_local, 0, sizeof(st_local));
+      st_local.pcg_conv = c->hscal[kConv];
+      st_local.pcg_iters = c->hflags[kDone] ? c->hflags[kIters] : c->P.pcg_maxit;
+      st_local.ilu_singular = c->hflags[kNsing];
+      int applied = 0;
+      if (int st = contact_resolve(c->contact, c->S, c->d, c->P, s, &applied, &st_local, nullptr))
+        return st;
+      if (applied)
+        if (int st = post_contact_update(c->d, c->P, s)) return st;
+      st_local.impulse_applied = applied;
+      if (stats) stats[i] = st_local;
+      c->time += c->P.dt;
+    }
+  }
+  CS_HIP(hipStreamSynchronize(s));
+  return kOk;
+}

@@ -133,6 +133,25 @@ struct DevCloth {
   double* f_blk;   // [(k*9 + q) * np + slot]
   double* f_dinv;  // [q * np + slot]
   double* b_blk;
+  int* ising;      // per-row singular-pivot flags of the last factorisation
+};
+
+// Which cloth a block works on.  One: the cloth handed to the launch by value
+// (single-cloth launches).  Many: entry blockIdx.y of device tables, one grid
+// row per cloth (batched launches); grids are sized for the largest cloth and
+// blocks past a cloth's own size return at once.
+struct One {
+  DevCloth c;
+  Params P;
+  __device__ __forceinline__ const DevCloth& cloth() const { return c; }
+  __device__ __forceinline__ const Params& params() const { return P; }
+};
+
+struct Many {
+  const DevCloth* tab;
+  const Params* ptab;
+  __device__ __forceinline__ DevCloth cloth() const { return tab[blockIdx.y]; }
+  __device__ __forceinline__ Params params() const { return ptab[blockIdx.y]; }
 };
 
 // Optional per-kernel-class timing with HIP events recorded on the cloth's

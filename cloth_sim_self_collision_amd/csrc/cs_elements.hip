@@ -159,8 +159,47 @@ extern "C" int cs_selftest_ordered_sum(int n, const double* t, double* out) {
     return kHipError;
   CS_HIP(hipMemset(w.ticket, 0, sizeof(unsigned int)));
   if (n) CS_HIP(hipMemcpy(dt, t, sizeof(double) * n, hipMemcpyHostToDevice));
-  launch_ordered_sum(dt, n, w, nullptr, SumStore{dout}, 0);
+  launch_ordered_sum(SumStore{SumJob{dt, n, w, nullptr}, dout}, n, 1, 0);
   CS_HIP(hipGetLastError());
   CS_HIP(hipMemcpy(out, dout, sizeof(double), hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+namespace cs {
+
+// plain streaming copy, 16 bytes per thread and iteration
+__global__ void k_stream_copy(long long n2, const double2* __restrict__ x, double2* __restrict__ y) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
+    y[i] = x[i];
+}
+
+}  // namespace cs
+
+extern "C" int cs_bandwidth_probe(long long n, int reps, double* gbs) {
+  using namespace cs;
+  if (n <= 0 || (n & 1) || reps <= 0 || !gbs) return kBadArg;
+  Scratch s;
+  double *x, *y;
+  if (s.get(&x, (size_t)n) || s.get(&y, (size_t)n)) return kHipError;
+  CS_HIP(hipMemset(x, 0, sizeof(double) * n));
+  const long long n2 = n / 2;
+  const int grid = 256 * 8;  // 8 blocks per CU, grid-stride
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, 0, n2, (const double2*)x, (double2*)y);
+  CS_HIP(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, 0, n2, (const double2*)x,
+                       (double2*)y);
+  CS_HIP(hipEventRecord(e1, 0));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CS_HIP(hipGetLastError());
+  *gbs = 2.0 * 8.0 * (double)n * reps / (ms * 1e-3) / 1e9;
   return kOk;
 }

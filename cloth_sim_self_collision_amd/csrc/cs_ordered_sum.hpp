@@ -294,10 +294,25 @@ __device__ inline void ordered_sum_walk(const double* __restrict__ t, int begin,
   __syncthreads();  // buf and sh.s are reused by the caller
 }
 
-// approximate sum of every tile (any order)
-static __global__ __launch_bounds__(kSumBlock) void k_sum_tiles(const double* __restrict__ t, int n,
-                                                         SumScratch w, const int* skip) {
-  if (skip && *skip) return;
+// One reduction: the addends, their count, the scratch and an optional flag
+// that cancels it.  A source policy Src provides
+//   __device__ SumJob job() const        -- the reduction of grid row blockIdx.y
+//   __device__ void finish(double) const -- consumes the exact sum (one thread)
+struct SumJob {
+  const double* t;
+  int n;
+  SumScratch w;
+  const int* skip;
+};
+
+// approximate sum of every tile (any order); grid = (tiles of the largest job, jobs)
+template <class Src>
+__global__ __launch_bounds__(kSumBlock) void k_sum_tiles(Src src) {
+  const SumJob job = src.job();
+  if (job.skip && *job.skip) return;
+  if ((int)blockIdx.x >= sum_tiles(job.n)) return;
+  const double* __restrict__ t = job.t;
+  const int n = job.n;
   __shared__ double wd[kSumWaves];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int base = blockIdx.x * kSumTile + tid * kSumPer;
@@ -312,18 +327,22 @@ static __global__ __launch_bounds__(kSumBlock) void k_sum_tiles(const double* __
   if (tid == 0) {
     double s = wd[0];
     for (int k = 1; k < kSumWaves; k++) s += wd[k];
-    w.tsum[blockIdx.x] = s;
+    job.w.tsum[blockIdx.x] = s;
   }
 }
 
-// Tile summaries, then the chain in the last block to finish; fin(s) runs on
-// one thread with the exact sum.  Grid = sum_tiles(n) blocks.
-template <class Fin>
-__global__ __launch_bounds__(kSumBlock) void k_sum_exact(const double* __restrict__ t, int n,
-                                                         SumScratch w, const int* skip, Fin fin) {
-  if (skip && *skip) return;
+// Tile summaries, then the chain in the last block of the job to finish;
+// src.finish(s) runs on one thread with the exact sum.  Same grid as k_sum_tiles.
+template <class Src>
+__global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
+  const SumJob job = src.job();
+  if (job.skip && *job.skip) return;
+  const double* __restrict__ t = job.t;
+  const int n = job.n;
+  const SumScratch& w = job.w;
+  const int b = blockIdx.x, nt = sum_tiles(n), tid = threadIdx.x;
+  if (b >= nt) return;
   __shared__ SumShared sh;
-  const int b = blockIdx.x, nt = gridDim.x, tid = threadIdx.x;
   const double two62 = 4611686018427387904.0;  // 2^62
   const long long lo52 = (1LL << 52) + 1, hi53 = (1LL << 53) - 1;
   {
@@ -420,20 +439,23 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(const double* __restric
     }
     __syncthreads();  // the window arrays are refilled
   }
-  if (tid == 0) fin(sh.s);
+  if (tid == 0) src.finish(sh.s);
 }
 
+// a plain sum stored to memory
 struct SumStore {
+  SumJob j;
   double* out;
-  __device__ void operator()(double s) const { out[0] = s; }
+  __device__ SumJob job() const { return j; }
+  __device__ void finish(double s) const { out[0] = s; }
 };
 
-template <class Fin>
-inline void launch_ordered_sum(const double* t, int n, const SumScratch& w, const int* skip,
-                               Fin fin, hipStream_t stream) {
-  const int nt = sum_tiles(n);
-  hipLaunchKernelGGL(k_sum_tiles, dim3(nt), dim3(kSumBlock), 0, stream, t, n, w, skip);
-  hipLaunchKernelGGL(k_sum_exact<Fin>, dim3(nt), dim3(kSumBlock), 0, stream, t, n, w, skip, fin);
+// jobs = grid rows; max_n = the largest job's length
+template <class Src>
+inline void launch_ordered_sum(const Src& src, int max_n, int jobs, hipStream_t stream) {
+  const dim3 grid(sum_tiles(max_n), jobs);
+  hipLaunchKernelGGL(k_sum_tiles<Src>, grid, dim3(kSumBlock), 0, stream, src);
+  hipLaunchKernelGGL(k_sum_exact<Src>, grid, dim3(kSumBlock), 0, stream, src);
 }
 
 }  // namespace cs

@@ -41,7 +41,10 @@ __global__ void k_bend_setup(DevCloth c, double stiff) {
 }
 
 // membrane element: 81 Hessian values (block (a,b) at (a*3+b)*9) + 9 gradient values
-__global__ void k_tri_elem(DevCloth c, double lam, double mu, int with_hess) {
+template <class Sel>
+__global__ void k_tri_elem(Sel sel, int with_hess) {
+  const DevCloth c = sel.cloth();
+  const double lam = sel.params().lam, mu = sel.params().mu;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= c.nt) return;
   double X[9], x[9];
@@ -63,7 +66,9 @@ __global__ void k_tri_elem(DevCloth c, double lam, double mu, int with_hess) {
     for (int i = 0; i < 3; i++) o[81 + a * 3 + i] = cst_grad(s, a, i);
 }
 
-__global__ void k_quad_grad(DevCloth c) {
+template <class Sel>
+__global__ void k_quad_grad(Sel sel) {
+  const DevCloth c = sel.cloth();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= c.nq) return;
   double x[12];
@@ -94,7 +99,9 @@ __device__ __forceinline__ void add_hess_src(const DevCloth& c, int src, double 
 
 // off-diagonal blocks; rows and columns of fixed vertices are zeroed
 // (matrix_square_sparse.cpp:272-302)
-__global__ void k_asm_crs(DevCloth c) {
+template <class Sel>
+__global__ void k_asm_crs(Sel sel) {
+  const DevCloth c = sel.cloth();
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= c.nnz) return;
   const int i = c.rowof[k], j = c.colind[k];
@@ -110,7 +117,10 @@ __global__ void k_asm_crs(DevCloth c) {
 }
 
 // diagonal blocks + right-hand side (solve_internal_sparse.h:285-313)
-__global__ void k_asm_dia_rhs(DevCloth c, Params P) {
+template <class Sel>
+__global__ void k_asm_dia_rhs(Sel sel) {
+  const DevCloth c = sel.cloth();
+  const Params P = sel.params();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.np) return;
   double acc[9];
@@ -230,17 +240,31 @@ __device__ void ilu_row(const DevCloth& c, int i, int stop, int* ising) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_ilu_factor(DevCloth c, int* ising) {
+// copy_in: also copy A into the factor storage and clear the pivot flags
+// first (batched launches; the single-cloth path uses device copies)
+template <class Sel>
+__global__ __launch_bounds__(1024) void k_ilu_factor(Sel sel, int copy_in) {
+  const DevCloth c = sel.cloth();
+  if (copy_in) {
+    for (long q = threadIdx.x; q < 9L * c.np; q += blockDim.x) c.fdia[q] = c.dia[q];
+    for (long q = threadIdx.x; q < 9L * c.nnz; q += blockDim.x) c.fcrs[q] = c.crs[q];
+    for (int q = threadIdx.x; q < c.np; q += blockDim.x) c.ising[q] = 0;
+    if (threadIdx.x == 0) c.flags[kNsing] = 0;
+    __syncthreads();
+  }
   for (int l = 0; l < c.nlf; l++) {
     const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
-    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x) ilu_row(c, c.lf_rows[t], c.np, ising);
+    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x) ilu_row(c, c.lf_rows[t], c.np, c.ising);
     __syncthreads();
   }
 }
 
 // re-run of the factorisation when more than 10 pivots were singular: stop at
 // the 11th singular row in row order, leave later rows untouched
-__global__ __launch_bounds__(1024) void k_ilu_redo(DevCloth c, int* ising) {
+template <class Sel>
+__global__ __launch_bounds__(1024) void k_ilu_redo(Sel sel) {
+  const DevCloth c = sel.cloth();
+  int* ising = c.ising;
   if (c.flags[kNsing] <= 10) return;
   __shared__ int stop;
   if (threadIdx.x == 0) {
@@ -270,9 +294,17 @@ __global__ __launch_bounds__(1024) void k_ilu_redo(DevCloth c, int* ising) {
 
 // forward then backward substitution, dst = (LU)^-1 src; optionally
 // prod = src .* dst for the next reduction (ilu_sparse.cpp:185-215, 295-324)
-__global__ __launch_bounds__(1024) void k_ilu_apply(DevCloth c, const double* src, double* dst,
+// src == nullptr: the PCG operands (r in, z out, r.*z into prod)
+template <class Sel>
+__global__ __launch_bounds__(1024) void k_ilu_apply(Sel sel, const double* src, double* dst,
                                                     double* prod) {
+  const DevCloth c = sel.cloth();
   if (c.flags[kDone]) return;
+  if (!src) {
+    src = c.r;
+    dst = c.z;
+    prod = c.prod;
+  }
   for (int l = 0; l < c.nlf; l++) {
     const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
     for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x) {
@@ -324,10 +356,12 @@ __global__ __launch_bounds__(1024) void k_ilu_apply(DevCloth c, const double* sr
 // are live at the same time.  Per row the arithmetic is exactly that of
 // k_ilu_apply (same blocks in the same column order).
 
-__global__ void k_pack_factor(DevCloth c) {
+template <class Sel>
+__global__ void k_pack_factor(Sel sel) {
+  const DevCloth c = sel.cloth();
   const int sl = blockIdx.x * blockDim.x + threadIdx.x;
   const int np = c.np;
-  if (sl >= np) return;
+  if (sl >= np || c.kl == 0) return;
   {
     const int i = c.lf_rows[sl];
     const int cnt = c.f_cnt[sl];
@@ -572,11 +606,17 @@ __device__ __forceinline__ void pipe_sweep(const DevCloth& c, const int* lptr, i
   }
 }
 
-template <int kTeams>
-__global__ __launch_bounds__(kPipeThreads) void k_ilu_apply_piped(DevCloth c,
-                                                                  const double* __restrict__ src,
+// src == nullptr: the PCG operands (r in, z out, r.*z into prod)
+template <int kTeams, class Sel>
+__global__ __launch_bounds__(kPipeThreads) void k_ilu_apply_piped(Sel sel, const double* src,
                                                                   double* dst, double* prod) {
+  const DevCloth c = sel.cloth();
   if (c.flags[kDone]) return;
+  if (!src) {
+    src = c.r;
+    dst = c.z;
+    prod = c.prod;
+  }
   __shared__ double ring[3][kRingRows];
   __shared__ int lfp[kPipeLevels + 1];
   __shared__ int lbp[kPipeLevels + 1];
@@ -598,22 +638,22 @@ static void launch_sweep(const DevCloth& c, const double* src, double* dst, doub
   const bool piped = packed && c.ring_ok && c.wmax <= kPipeThreads && c.nlf <= kPipeLevels &&
                      c.nlb <= kPipeLevels;
   if (c.sweep_variant == 4 && piped && c.wmax <= kPipeThreads / 2) {
-    hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+    hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
   } else if (c.sweep_variant == 5 && piped) {
-    hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+    hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
   } else if ((c.sweep_variant == 0 || c.sweep_variant >= 4) && piped) {
     if (c.wmax <= kPipeThreads / 4)
-      hipLaunchKernelGGL(k_ilu_apply_piped<4>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+      hipLaunchKernelGGL(k_ilu_apply_piped<4>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
     else if (c.wmax <= kPipeThreads / 2)
-      hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+      hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
     else
-      hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, c, src, dst, prod);
+      hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
   } else if (packed && c.ring_ok && c.sweep_variant <= 1) {
     hipLaunchKernelGGL(k_ilu_apply_packed<true>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
   } else if (packed && c.sweep_variant <= 2) {
     hipLaunchKernelGGL(k_ilu_apply_packed<false>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
   } else {
-    hipLaunchKernelGGL(k_ilu_apply, dim3(1), dim3(1024), 0, s, c, src, dst, prod);
+    hipLaunchKernelGGL(k_ilu_apply<One>, dim3(1), dim3(1024), 0, s, One{c, {}}, src, dst, prod);
   }
 }
 
@@ -647,8 +687,13 @@ __global__ void k_matvec(int np, const int* rowptr, const int* colind, const dou
 //   ap    = 0*z + 1.0*(A p_new)  -- the reference writes A p into the buffer
 //           that holds z (ilu_sparse.cpp:670-672), scaling it by beta = 0
 //   prod  = p_new .* ap
-__global__ void k_spmv_dir(DevCloth c, int first, const double* __restrict__ pold,
-                           double* __restrict__ pnew) {
+// iteration `it`: p_new = c.p[it & 1], p_old = c.p[(it + 1) & 1]
+template <class Sel>
+__global__ void k_spmv_dir(Sel sel, int it) {
+  const DevCloth c = sel.cloth();
+  const int first = it == 0;
+  const double* __restrict__ pold = c.p[(it + 1) & 1];
+  double* __restrict__ pnew = c.p[it & 1];
   if (c.flags[kDone]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.np) return;
@@ -698,7 +743,10 @@ __global__ void k_spmv_dir(DevCloth c, int first, const double* __restrict__ pol
 }
 
 // r -= alpha A p; x += alpha p; prod = r .* r  (matrix_square_sparse.cpp:327-336)
-__global__ void k_update_rx(DevCloth c, const double* __restrict__ p) {
+template <class Sel>
+__global__ void k_update_rx(Sel sel, int it) {
+  const DevCloth c = sel.cloth();
+  const double* __restrict__ p = c.p[it & 1];
   if (c.flags[kDone]) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
@@ -709,7 +757,9 @@ __global__ void k_update_rx(DevCloth c, const double* __restrict__ p) {
   c.prod[e] = r * r;
 }
 
-__global__ void k_pcg_init(DevCloth c, double tol2) {
+template <class Sel>
+__global__ void k_pcg_init(Sel sel, double tol2) {
+  const DevCloth c = sel.cloth();
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e == 0) {
     for (int q = 0; q < kNumFlag; q++) c.flags[q] = 0;
@@ -731,12 +781,20 @@ __global__ void k_pcg_init(DevCloth c, double tol2) {
 // what the PCG does with a finished reduction (ilu_sparse.cpp:648-708)
 enum DotMode : int { kDotRr0 = 0, kDotRz0 = 1, kDotPap = 2, kDotRr = 3, kDotRz = 4, kDotFinal = 5 };
 
-// PCG bookkeeping on the exact dot product (ilu_sparse.cpp:646-700)
-struct DotFinish {
-  DevCloth c;
+// exact left-to-right sum of prod, then the PCG bookkeeping of `mode`
+// (ilu_sparse.cpp:646-700); every mode but the first is skipped once the
+// solve has converged
+template <class Sel>
+struct DotSrc {
+  Sel sel;
   int mode;
   int iter;
-  __device__ void operator()(double s) const {
+  __device__ SumJob job() const {
+    const DevCloth c = sel.cloth();
+    return SumJob{c.prod, 3 * c.np, c.sum, mode == kDotRr0 ? nullptr : c.flags + kDone};
+  }
+  __device__ void finish(double s) const {
+    const DevCloth c = sel.cloth();
     c.scal[kLastDot] = s;
     switch (mode) {
       case kDotRr0:
@@ -776,21 +834,23 @@ struct DotFinish {
   }
 };
 
-// exact left-to-right sum of prod, then the PCG bookkeeping of `mode`; every
-// mode but the first is skipped once the solve has converged
 static void launch_dot(const DevCloth& c, int mode, int iter, hipStream_t s) {
-  const int* skip = mode == kDotRr0 ? nullptr : c.flags + kDone;
-  launch_ordered_sum(c.prod, 3 * c.np, c.sum, skip, DotFinish{c, mode, iter}, s);
+  launch_ordered_sum(DotSrc<One>{One{c, {}}, mode, iter}, 3 * c.np, 1, s);
 }
 
-__global__ void k_square(DevCloth c, const double* v) {
+template <class Sel>
+__global__ void k_square(Sel sel) {
+  const DevCloth c = sel.cloth();
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
-  c.prod[e] = v[e] * v[e];
+  c.prod[e] = c.r[e] * c.r[e];
 }
 
 // x += dx, v = dx / dt (solve_internal_sparse.h:323-325)
-__global__ void k_finish_internal(DevCloth c, double dt) {
+template <class Sel>
+__global__ void k_finish_internal(Sel sel) {
+  const DevCloth c = sel.cloth();
+  const double dt = sel.params().dt;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
   const double d = c.dx[e];
@@ -828,29 +888,32 @@ int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s) {
 
 int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s, Prof* prof) {
   ProfScope ps(prof, kProfAssemble);
-  hipLaunchKernelGGL(k_tri_elem, dim3(grid_for(c.nt)), dim3(kTB), 0, s, c, P.lam, P.mu, 1);
-  if (c.nq > 0) hipLaunchKernelGGL(k_quad_grad, dim3(grid_for(c.nq)), dim3(kTB), 0, s, c);
-  if (c.nnz > 0) hipLaunchKernelGGL(k_asm_crs, dim3(grid_for(c.nnz)), dim3(kTB), 0, s, c);
-  hipLaunchKernelGGL(k_asm_dia_rhs, dim3(grid_for(c.np)), dim3(kTB), 0, s, c, P);
+  const One one{c, P};
+  hipLaunchKernelGGL(k_tri_elem<One>, dim3(grid_for(c.nt)), dim3(kTB), 0, s, one, 1);
+  if (c.nq > 0) hipLaunchKernelGGL(k_quad_grad<One>, dim3(grid_for(c.nq)), dim3(kTB), 0, s, one);
+  if (c.nnz > 0) hipLaunchKernelGGL(k_asm_crs<One>, dim3(grid_for(c.nnz)), dim3(kTB), 0, s, one);
+  hipLaunchKernelGGL(k_asm_dia_rhs<One>, dim3(grid_for(c.np)), dim3(kTB), 0, s, one);
   CS_HIP(hipGetLastError());
   return kOk;
 }
 
-int launch_ilu(const DevCloth& c, int* ising, hipStream_t s, Prof* prof) {
+int launch_ilu(const DevCloth& c, hipStream_t s, Prof* prof) {
   ProfScope ps(prof, kProfFactor);
   CS_HIP(hipMemcpyAsync(c.fdia, c.dia, sizeof(double) * 9 * c.np, hipMemcpyDeviceToDevice, s));
   if (c.nnz > 0)
     CS_HIP(hipMemcpyAsync(c.fcrs, c.crs, sizeof(double) * 9 * (size_t)c.nnz,
                           hipMemcpyDeviceToDevice, s));
-  CS_HIP(hipMemsetAsync(ising, 0, sizeof(int) * c.np, s));
-  hipLaunchKernelGGL(k_ilu_factor, dim3(1), dim3(1024), 0, s, c, ising);
-  hipLaunchKernelGGL(k_ilu_redo, dim3(1), dim3(1024), 0, s, c, ising);
+  CS_HIP(hipMemsetAsync(c.ising, 0, sizeof(int) * c.np, s));
+  CS_HIP(hipMemsetAsync(c.flags + kNsing, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_ilu_factor<One>, dim3(1), dim3(1024), 0, s, One{c, {}}, 0);
+  hipLaunchKernelGGL(k_ilu_redo<One>, dim3(1), dim3(1024), 0, s, One{c, {}});
   CS_HIP(hipGetLastError());
   return launch_pack_factor(c, s);
 }
 
 int launch_pack_factor(const DevCloth& c, hipStream_t s) {
-  if (c.kl > 0) hipLaunchKernelGGL(k_pack_factor, dim3(grid_for(c.np)), dim3(kTB), 0, s, c);
+  if (c.kl > 0)
+    hipLaunchKernelGGL(k_pack_factor<One>, dim3(grid_for(c.np)), dim3(kTB), 0, s, One{c, {}});
   CS_HIP(hipGetLastError());
   return kOk;
 }
@@ -877,7 +940,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   const long n3 = 3L * c.np;
   {
     ProfScope ps(prof, kProfOther);
-    hipLaunchKernelGGL(k_pcg_init, dim3(grid_for(n3)), dim3(kTB), 0, s, c, tol * tol);
+    hipLaunchKernelGGL(k_pcg_init<One>, dim3(grid_for(n3)), dim3(kTB), 0, s, One{c, {}}, tol * tol);
   }
   {
     ProfScope ps(prof, kProfReduce);
@@ -895,12 +958,9 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   bool done = false;
   int it = 0;
   for (; it < maxit && !done; it++) {
-    double* pnew = c.p[it & 1];
-    const double* pold = c.p[(it + 1) & 1];
     {
       ProfScope ps(prof, kProfSpmv);
-      hipLaunchKernelGGL(k_spmv_dir, dim3(grid_for(c.np)), dim3(kTB), 0, s, c, it == 0 ? 1 : 0,
-                         pold, pnew);
+      hipLaunchKernelGGL(k_spmv_dir<One>, dim3(grid_for(c.np)), dim3(kTB), 0, s, One{c, {}}, it);
     }
     {
       ProfScope ps(prof, kProfReduce);
@@ -908,7 +968,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     }
     {
       ProfScope ps(prof, kProfAxpy);
-      hipLaunchKernelGGL(k_update_rx, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)pnew);
+      hipLaunchKernelGGL(k_update_rx<One>, dim3(grid_for(n3)), dim3(kTB), 0, s, One{c, {}}, it);
     }
     {
       ProfScope ps(prof, kProfReduce);
@@ -931,7 +991,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   }
   if (!done) {
     // cap reached: report the achieved ratio (ilu_sparse.cpp:706-707)
-    hipLaunchKernelGGL(k_square, dim3(grid_for(n3)), dim3(kTB), 0, s, c, (const double*)c.r);
+    hipLaunchKernelGGL(k_square<One>, dim3(grid_for(n3)), dim3(kTB), 0, s, One{c, {}});
     ProfScope ps(prof, kProfReduce);
     launch_dot(c, kDotFinal, maxit, s);
     CS_HIP(hipGetLastError());
@@ -940,14 +1000,142 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
 }
 
 int launch_gradients(const DevCloth& c, const Params& P, hipStream_t s) {
-  hipLaunchKernelGGL(k_tri_elem, dim3(grid_for(c.nt)), dim3(kTB), 0, s, c, P.lam, P.mu, 0);
-  if (c.nq > 0) hipLaunchKernelGGL(k_quad_grad, dim3(grid_for(c.nq)), dim3(kTB), 0, s, c);
+  const One one{c, P};
+  hipLaunchKernelGGL(k_tri_elem<One>, dim3(grid_for(c.nt)), dim3(kTB), 0, s, one, 0);
+  if (c.nq > 0) hipLaunchKernelGGL(k_quad_grad<One>, dim3(grid_for(c.nq)), dim3(kTB), 0, s, one);
   CS_HIP(hipGetLastError());
   return kOk;
 }
 
 int launch_finish_internal(const DevCloth& c, const Params& P, hipStream_t s) {
-  hipLaunchKernelGGL(k_finish_internal, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, c, P.dt);
+  hipLaunchKernelGGL(k_finish_internal<One>, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, One{c, P});
+  CS_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace cs
+
+// ---------------------------------------------------------------------------
+// batched internal step: the kernels above launched once for every cloth of a
+// batch (grid row = cloth, Many selector).  Per cloth the arithmetic and the
+// convergence handling are those of the single-cloth path, so each cloth's
+// result is identical to stepping it alone.
+
+namespace cs {
+
+template <class Sel>
+__global__ void k_save_start(Sel sel) {
+  const DevCloth c = sel.cloth();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * c.np) return;
+  c.x1[e] = c.x[e];
+}
+
+__global__ void k_gather_done(const DevCloth* tab, int n, int* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = tab[i].flags[kDone];
+}
+
+static void launch_sweep_batch(const BatchView& b, hipStream_t s) {
+  const Many m{b.tab, b.ptab};
+  if (b.piped) {
+    const dim3 g(1, b.n);
+    if (b.max_wmax <= kPipeThreads / 4)
+      hipLaunchKernelGGL((k_ilu_apply_piped<4, Many>), g, dim3(kPipeThreads), 0, s, m, nullptr, nullptr, nullptr);
+    else if (b.max_wmax <= kPipeThreads / 2)
+      hipLaunchKernelGGL((k_ilu_apply_piped<2, Many>), g, dim3(kPipeThreads), 0, s, m, nullptr, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((k_ilu_apply_piped<1, Many>), g, dim3(kPipeThreads), 0, s, m, nullptr, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL(k_ilu_apply<Many>, dim3(1, b.n), dim3(1024), 0, s, m, nullptr, nullptr, nullptr);
+  }
+}
+
+static void launch_dot_batch(const BatchView& b, int mode, int iter, hipStream_t s) {
+  launch_ordered_sum(DotSrc<Many>{Many{b.tab, b.ptab}, mode, iter}, 3 * b.max_np, b.n, s);
+}
+
+int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int* dgather,
+                        int* hgather, hipStream_t s, Prof* prof) {
+  const Many m{b.tab, b.ptab};
+  const int B = b.n;
+  const unsigned gv = grid_for(3L * b.max_np), gp = grid_for(b.max_np);
+  {
+    ProfScope ps(prof, kProfAssemble);
+    hipLaunchKernelGGL(k_save_start<Many>, dim3(gv, B), dim3(kTB), 0, s, m);
+    hipLaunchKernelGGL(k_tri_elem<Many>, dim3(grid_for(b.max_nt), B), dim3(kTB), 0, s, m, 1);
+    if (b.max_nq > 0)
+      hipLaunchKernelGGL(k_quad_grad<Many>, dim3(grid_for(b.max_nq), B), dim3(kTB), 0, s, m);
+    if (b.max_nnz > 0)
+      hipLaunchKernelGGL(k_asm_crs<Many>, dim3(grid_for(b.max_nnz), B), dim3(kTB), 0, s, m);
+    hipLaunchKernelGGL(k_asm_dia_rhs<Many>, dim3(gp, B), dim3(kTB), 0, s, m);
+  }
+  {
+    ProfScope ps(prof, kProfFactor);
+    hipLaunchKernelGGL(k_ilu_factor<Many>, dim3(1, B), dim3(1024), 0, s, m, 1);
+    hipLaunchKernelGGL(k_ilu_redo<Many>, dim3(1, B), dim3(1024), 0, s, m);
+    hipLaunchKernelGGL(k_pack_factor<Many>, dim3(gp, B), dim3(kTB), 0, s, m);
+  }
+  CS_HIP(hipGetLastError());
+  {
+    ProfScope ps(prof, kProfOther);
+    hipLaunchKernelGGL(k_pcg_init<Many>, dim3(gv, B), dim3(kTB), 0, s, m, tol * tol);
+  }
+  {
+    ProfScope ps(prof, kProfReduce);
+    launch_dot_batch(b, kDotRr0, 0, s);
+  }
+  {
+    ProfScope ps(prof, kProfSweep);
+    launch_sweep_batch(b, s);
+  }
+  {
+    ProfScope ps(prof, kProfReduce);
+    launch_dot_batch(b, kDotRz0, 0, s);
+  }
+  CS_HIP(hipGetLastError());
+  bool done = false;
+  for (int it = 0; it < maxit && !done; it++) {
+    {
+      ProfScope ps(prof, kProfSpmv);
+      hipLaunchKernelGGL(k_spmv_dir<Many>, dim3(gp, B), dim3(kTB), 0, s, m, it);
+    }
+    {
+      ProfScope ps(prof, kProfReduce);
+      launch_dot_batch(b, kDotPap, it, s);
+    }
+    {
+      ProfScope ps(prof, kProfAxpy);
+      hipLaunchKernelGGL(k_update_rx<Many>, dim3(gv, B), dim3(kTB), 0, s, m, it);
+    }
+    {
+      ProfScope ps(prof, kProfReduce);
+      launch_dot_batch(b, kDotRr, it, s);
+    }
+    {
+      ProfScope ps(prof, kProfSweep);
+      launch_sweep_batch(b, s);
+    }
+    {
+      ProfScope ps(prof, kProfReduce);
+      launch_dot_batch(b, kDotRz, it, s);
+    }
+    CS_HIP(hipGetLastError());
+    if ((it + 1) % poll == 0 || it + 1 == maxit) {
+      hipLaunchKernelGGL(k_gather_done, dim3(grid_for(B)), dim3(kTB), 0, s, b.tab, B, dgather);
+      CS_HIP(hipMemcpyAsync(hgather, dgather, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+      CS_HIP(hipStreamSynchronize(s));
+      done = true;
+      for (int i = 0; i < B; i++) done = done && hgather[i] != 0;
+    }
+  }
+  if (!done) {
+    // cloths at the cap report their achieved ratio; converged ones skip
+    hipLaunchKernelGGL(k_square<Many>, dim3(gv, B), dim3(kTB), 0, s, m);
+    ProfScope ps(prof, kProfReduce);
+    launch_dot_batch(b, kDotFinal, maxit, s);
+  }
+  hipLaunchKernelGGL(k_finish_internal<Many>, dim3(gv, B), dim3(kTB), 0, s, m);
   CS_HIP(hipGetLastError());
   return kOk;
 }

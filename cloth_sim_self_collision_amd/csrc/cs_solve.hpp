@@ -9,7 +9,7 @@ int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s);
 // element kernels + gather assembly of A and b (after boundary conditions)
 int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s, Prof* prof = nullptr);
 // copy A into the factor storage and factorise in place
-int launch_ilu(const DevCloth& c, int* ising, hipStream_t s, Prof* prof = nullptr);
+int launch_ilu(const DevCloth& c, hipStream_t s, Prof* prof = nullptr);
 // rearrange the factor for the packed sweeps (after any change of fdia/fcrs)
 int launch_pack_factor(const DevCloth& c, hipStream_t s);
 int launch_matvec(const DevCloth& c, double alpha, const double* x, double beta, double* y,
@@ -24,5 +24,20 @@ int launch_apply_sweep(const DevCloth& c, const int* verts, int n, const double 
 // membrane and bending gradients only (UpdateIntermidiateVelocity)
 int launch_gradients(const DevCloth& c, const Params& P, hipStream_t s);
 int launch_finish_internal(const DevCloth& c, const Params& P, hipStream_t s);
+
+// a batch of cloths as the batched kernels see it: device tables of the
+// cloths' DevCloth and Params, and the largest sizes (grid extents)
+struct BatchView {
+  const DevCloth* tab;
+  const Params* ptab;
+  int n;
+  int max_np, max_nt, max_nq, max_nnz, max_wmax;
+  bool piped;  // every cloth can use the level-pipelined sweep
+};
+// StepTime_InternalDynamicsILU for every cloth of the batch (start positions
+// saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); gather/hgather hold n
+// ints (device / pinned host) for the convergence poll
+int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int* dgather,
+                        int* hgather, hipStream_t s, Prof* prof = nullptr);
 
 }  // namespace cs

@@ -303,6 +303,49 @@ def element_contact(p, stiff, clearance, obstacle):
     return dW.reshape(n, 3), ddW.reshape(n, 9)
 
 
+class DeviceBatch:
+    """Several DeviceCloths stepped together by shared launches
+    (cs_batch_*).  Each cloth's result equals stepping it alone."""
+
+    def __init__(self, cloths):
+        self.cloths = list(cloths)
+        arr = (C.c_void_p * len(self.cloths))(*[c.h.value if isinstance(c.h, C.c_void_p) else c.h
+                                                for c in self.cloths])
+        h = C.c_void_p()
+        check(_lib.load().cs_batch_create(len(self.cloths), arr, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().cs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, n: int = 1) -> list:
+        """n StepTime iterations of every cloth; per-cloth stats of the last step."""
+        st = (_lib.StepStats * len(self.cloths))()
+        check(_lib.load().cs_batch_step(self.h, n, C.cast(st, C.c_void_p)))
+        return [s.as_dict() for s in st]
+
+
+def bandwidth_probe(n_doubles: int = 1 << 26, reps: int = 20) -> float:
+    """Measured device copy bandwidth in GB/s (read + write bytes)."""
+    g = C.c_double()
+    check(_lib.load().cs_bandwidth_probe(int(n_doubles), int(reps), C.byref(g)))
+    return g.value
+
+
 def selftest_divsqrt(a, b):
     a = f64(a)
     b = f64(b)

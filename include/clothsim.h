@@ -137,6 +137,22 @@ int cs_set_sweep(cs_cloth* c, int n, const int* verts, const double* delta, doub
 int cs_set_time(cs_cloth* c, double t);
 int cs_get_time(const cs_cloth* c, double* t);
 
+/* ---- batches: many cloths stepped by shared kernel launches ----
+ * Every kernel of the internal step (assembly, ILU(0), each PCG operation) is
+ * launched once for all cloths of the batch, one grid row per cloth, so a
+ * batch fills the GPU where one cloth occupies a few CUs.  Each cloth keeps
+ * its own convergence test and its result is bit-identical to stepping it
+ * alone; the contact pass runs per cloth.  The caller keeps ownership of the
+ * cloths; they must share pcg_tol and pcg_maxit, and must not be stepped on
+ * their own while a cs_batch_step on them runs. */
+typedef struct cs_batch cs_batch;
+int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out);
+int cs_batch_destroy(cs_batch* b);
+int cs_batch_size(cs_batch* b, int* n);
+/* nsteps StepTime iterations of every cloth; stats[i] (n entries, may be
+ * NULL) receives cloth i's last step */
+int cs_batch_step(cs_batch* b, int nsteps, cs_step_stats* stats);
+
 /* implementation choices (results are identical for every value):
  * CS_OPT_SWEEP: 0 automatic (level-pipelined packed sweep when the
  *               mesh allows it), 1 packed factor with the LDS ring,
@@ -208,6 +224,10 @@ int cs_inspect_setup(const cs_mesh* mesh, int* sizes, int* rowptr, int* colind, 
 int cs_selftest_divsqrt(int n, const double* a, const double* b, double* out);
 /* the PCG reduction on the device: *out = ((0.0 + t[0]) + t[1]) + ... */
 int cs_selftest_ordered_sum(int n, const double* t, double* out);
+/* device copy bandwidth: reps + 1 launches of a streaming copy of n doubles
+ * (n even) between two fresh buffers; the last reps are timed with events.
+ * *gbs = 16 n reps / seconds / 1e9 (read + write) */
+int cs_bandwidth_probe(long long n, int reps, double* gbs);
 
 #ifdef __cplusplus
 }

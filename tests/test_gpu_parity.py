@@ -244,3 +244,32 @@ def test_cpp_shim_step_loop_matches_reference(sim_mod, oracle_mod, tmp_path):
     for _ in range(steps):
         x, v, *_ = o.step(x, v)
     assert np.array_equal(res[:3 * cl.np_], x) and np.array_equal(res[3 * cl.np_:], v)
+
+
+def test_batch_matches_each_cloth_alone(sim_mod, oracle_mod):
+    """Cloths of different sizes stepped by shared launches (cs_batch_step):
+    every cloth follows its own oracle trajectory bit for bit, including a
+    swept fold and PCG runs that stop at different iterations."""
+    cls = [scene.drape_over_sphere(16), scene.fold_sweep(24, size=0.3), scene.drape_over_sphere(32),
+           scene.reference_demo(0)]
+    for cl in cls:
+        cl.pcg_tol, cl.pcg_maxit = 1e-6, 1000
+    steps = 25
+    devs = [sim_mod.DeviceCloth(cl) for cl in cls]
+    try:
+        with sim_mod.DeviceBatch(devs) as b:
+            st = b.step(steps)
+        for cl, d, s in zip(cls, devs, st):
+            o = oracle_mod.Oracle(cl, "port")
+            x, v, t = cl.xyz.copy(), cl.uvw.copy(), 0.0
+            for k in range(steps):
+                scene.apply_sweep(cl, x, t + cl.dt)
+                x, v, imp, it, conv = o.step(x, v)
+                t += cl.dt
+            gx, gv = d.get_state()
+            assert s["pcg_iters"] == it, (cl.name, s, it)
+            assert np.array_equal(gx, x) and np.array_equal(gv, v), (
+                cl.name, np.abs(gx - x).max(), np.abs(gv - v).max())
+    finally:
+        for d in devs:
+            d.close()
