@@ -97,6 +97,8 @@ def main() -> int:
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--maxit", type=int, default=None,
                     help="PCG iteration cap override (short profiling runs only)")
+    ap.add_argument("--sweep-variant", type=int, default=0,
+                    help="ILU sweep implementation (CS_OPT_SWEEP; results are identical)")
     ap.add_argument("--copy-mb", type=float, default=512.0,
                     help="size of the streaming-copy probe that measures the copy peak")
     args = ap.parse_args()
@@ -118,6 +120,7 @@ def main() -> int:
     if args.maxit is not None:
         cloth.pcg_maxit = args.maxit
     d = DeviceCloth(cloth)
+    d.set_option(1, args.sweep_variant)
     # warmup; the first step is also compared against the CPU reference
     first_x = None
     for k in range(max(args.warmup, 1)):
@@ -140,6 +143,7 @@ def main() -> int:
               file=sys.stderr, flush=True)
     t_total = time.perf_counter() - t_start
     prof = d.profile_read() if not args.no_profile else {}
+    red = d.reduction_stats() if not args.no_profile else None
     d.profile(False)
 
     value = cloth.np_ * args.steps / t_total
@@ -176,6 +180,7 @@ def main() -> int:
         "contacts": {"n_prox": stats[-1]["n_prox"], "n_ccd": stats[-1]["n_ccd"],
                      "n_riz_pass": stats[-1]["n_riz_pass"]},
         "kernel_ms": {k: round(v[0], 3) for k, v in prof.items()} if prof else None,
+        "reduction_stats": red,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

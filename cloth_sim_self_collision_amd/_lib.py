@@ -66,6 +66,7 @@ SIGNATURES = {
     "cs_set_step_start": (C.c_int, [_V, _D]),
     "cs_set_obstacle_values": (C.c_int, [_V, _D, _D]),
     "cs_update_velocity": (C.c_int, [_V]),
+    "cs_reduction_stats": (C.c_int, [_V, C.POINTER(C.c_longlong)]),
     "cs_batch_create": (C.c_int, [C.c_int, C.POINTER(_V), C.POINTER(_V)]),
     "cs_batch_destroy": (C.c_int, [_V]),
     "cs_batch_size": (C.c_int, [_V, C.POINTER(C.c_int)]),

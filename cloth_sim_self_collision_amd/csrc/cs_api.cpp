@@ -63,6 +63,7 @@ struct Cloth {
   int* hflags = nullptr;  // pinned
   double* hscal = nullptr;
   int* ising = nullptr;   // per-row singular-pivot flags
+  unsigned long long* sum_stats = nullptr;  // reduction counters (while profiling)
   ContactState contact;
   bool have_x1 = false;
   int poll = 8;
@@ -191,19 +192,27 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&d.sum.pmin, ntile);
   A.reserve(&d.sum.pmax, ntile);
   A.reserve(&d.sum.ticket, 1);
+  A.reserve(&c->sum_stats, kNumSumStat);
   A.reserve(&d.scal, kNumScal);
   A.reserve(&d.flags, kNumFlag);
   A.reserve(&d.bb, 6 * (size_t)nn);
   A.reserve(&c->ising, np);
-  int *fcnt, *fcol, *bcnt, *bcol;
-  const size_t kl = S.kl > 0 ? (size_t)S.kl : 0, ku = S.ku > 0 ? (size_t)S.ku : 0;
-  A.reserve(&fcnt, kl ? np : 0);
-  A.reserve(&fcol, kl * np);
-  A.reserve(&bcnt, ku ? np : 0);
-  A.reserve(&bcol, ku * np);
-  A.reserve(&d.f_blk, kl * 9 * np);
-  A.reserve(&d.f_dinv, kl ? 9 * (size_t)np : 0);
-  A.reserve(&d.b_blk, ku * 9 * np);
+  int *frc, *fcol, *brc, *bcol, *fsl;
+  const int kl = S.kl > 0 ? S.kl : 0, ku = S.ku > 0 ? S.ku : 0;
+  A.reserve(&frc, kl ? 4 * (size_t)np : 0);
+  A.reserve(&fsl, kl ? (size_t)np : 0);
+  A.reserve(&d.sf, 3 * (size_t)np);
+  A.reserve(&d.sb, 3 * (size_t)np);
+  A.reserve(&fcol, kl ? (size_t)col_groups(std::max(kl, ku)) * 4 * np : 0);
+  A.reserve(&brc, kl ? 2 * (size_t)np : 0);
+  A.reserve(&bcol, kl ? (size_t)col_groups(std::max(kl, ku)) * 4 * np : 0);
+  const int kq = kl > 0 ? std::max(kl, ku) : 0;
+  A.reserve(&d.f_blk, (size_t)kq * 12 * np);
+  A.reserve(&d.f_dinv, kq ? 12 * (size_t)np : 0);
+  A.reserve(&d.b_blk, (size_t)kq * 12 * np);
+  int *qfp, *qbp;
+  A.reserve(&qfp, S.qf_ptr.size());
+  A.reserve(&qbp, S.qb_ptr.size());
   double* host_pd;
   A.reserve(&host_pd, 4 * (size_t)np);
   if (int st = A.commit()) return st;
@@ -235,12 +244,18 @@ int build(Cloth* c, const cs_mesh* m) {
   d.refit_order = rord;
   d.kl = S.kl;
   d.ku = S.ku;
+  d.kpack = kq;
   d.ring_ok = S.ring_ok ? 1 : 0;
-  d.f_cnt = fcnt;
+  d.f_rci = frc;
+  d.fslot = kl ? fsl : nullptr;  // null: no packed sweep, no slot-order copy of r
   d.f_col = fcol;
-  d.b_cnt = bcnt;
+  d.b_rc = brc;
   d.b_col = bcol;
   d.ising = c->ising;
+  d.qf_ptr = qfp;
+  d.qb_ptr = qbp;
+  d.nqf = S.nqf;
+  d.nqb = S.nqb;
   hipStream_t s = c->stream;
   CS_HIP(hipMemsetAsync(d.sum.ticket, 0, sizeof(unsigned int), s));
   CS_HIP(hipMemcpyAsync(tri, m->tri, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, s));
@@ -273,9 +288,12 @@ int build(Cloth* c, const cs_mesh* m) {
   if (int st = upload(leafof, S.leaf_of, s)) return st;
   if (int st = upload(rord, S.refit_order, s)) return st;
   if (S.kl > 0) {
-    if (int st = upload(fcnt, S.f_cnt, s)) return st;
+    if (int st = upload(qfp, S.qf_ptr, s)) return st;
+    if (int st = upload(qbp, S.qb_ptr, s)) return st;
+    if (int st = upload(frc, S.f_rci, s)) return st;
+    if (int st = upload(fsl, S.fslot, s)) return st;
     if (int st = upload(fcol, S.f_col, s)) return st;
-    if (int st = upload(bcnt, S.b_cnt, s)) return st;
+    if (int st = upload(brc, S.b_rc, s)) return st;
     if (int st = upload(bcol, S.b_col, s)) return st;
   }
   CS_HIP(hipMemcpyAsync(d.x, m->xyz, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s));
@@ -552,7 +570,7 @@ int cs_set_sweep(cs_cloth* p, int n, const int* verts, const double* delta, doub
 int cs_set_option(cs_cloth* p, int option, int value) {
   if (!p) return kBadArg;
   Cloth* c = H(p);
-  if (option == CS_OPT_SWEEP && value >= 0 && value <= 5) {
+  if (option == CS_OPT_SWEEP && value >= 0 && value <= 1) {
     c->d.sweep_variant = value;
     return kOk;
   }
@@ -578,6 +596,18 @@ int cs_profile_enable(cs_cloth* p, int on) {
   c->prof.stream = c->stream;
   c->prof.reset();
   c->prof.on = on != 0;
+  CS_HIP(hipMemsetAsync(c->sum_stats, 0, sizeof(unsigned long long) * kNumSumStat, c->stream));
+  c->d.sum.stats = on ? c->sum_stats : nullptr;
+  return sync(c);
+}
+
+int cs_reduction_stats(cs_cloth* p, long long* out) {
+  if (!p || !out) return kBadArg;
+  Cloth* c = H(p);
+  unsigned long long h[kNumSumStat];
+  CS_HIP(hipMemcpyAsync(h, c->sum_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  if (int st = sync(c)) return st;
+  for (int k = 0; k < kNumSumStat; k++) out[k] = (long long)h[k];
   return kOk;
 }
 
@@ -858,7 +888,8 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
     v.max_nq = std::max(v.max_nq, d.nq);
     v.max_nnz = std::max(v.max_nnz, d.nnz);
     v.max_wmax = std::max(v.max_wmax, d.wmax);
-    v.piped = v.piped && d.kl > 0 && d.ring_ok && d.wmax <= 512 && d.nlf <= 2048 && d.nlb <= 2048;
+    v.piped = v.piped && quad_ok(d) && (v.kpack == 0 || v.kpack == d.kpack);
+    v.kpack = d.kpack;
   }
   b->htab.resize(n);
   b->hptab.resize(n);

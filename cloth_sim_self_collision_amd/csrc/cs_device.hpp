@@ -81,6 +81,9 @@ struct DevCloth {
   const int* lf_rows;
   const int* lb_ptr;
   const int* lb_rows;
+  const int* qf_ptr;  // sub-levels of at most kQuadRows slots
+  const int* qb_ptr;
+  int nqf, nqb;
   const int* crs_src_ptr;
   const int* crs_src;
   const int* dia_src_ptr;
@@ -124,14 +127,20 @@ struct DevCloth {
   double* bb;
   // packed sweeps (null when a row has more than kMaxBlocksPerRow blocks)
   int kl, ku, ring_ok;
+  int kpack;          // packed blocks per row of both sweeps: max(kl, ku), 0 = not packed
   int wmax;           // widest level of either sweep
-  int sweep_variant;  // 0 auto (pipelined), 1 packed + ring, 2 packed, 3 row-by-row, 4/5 pipelined 2/1 teams
-  const int* f_cnt;
-  const int* f_col;
-  const int* b_cnt;
+  int sweep_variant;  // 0 automatic (quad sweep), 1 row-by-row
+  const int* f_rci;  // {row, block count, backward slot, 0} per forward slot
+  const int* f_col;  // [((k / 4) * np + slot) * 4 + k % 4]
+  const int* b_rc;   // {row, block count} per backward slot
   const int* b_col;
-  double* f_blk;   // [(k*9 + q) * np + slot]
-  double* f_dinv;  // [q * np + slot]
+  const int* fslot;  // forward slot of each row
+  double* sf;        // sweep right-hand side in forward-slot order [3 * slot + r]
+  double* sb;        // forward result in backward-slot order
+  // quad layout: entry (r, j) of block k of a slot's row, r the output
+  // component, at [((k * 3 + j) * np + slot) * 4 + r] (r = 3 is padding)
+  double* f_blk;
+  double* f_dinv;  // D^-1 entry (r, j) at [(j * np + slot) * 4 + r]
   double* b_blk;
   int* ising;      // per-row singular-pivot flags of the last factorisation
 };

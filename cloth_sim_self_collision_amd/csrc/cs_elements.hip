@@ -152,7 +152,7 @@ extern "C" int cs_selftest_ordered_sum(int n, const double* t, double* out) {
   if (n < 0 || (n > 0 && !t) || !out) return kBadArg;
   Scratch s;
   double *dt, *dout;
-  SumScratch w;
+  SumScratch w{};
   const int nt = sum_tiles(n);
   if (s.get(&dt, n) || s.get(&dout, 1) || s.get(&w.tsum, nt) || s.get(&w.code, nt) ||
       s.get(&w.tot, nt) || s.get(&w.pmin, nt) || s.get(&w.pmax, nt) || s.get(&w.ticket, 1))

@@ -63,7 +63,11 @@ struct SumScratch {
   long long* pmin;       // minimum inclusive integer prefix inside the tile
   long long* pmax;       // maximum inclusive integer prefix
   unsigned int* ticket;  // blocks finished; zero between launches
+  // optional counters (null: off): sums, tiles, tiles applied in O(1),
+  // walker steps, burst elements
+  unsigned long long* stats;
 };
+enum SumStat : int { kStatSums = 0, kStatTiles, kStatFast, kStatSteps, kStatBurst, kNumSumStat };
 
 __host__ __device__ inline int sum_tiles(int n) { return n > 0 ? (n + kSumTile - 1) / kSumTile : 1; }
 
@@ -192,7 +196,8 @@ __device__ inline int ordered_sum_burst(const double* buf, int base, int m, int 
 // Adds t[begin, end) (at most one tile) onto sh.s exactly; called by all
 // threads of the block, the result is in sh.s.
 __device__ inline void ordered_sum_walk(const double* __restrict__ t, int begin, int end,
-                                        SumShared& sh) {
+                                        SumShared& sh, unsigned long long* stats) {
+  unsigned long long steps = 0, burst = 0;
   const int tid = threadIdx.x;
   const double two62 = 4611686018427387904.0;  // 2^62
   const long long lo52 = (1LL << 52) + 1, hi53 = (1LL << 53) - 1;
@@ -208,6 +213,7 @@ __device__ inline void ordered_sum_walk(const double* __restrict__ t, int begin,
   while (pos < end) {
     const double s = sh.s;
     if (isnan(s)) break;  // NaN absorbs every later addend
+    steps++;
     // modes: 0 zero, 1 subnormal, 2 normal, 3 infinite
     int mode;
     int scale = 0;
@@ -288,8 +294,13 @@ __device__ inline void ordered_sum_walk(const double* __restrict__ t, int begin,
         sh.s = sb;
       }
       __syncthreads();
+      burst += sh.pos - m;
       pos = sh.pos;
     }
+  }
+  if (stats && tid == 0) {
+    atomicAdd(stats + kStatSteps, steps);
+    atomicAdd(stats + kStatBurst, burst);
   }
   __syncthreads();  // buf and sh.s are reused by the caller
 }
@@ -400,6 +411,7 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
     sh.s = 0.0;
   }
   __syncthreads();
+  int fallback = 0;
   for (int b0 = 0; b0 < nt; b0 += kChainWin) {
     const int wend = min(nt, b0 + kChainWin);
     for (int j = b0 + tid; j < wend; j += kSumBlock) {
@@ -434,12 +446,20 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
       __syncthreads();
       bb = sh.tile;
       if (bb >= wend) break;
-      ordered_sum_walk(t, bb * kSumTile, min(n, (bb + 1) * kSumTile), sh);
+      ordered_sum_walk(t, bb * kSumTile, min(n, (bb + 1) * kSumTile), sh, w.stats);
+      fallback++;
       bb++;
     }
     __syncthreads();  // the window arrays are refilled
   }
-  if (tid == 0) src.finish(sh.s);
+  if (tid == 0) {
+    src.finish(sh.s);
+    if (w.stats) {
+      atomicAdd(w.stats + kStatSums, 1ull);
+      atomicAdd(w.stats + kStatTiles, (unsigned long long)nt);
+      atomicAdd(w.stats + kStatFast, (unsigned long long)(nt - fallback));
+    }
+  }
 }
 
 // a plain sum stored to memory

@@ -283,6 +283,15 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
     s.nlb = nb;
     levels_to_lists(lf, nf, s.lf_ptr, s.lf_rows, &s.max_lf);
     levels_to_lists(lb, nb, s.lb_ptr, s.lb_rows, &s.max_lb);
+    auto split = [](const std::vector<int>& ptr, int nl, std::vector<int>& q) {
+      q.assign(1, 0);
+      for (int l = 0; l < nl; l++)
+        for (int b = ptr[l]; b < ptr[l + 1]; b += kQuadRows) q.push_back(std::min(b + kQuadRows, ptr[l + 1]));
+    };
+    split(s.lf_ptr, s.nlf, s.qf_ptr);
+    split(s.lb_ptr, s.nlb, s.qb_ptr);
+    s.nqf = (int)s.qf_ptr.size() - 1;
+    s.nqb = (int)s.qb_ptr.size() - 1;
   }
   // packed sweep layout and the ring-buffer safety check
   {
@@ -292,17 +301,30 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
       s.ku = std::max(s.ku, s.rowptr[i + 1] - s.diaind[i]);
     }
     if (s.kl <= kMaxBlocksPerRow && s.ku <= kMaxBlocksPerRow) {
-      s.f_cnt.assign(np, 0);
-      s.b_cnt.assign(np, 0);
-      s.f_col.assign((size_t)std::max(s.kl, 1) * np, 0);
-      s.b_col.assign((size_t)std::max(s.ku, 1) * np, 0);
+      s.f_rci.assign(4 * (size_t)np, 0);
+      s.b_rc.assign(2 * (size_t)np, 0);
+      s.fslot.assign(np, 0);
+      std::vector<int> bslot(np);
+      for (int sl = 0; sl < np; sl++) {
+        s.fslot[s.lf_rows[sl]] = sl;
+        bslot[s.lb_rows[sl]] = sl;
+      }
+      const int kq = std::max(std::max(s.kl, s.ku), 1);  // both sweeps padded alike
+      s.f_col.assign((size_t)col_groups(kq) * 4 * np, 0);
+      s.b_col.assign((size_t)col_groups(kq) * 4 * np, 0);
+      auto cidx = [np](int k, int sl) { return ((size_t)(k / 4) * np + sl) * 4 + k % 4; };
       for (int sl = 0; sl < np; sl++) {
         const int i = s.lf_rows[sl];
-        s.f_cnt[sl] = s.diaind[i] - s.rowptr[i];
-        for (int k = 0; k < s.f_cnt[sl]; k++) s.f_col[(size_t)k * np + sl] = s.colind[s.rowptr[i] + k];
+        const int fc = s.diaind[i] - s.rowptr[i];
+        s.f_rci[4 * sl] = i;
+        s.f_rci[4 * sl + 1] = fc;
+        s.f_rci[4 * sl + 2] = bslot[i];
+        for (int k = 0; k < fc; k++) s.f_col[cidx(k, sl)] = s.colind[s.rowptr[i] + k];
         const int j = s.lb_rows[sl];
-        s.b_cnt[sl] = s.rowptr[j + 1] - s.diaind[j];
-        for (int k = 0; k < s.b_cnt[sl]; k++) s.b_col[(size_t)k * np + sl] = s.colind[s.diaind[j] + k];
+        const int bc = s.rowptr[j + 1] - s.diaind[j];
+        s.b_rc[2 * sl] = j;
+        s.b_rc[2 * sl + 1] = bc;
+        for (int k = 0; k < bc; k++) s.b_col[cidx(k, sl)] = s.colind[s.diaind[j] + k];
       }
       // live interval of a row in a sweep: [level written, last level read];
       // rows that share a ring slot must have disjoint intervals

@@ -23,10 +23,17 @@ struct MeshSetup {
   // strictly lower part) and backward substitution (strictly upper part)
   std::vector<int> lf_ptr, lf_rows, lb_ptr, lb_rows;
   int nlf = 0, nlb = 0, max_lf = 0, max_lb = 0;
+  // the same slot orders cut into sub-levels of at most kQuadRows rows (the
+  // rows of a level are independent, so running a level in pieces keeps every
+  // dependency satisfied)
+  std::vector<int> qf_ptr, qb_ptr;
+  int nqf = 0, nqb = 0;
   // packed sweeps: rows in level order ("slots"), at most kl lower / ku upper
-  // blocks per row; column indices stored slot-minor: col[k*np + slot]
+  // blocks per row.  f_rci[4*slot] = {row, block count, the row's backward
+  // slot, 0}, b_rc[2*slot] = {row, block count}, fslot[row] = forward slot;
+  // column indices in groups of four per slot: col[((k / 4) * np + slot) * 4 + k % 4]
   int kl = 0, ku = 0;
-  std::vector<int> f_cnt, f_col, b_cnt, b_col;
+  std::vector<int> f_rci, b_rc, fslot, f_col, b_col;
   // the LDS ring of kRingRows rows can hold the sweep dependencies: no two rows
   // sharing a ring slot are live at the same level
   bool ring_ok = false;
@@ -52,7 +59,10 @@ struct MeshSetup {
 };
 
 constexpr int kRingRows = 4096;
+constexpr int kQuadRows = 128;  // rows per sub-level of the quad sweep (4 lanes each)
 constexpr int kMaxBlocksPerRow = 8;
+// packed layout: column indices in groups of four per slot
+inline constexpr int col_groups(int k) { return (k + 3) / 4; }
 
 // Returns an empty string on success, else a description of the problem.
 std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,

@@ -349,12 +349,14 @@ __global__ __launch_bounds__(1024) void k_ilu_apply(Sel sel, const double* src, 
 }
 
 // ---------------------------------------------------------------------------
-// packed sweeps: the factor rearranged so that the rows of one level are
-// consecutive "slots" and every block component is contiguous over slots
-// (coalesced loads), the dependency values kept in an LDS ring of
-// kRingRows rows when the setup proved that no two rows sharing a ring slot
-// are live at the same time.  Per row the arithmetic is exactly that of
-// k_ilu_apply (same blocks in the same column order).
+// packed sweeps.  The rows of a sub-level (at most kQuadRows rows of one
+// level) are consecutive "slots".  Each row is handled by four lanes: lane r
+// (0..2) computes output component r, lane 3 idles.  The packed factor gives
+// every lane its own entries, [((k * 3 + j) * np + slot) * 4 + r] for entry
+// (r, j) of block k, so one load instruction of a wave reads 512 contiguous
+// bytes.  The forward sweep reads its right-hand side in forward-slot order
+// (sf) and writes its result in backward-slot order (sb), so that no load of
+// either sweep depends on another load.
 
 template <class Sel>
 __global__ void k_pack_factor(Sel sel) {
@@ -363,297 +365,233 @@ __global__ void k_pack_factor(Sel sel) {
   const int np = c.np;
   if (sl >= np || c.kl == 0) return;
   {
-    const int i = c.lf_rows[sl];
-    const int cnt = c.f_cnt[sl];
+    const int i = c.f_rci[4 * sl], cnt = c.f_rci[4 * sl + 1];
     const double* src = c.fcrs + (size_t)c.rowptr[i] * 9;
-    for (int k = 0; k < cnt; k++)
-      for (int q = 0; q < 9; q++) c.f_blk[(size_t)(k * 9 + q) * np + sl] = src[k * 9 + q];
-    for (int q = 0; q < 9; q++) c.f_dinv[(size_t)q * np + sl] = c.fdia[(size_t)i * 9 + q];
+    for (int k = 0; k < c.kpack; k++)
+      for (int j = 0; j < 3; j++)
+        for (int r = 0; r < 4; r++)
+          c.f_blk[(((size_t)(k * 3 + j) * np + sl) << 2) + r] =
+              (k < cnt && r < 3) ? src[k * 9 + r * 3 + j] : 0.0;
+    const double* d = c.fdia + (size_t)i * 9;
+    for (int j = 0; j < 3; j++)
+      for (int r = 0; r < 4; r++)
+        c.f_dinv[(((size_t)j * np + sl) << 2) + r] = r < 3 ? d[r * 3 + j] : 0.0;
   }
   {
-    const int i = c.lb_rows[sl];
-    const int cnt = c.b_cnt[sl];
+    const int i = c.b_rc[2 * sl], cnt = c.b_rc[2 * sl + 1];
     const double* src = c.fcrs + (size_t)c.diaind[i] * 9;
-    for (int k = 0; k < cnt; k++)
-      for (int q = 0; q < 9; q++) c.b_blk[(size_t)(k * 9 + q) * np + sl] = src[k * 9 + q];
+    for (int k = 0; k < c.kpack; k++)
+      for (int j = 0; j < 3; j++)
+        for (int r = 0; r < 4; r++)
+          c.b_blk[(((size_t)(k * 3 + j) * np + sl) << 2) + r] =
+              (k < cnt && r < 3) ? src[k * 9 + r * 3 + j] : 0.0;
   }
 }
 
-template <bool kRing>
-__global__ __launch_bounds__(512) void k_ilu_apply_packed(DevCloth c, const double* __restrict__ src,
-                                                          double* dst, double* prod) {
-  if (c.flags[kDone]) return;
-  __shared__ double ring[3][kRingRows];
-  const int np = c.np;
-  constexpr int kMask = kRingRows - 1;
-  for (int l = 0; l < c.nlf; l++) {
-    const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
-    for (int sl = b + (int)threadIdx.x; sl < e; sl += blockDim.x) {
-      const int i = c.lf_rows[sl];
-      const int cnt = c.f_cnt[sl];
-      double t0 = src[3 * i], t1 = src[3 * i + 1], t2 = src[3 * i + 2];
-#pragma unroll
-      for (int k = 0; k < kMaxBlocksPerRow; k++) {
-        if (k < cnt) {
-          const int j = c.f_col[(size_t)k * np + sl];
-          double y0, y1, y2;
-          if (kRing) {
-            y0 = ring[0][j & kMask];
-            y1 = ring[1][j & kMask];
-            y2 = ring[2][j & kMask];
-          } else {
-            y0 = dst[3 * j];
-            y1 = dst[3 * j + 1];
-            y2 = dst[3 * j + 2];
-          }
-          const double* a = c.f_blk + (size_t)(k * 9) * np + sl;
-          const double a0 = a[0], a1 = a[(size_t)np], a2 = a[2 * (size_t)np];
-          const double a3 = a[3 * (size_t)np], a4 = a[4 * (size_t)np], a5 = a[5 * (size_t)np];
-          const double a6 = a[6 * (size_t)np], a7 = a[7 * (size_t)np], a8 = a[8 * (size_t)np];
-          t0 -= a0 * y0 + a1 * y1 + a2 * y2;
-          t1 -= a3 * y0 + a4 * y1 + a5 * y2;
-          t2 -= a6 * y0 + a7 * y1 + a8 * y2;
-        }
-      }
-      const double* d = c.f_dinv + sl;
-      const double d0 = d[0], d1 = d[(size_t)np], d2 = d[2 * (size_t)np];
-      const double d3 = d[3 * (size_t)np], d4 = d[4 * (size_t)np], d5 = d[5 * (size_t)np];
-      const double d6 = d[6 * (size_t)np], d7 = d[7 * (size_t)np], d8 = d[8 * (size_t)np];
-      const double z0 = d0 * t0 + d1 * t1 + d2 * t2;
-      const double z1 = d3 * t0 + d4 * t1 + d5 * t2;
-      const double z2 = d6 * t0 + d7 * t1 + d8 * t2;
-      if (kRing) {
-        ring[0][i & kMask] = z0;
-        ring[1][i & kMask] = z1;
-        ring[2][i & kMask] = z2;
-      }
-      dst[3 * i] = z0;
-      dst[3 * i + 1] = z1;
-      dst[3 * i + 2] = z2;
-    }
-    __syncthreads();
-  }
-  for (int l = 0; l < c.nlb; l++) {
-    const int b = c.lb_ptr[l], e = c.lb_ptr[l + 1];
-    for (int sl = b + (int)threadIdx.x; sl < e; sl += blockDim.x) {
-      const int i = c.lb_rows[sl];
-      const int cnt = c.b_cnt[sl];
-      double t0 = dst[3 * i], t1 = dst[3 * i + 1], t2 = dst[3 * i + 2];
-#pragma unroll
-      for (int k = 0; k < kMaxBlocksPerRow; k++) {
-        if (k < cnt) {
-          const int j = c.b_col[(size_t)k * np + sl];
-          double y0, y1, y2;
-          if (kRing) {
-            y0 = ring[0][j & kMask];
-            y1 = ring[1][j & kMask];
-            y2 = ring[2][j & kMask];
-          } else {
-            y0 = dst[3 * j];
-            y1 = dst[3 * j + 1];
-            y2 = dst[3 * j + 2];
-          }
-          const double* a = c.b_blk + (size_t)(k * 9) * np + sl;
-          const double a0 = a[0], a1 = a[(size_t)np], a2 = a[2 * (size_t)np];
-          const double a3 = a[3 * (size_t)np], a4 = a[4 * (size_t)np], a5 = a[5 * (size_t)np];
-          const double a6 = a[6 * (size_t)np], a7 = a[7 * (size_t)np], a8 = a[8 * (size_t)np];
-          t0 -= a0 * y0 + a1 * y1 + a2 * y2;
-          t1 -= a3 * y0 + a4 * y1 + a5 * y2;
-          t2 -= a6 * y0 + a7 * y1 + a8 * y2;
-        }
-      }
-      if (kRing) {
-        ring[0][i & kMask] = t0;
-        ring[1][i & kMask] = t1;
-        ring[2][i & kMask] = t2;
-      }
-      dst[3 * i] = t0;
-      dst[3 * i + 1] = t1;
-      dst[3 * i + 2] = t2;
-    }
-    __syncthreads();
-  }
-  if (prod)
-    for (int e = threadIdx.x; e < 3 * np; e += blockDim.x) prod[e] = src[e] * dst[e];
+// x (row order) -> sf (forward-slot order)
+template <class Sel>
+__global__ void k_gather_fslots(Sel sel, const double* x) {
+  const DevCloth c = sel.cloth();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * c.np) return;
+  const int sl = e / 3, r = e - 3 * sl;
+  c.sf[e] = x[3 * c.f_rci[4 * sl] + r];
 }
 
-// ---------------------------------------------------------------------------
-// level-pipelined packed sweep.  The 512 threads form kTeams teams; team
-// (l mod kTeams) computes level l, one row per thread, and right after that
-// issues the loads of everything level l + kTeams needs that does not depend
-// on the sweep itself: the right-hand side, column indices, factor blocks and
-// inverted diagonal.  Those loads then have kTeams - 1 level times to arrive,
-// and the row index they use was itself loaded one round earlier, so no load
-// waits on another inside a round.  At compute time only the dependency
-// values are read, from the LDS ring.  Per row the arithmetic is exactly that
-// of k_ilu_apply (same blocks, same column order, same expression shape);
-// blocks past the row's count are loaded (the packing pads to the widest row)
-// but never enter the arithmetic.
-constexpr int kPipeThreads = 512;
-constexpr int kPipeLevels = 2048;  // level-pointer capacity kept in LDS
+constexpr int kQuadThreads = 4 * kQuadRows;  // 512: one sub-level of 128 rows per pass
 
-struct PipeRow {
-  int cnt;
-  int i;  // -1: no row for this thread at this level
-  double t0, t1, t2;
-  int col[kMaxBlocksPerRow];
-  double a[kMaxBlocksPerRow * 9];
-  double d[9];
-};
-
-template <bool kFwd>
-__device__ __forceinline__ void pipe_load(const DevCloth& c, int K, int sl, int i,
-                                          const double* __restrict__ rhs, PipeRow& r) {
-  const int np = c.np;
-  r.i = i;
-  if (i < 0) return;
-  r.cnt = kFwd ? c.f_cnt[sl] : c.b_cnt[sl];
-  r.t0 = rhs[3 * i];
-  r.t1 = rhs[3 * i + 1];
-  r.t2 = rhs[3 * i + 2];
-  const int* col = kFwd ? c.f_col : c.b_col;
-  const double* blk = kFwd ? c.f_blk : c.b_blk;
-#pragma unroll
-  for (int k = 0; k < kMaxBlocksPerRow; k++) {
-    if (k < K) {
-      r.col[k] = col[(size_t)k * np + sl];
-#pragma unroll
-      for (int q = 0; q < 9; q++) r.a[k * 9 + q] = blk[(size_t)(k * 9 + q) * np + sl];
-    }
-  }
-  if (kFwd) {
-#pragma unroll
-    for (int q = 0; q < 9; q++) r.d[q] = c.f_dinv[(size_t)q * np + sl];
-  }
-}
-
-template <bool kFwd>
-__device__ __forceinline__ void pipe_compute(int K, double (*ring)[kRingRows], double* dst,
-                                             PipeRow& r) {
-  constexpr int kMask = kRingRows - 1;
-  if (r.i < 0) return;
-  double y[kMaxBlocksPerRow][3];
-#pragma unroll
-  for (int k = 0; k < kMaxBlocksPerRow; k++) {
-    if (k < K) {
-      const int j = r.col[k] & kMask;
-      y[k][0] = ring[0][j];
-      y[k][1] = ring[1][j];
-      y[k][2] = ring[2][j];
-    }
-  }
-  double t0 = r.t0, t1 = r.t1, t2 = r.t2;
-#pragma unroll
-  for (int k = 0; k < kMaxBlocksPerRow; k++) {
-    if (k < K && k < r.cnt) {
-      const double* a = r.a + k * 9;
-      t0 -= a[0] * y[k][0] + a[1] * y[k][1] + a[2] * y[k][2];
-      t1 -= a[3] * y[k][0] + a[4] * y[k][1] + a[5] * y[k][2];
-      t2 -= a[6] * y[k][0] + a[7] * y[k][1] + a[8] * y[k][2];
-    }
-  }
-  double z0 = t0, z1 = t1, z2 = t2;
-  if (kFwd) {
-    const double* d = r.d;
-    z0 = d[0] * t0 + d[1] * t1 + d[2] * t2;
-    z1 = d[3] * t0 + d[4] * t1 + d[5] * t2;
-    z2 = d[6] * t0 + d[7] * t1 + d[8] * t2;
-  }
-  const int i = r.i;
-  ring[0][i & kMask] = z0;
-  ring[1][i & kMask] = z1;
-  ring[2][i & kMask] = z2;
-  dst[3 * i] = z0;
-  dst[3 * i + 1] = z1;
-  dst[3 * i + 2] = z2;
+// value of lane j of this lane's quad
+template <int j>
+__device__ __forceinline__ double quad_bcast(double v) {
+  constexpr int ctrl = j | (j << 2) | (j << 4) | (j << 6);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
 }
 
 // Workgroup barrier that orders LDS only: the ring writes of a level are
-// visible after it, while the prefetch loads issued before it stay in
-// flight (a full __syncthreads() would wait for them).
+// visible after it, while the loads issued for later levels stay in flight
+// (a full __syncthreads() would wait for them).
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <bool kFwd, int kTeams>
-__device__ __forceinline__ void pipe_sweep(const DevCloth& c, const int* lptr, int nl,
-                                           const int* __restrict__ rows, const double* rhs,
-                                           double* dst, double (*ring)[kRingRows]) {
-  constexpr int kTeam = kPipeThreads / kTeams;
-  const int team = threadIdx.x / kTeam, lt = threadIdx.x - team * kTeam;
-  const int K = kFwd ? c.kl : c.ku;
-  auto row_at = [&](int l, int& sl) -> int {
-    if (l >= nl) return -1;
-    sl = lptr[l] + lt;
-    return sl < lptr[l + 1] ? rows[sl] : -1;
-  };
-  PipeRow r;
-  int sl = 0, sl_next = 0;
-  {
-    const int i = row_at(team, sl);
-    pipe_load<kFwd>(c, K, sl, i, rhs, r);
+struct QuadRow {
+  int i;     // row, -1: none for this quad in this sub-level
+  int cnt;   // blocks of the row
+  int bs;    // forward: the row's backward slot
+  double t;  // this lane's component of the right-hand side
+  int col[kMaxBlocksPerRow];
+  double a[kMaxBlocksPerRow * 3];
+  double d[3];
+};
+
+// Branch-free on purpose: a quad without a row at this sub-level loads slot 0
+// and is only marked (i = -1), lane 3 loads lane 2's right-hand side.  Loads
+// on one path only would make the compiler wait for every outstanding load
+// where the paths merge, defeating the prefetch.
+template <bool kFwd, int K>
+__device__ __forceinline__ void quad_load(const DevCloth& c, const int* qptr, int nq, int l,
+                                          const double* __restrict__ rhs, QuadRow& q) {
+  const int lane = threadIdx.x & 3, qi = threadIdx.x >> 2;
+  const int lc = l < nq ? l : 0;
+  const int s0 = qptr[lc] + qi;
+  const bool valid = l < nq && s0 < qptr[lc + 1];
+  const int sl = valid ? s0 : 0;
+  const int np = c.np;
+  if (kFwd) {
+    const int4 v = reinterpret_cast<const int4*>(c.f_rci)[sl];
+    q.i = valid ? v.x : -1;
+    q.cnt = v.y;
+    q.bs = v.z;
+  } else {
+    const int2 v = reinterpret_cast<const int2*>(c.b_rc)[sl];
+    q.i = valid ? v.x : -1;
+    q.cnt = v.y;
   }
-  int i_next = row_at(team + kTeams, sl_next);
-  for (int l = 0; l < nl; l++) {
-    if (l % kTeams == team) {
-      pipe_compute<kFwd>(K, ring, dst, r);
-      pipe_load<kFwd>(c, K, sl_next, i_next, rhs, r);
-      i_next = row_at(l + 2 * kTeams, sl_next);
-    }
-    lds_barrier();
+  q.t = rhs[3 * sl + (lane < 3 ? lane : 2)];
+  const int4* col = reinterpret_cast<const int4*>(kFwd ? c.f_col : c.b_col);
+#pragma unroll
+  for (int g = 0; g < col_groups(K); g++) {
+    const int4 v = col[(size_t)g * np + sl];
+    q.col[4 * g] = v.x;
+    q.col[4 * g + 1] = v.y;
+    q.col[4 * g + 2] = v.z;
+    q.col[4 * g + 3] = v.w;
+  }
+  const double* blk = kFwd ? c.f_blk : c.b_blk;
+#pragma unroll
+  for (int k = 0; k < K; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) q.a[k * 3 + j] = blk[(((size_t)(k * 3 + j) * np + sl) << 2) + lane];
+  if (kFwd) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) q.d[j] = c.f_dinv[(((size_t)j * np + sl) << 2) + lane];
   }
 }
 
-// src == nullptr: the PCG operands (r in, z out, r.*z into prod)
-template <int kTeams, class Sel>
-__global__ __launch_bounds__(kPipeThreads) void k_ilu_apply_piped(Sel sel, const double* src,
-                                                                  double* dst, double* prod) {
+// one row component; every lane of the block runs it (the DPP exchange needs
+// whole quads).  Branch-free up to the global store: blocks past the row's
+// count are selected away, and lanes without a row (and lane 3) write their
+// value to a dummy ring slot.  Everything loaded is thus consumed on every
+// path, which keeps the compiler from waiting for the next sub-level's loads.
+template <bool kFwd, int K>
+__device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], double* out,
+                                             const QuadRow& q) {
+  constexpr int kMask = kRingRows - 1;
+  const int lane = threadIdx.x & 3;
+  const bool has = q.i >= 0;
+  double y[K][3];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int j = q.col[k] & kMask;
+    y[k][0] = ring[0][j];
+    y[k][1] = ring[1][j];
+    y[k][2] = ring[2][j];
+  }
+  double t = q.t;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const double* a = q.a + k * 3;
+    const double u = t - (a[0] * y[k][0] + a[1] * y[k][1] + a[2] * y[k][2]);
+    t = (k < q.cnt) ? u : t;
+  }
+  double z = t;
+  if (kFwd) {
+    const double t0 = quad_bcast<0>(t), t1 = quad_bcast<1>(t), t2 = quad_bcast<2>(t);
+    z = q.d[0] * t0 + q.d[1] * t1 + q.d[2] * t2;
+  }
+  const bool own = has && lane < 3;
+  ring[lane < 3 ? lane : 2][own ? (q.i & kMask) : kRingRows] = z;
+  if (own) {
+    if (kFwd)
+      out[3 * q.bs + lane] = z;  // backward-slot order
+    else
+      out[3 * q.i + lane] = z;   // row order
+  }
+}
+
+// one sweep over the sub-levels; two register sets alternate so that the
+// loads of sub-level l + 2 are in flight while l + 1 is computed
+template <bool kFwd, int K>
+__device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, int nq,
+                                           const double* rhs, double* out,
+                                           double (*ring)[kRingRows + 1]) {
+  QuadRow qa, qb;
+  quad_load<kFwd, K>(c, qptr, nq, 0, rhs, qa);
+  quad_load<kFwd, K>(c, qptr, nq, 1, rhs, qb);
+  // both halves unconditional (an odd count ends with an empty sub-level):
+  // a skipped half would make the loads of qa the newest on one path, and the
+  // compiler would then wait for all loads at the top of every iteration
+  for (int l = 0; l < nq; l += 2) {
+    quad_compute<kFwd, K>(ring, out, qa);
+    lds_barrier();
+    quad_load<kFwd, K>(c, qptr, nq, l + 2, rhs, qa);
+    quad_compute<kFwd, K>(ring, out, qb);
+    lds_barrier();
+    quad_load<kFwd, K>(c, qptr, nq, l + 3, rhs, qb);
+  }
+}
+
+// M^-1 (L+D)(I+D^-1 U) solve of the factor: forward sweep from c.sf (the
+// right-hand side in forward-slot order) into c.sb, backward sweep into dst
+// (row order); prod = src_row .* dst when prod is given
+// K: the packed blocks per row (DevCloth::kpack)
+template <int K, class Sel>
+__global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const double* src_row,
+                                                                 double* dst, double* prod) {
   const DevCloth c = sel.cloth();
   if (c.flags[kDone]) return;
-  if (!src) {
-    src = c.r;
+  if (!src_row) {
+    src_row = c.r;
     dst = c.z;
     prod = c.prod;
   }
-  __shared__ double ring[3][kRingRows];
-  __shared__ int lfp[kPipeLevels + 1];
-  __shared__ int lbp[kPipeLevels + 1];
-  for (int l = threadIdx.x; l <= c.nlf; l += blockDim.x) lfp[l] = c.lf_ptr[l];
-  for (int l = threadIdx.x; l <= c.nlb; l += blockDim.x) lbp[l] = c.lb_ptr[l];
+  __shared__ double ring[3][kRingRows + 1];  // + a dummy slot
+  __shared__ int qfp[kQuadLevels + 1];
+  __shared__ int qbp[kQuadLevels + 1];
+  for (int l = threadIdx.x; l <= c.nqf; l += blockDim.x) qfp[l] = c.qf_ptr[l];
+  for (int l = threadIdx.x; l <= c.nqb; l += blockDim.x) qbp[l] = c.qb_ptr[l];
   __syncthreads();
-  pipe_sweep<true, kTeams>(c, lfp, c.nlf, c.lf_rows, src, dst, ring);
-  __syncthreads();  // the backward sweep reads the complete forward result from dst
-  pipe_sweep<false, kTeams>(c, lbp, c.nlb, c.lb_rows, dst, dst, ring);
+  quad_sweep<true, K>(c, qfp, c.nqf, c.sf, c.sb, ring);
+  __syncthreads();  // the backward sweep reads the complete forward result
+  quad_sweep<false, K>(c, qbp, c.nqb, c.sb, dst, ring);
   __syncthreads();
   if (prod)
-    for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src[e] * dst[e];
+    for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src_row[e] * dst[e];
 }
 
-// one forward+backward substitution launch (variants give identical results)
+template <class Sel>
+static void launch_quad(int kq, dim3 grid, const Sel& sel, const double* src, double* dst,
+                        double* prod, hipStream_t s) {
+  const dim3 tb(kQuadThreads);
+  switch (kq) {
+    case 1: hipLaunchKernelGGL((k_ilu_apply_quad<1, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 2: hipLaunchKernelGGL((k_ilu_apply_quad<2, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 3: hipLaunchKernelGGL((k_ilu_apply_quad<3, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 4: hipLaunchKernelGGL((k_ilu_apply_quad<4, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 5: hipLaunchKernelGGL((k_ilu_apply_quad<5, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 6: hipLaunchKernelGGL((k_ilu_apply_quad<6, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 7: hipLaunchKernelGGL((k_ilu_apply_quad<7, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    default: hipLaunchKernelGGL((k_ilu_apply_quad<8, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+  }
+}
+
+// one forward+backward substitution launch (variants give identical results):
+// 0 automatic (the quad sweep when the mesh allows it), 1 row-by-row CSR
+// sweep.  `sf_ready`: c.sf already holds src in forward-slot order (the PCG
+// producers write it).
 static void launch_sweep(const DevCloth& c, const double* src, double* dst, double* prod,
-                         hipStream_t s) {
-  const bool packed = c.kl > 0;
-  const bool piped = packed && c.ring_ok && c.wmax <= kPipeThreads && c.nlf <= kPipeLevels &&
-                     c.nlb <= kPipeLevels;
-  if (c.sweep_variant == 4 && piped && c.wmax <= kPipeThreads / 2) {
-    hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
-  } else if (c.sweep_variant == 5 && piped) {
-    hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
-  } else if ((c.sweep_variant == 0 || c.sweep_variant >= 4) && piped) {
-    if (c.wmax <= kPipeThreads / 4)
-      hipLaunchKernelGGL(k_ilu_apply_piped<4>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
-    else if (c.wmax <= kPipeThreads / 2)
-      hipLaunchKernelGGL(k_ilu_apply_piped<2>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
-    else
-      hipLaunchKernelGGL(k_ilu_apply_piped<1>, dim3(1), dim3(kPipeThreads), 0, s, One{c, {}}, src, dst, prod);
-  } else if (packed && c.ring_ok && c.sweep_variant <= 1) {
-    hipLaunchKernelGGL(k_ilu_apply_packed<true>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
-  } else if (packed && c.sweep_variant <= 2) {
-    hipLaunchKernelGGL(k_ilu_apply_packed<false>, dim3(1), dim3(512), 0, s, c, src, dst, prod);
+                         hipStream_t s, bool sf_ready = false) {
+  const One one{c, {}};
+  if (quad_ok(c) && c.sweep_variant != 1) {
+    if (!sf_ready)
+      hipLaunchKernelGGL(k_gather_fslots<One>, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, one, src);
+    launch_quad(c.kpack, dim3(1), one, src, dst, prod, s);
   } else {
-    hipLaunchKernelGGL(k_ilu_apply<One>, dim3(1), dim3(1024), 0, s, One{c, {}}, src, dst, prod);
+    hipLaunchKernelGGL(k_ilu_apply<One>, dim3(1), dim3(1024), 0, s, one, src, dst, prod);
   }
 }
 
@@ -755,6 +693,10 @@ __global__ void k_update_rx(Sel sel, int it) {
   c.r[e] = r;
   c.dx[e] = c.dx[e] + alpha * p[e];
   c.prod[e] = r * r;
+  if (c.fslot) {
+    const int i = e / 3;
+    c.sf[3 * c.fslot[i] + (e - 3 * i)] = r;  // the next sweep's input, slot order
+  }
 }
 
 template <class Sel>
@@ -772,6 +714,10 @@ __global__ void k_pcg_init(Sel sel, double tol2) {
   c.r[e] = b;
   c.dx[e] = 0;
   c.prod[e] = b * b;
+  if (c.fslot) {
+    const int i = e / 3;
+    c.sf[3 * c.fslot[i] + (e - 3 * i)] = b;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -948,7 +894,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   }
   {
     ProfScope ps(prof, kProfSweep);
-    launch_sweep(c, c.r, c.z, c.prod, s);
+    launch_sweep(c, c.r, c.z, c.prod, s, true);
   }
   {
     ProfScope ps(prof, kProfReduce);
@@ -976,7 +922,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     }
     {
       ProfScope ps(prof, kProfSweep);
-      launch_sweep(c, c.r, c.z, c.prod, s);
+      launch_sweep(c, c.r, c.z, c.prod, s, true);
     }
     {
       ProfScope ps(prof, kProfReduce);
@@ -1039,13 +985,8 @@ __global__ void k_gather_done(const DevCloth* tab, int n, int* out) {
 static void launch_sweep_batch(const BatchView& b, hipStream_t s) {
   const Many m{b.tab, b.ptab};
   if (b.piped) {
-    const dim3 g(1, b.n);
-    if (b.max_wmax <= kPipeThreads / 4)
-      hipLaunchKernelGGL((k_ilu_apply_piped<4, Many>), g, dim3(kPipeThreads), 0, s, m, nullptr, nullptr, nullptr);
-    else if (b.max_wmax <= kPipeThreads / 2)
-      hipLaunchKernelGGL((k_ilu_apply_piped<2, Many>), g, dim3(kPipeThreads), 0, s, m, nullptr, nullptr, nullptr);
-    else
-      hipLaunchKernelGGL((k_ilu_apply_piped<1, Many>), g, dim3(kPipeThreads), 0, s, m, nullptr, nullptr, nullptr);
+    // every cloth packed: r is already in slot order (k_pcg_init / k_update_rx)
+    launch_quad(b.kpack, dim3(1, b.n), m, nullptr, nullptr, nullptr, s);
   } else {
     hipLaunchKernelGGL(k_ilu_apply<Many>, dim3(1, b.n), dim3(1024), 0, s, m, nullptr, nullptr, nullptr);
   }

@@ -5,6 +5,13 @@
 
 namespace cs {
 
+constexpr int kQuadLevels = 4096;  // sub-level pointers the quad sweep keeps in LDS
+// the packed (quad) sweep applies: packed factor, LDS ring proven safe,
+// sub-level pointers fit
+inline bool quad_ok(const DevCloth& c) {
+  return c.kpack > 0 && c.ring_ok && c.nqf <= kQuadLevels && c.nqb <= kQuadLevels;
+}
+
 int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s);
 // element kernels + gather assembly of A and b (after boundary conditions)
 int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s, Prof* prof = nullptr);
@@ -32,7 +39,8 @@ struct BatchView {
   const Params* ptab;
   int n;
   int max_np, max_nt, max_nq, max_nnz, max_wmax;
-  bool piped;  // every cloth can use the level-pipelined sweep
+  bool piped;  // every cloth can use the packed (quad) sweep ...
+  int kpack;   // ... with this many packed blocks per row
 };
 // StepTime_InternalDynamicsILU for every cloth of the batch (start positions
 // saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); gather/hgather hold n

@@ -170,6 +170,11 @@ class DeviceCloth:
     def profile(self, on: bool = True) -> None:
         check(_lib.load().cs_profile_enable(self.h, 1 if on else 0))
 
+    def reduction_stats(self) -> dict:
+        out = (C.c_longlong * 5)()
+        check(_lib.load().cs_reduction_stats(self.h, out))
+        return dict(zip(("sums", "tiles", "fast_tiles", "walker_steps", "burst_elements"), out))
+
     def profile_read(self) -> dict:
         out = {}
         for k, name in enumerate(self.PROFILE_KINDS):

@@ -154,10 +154,8 @@ int cs_batch_size(cs_batch* b, int* n);
 int cs_batch_step(cs_batch* b, int nsteps, cs_step_stats* stats);
 
 /* implementation choices (results are identical for every value):
- * CS_OPT_SWEEP: 0 automatic (level-pipelined packed sweep when the
- *               mesh allows it), 1 packed factor with the LDS ring,
- *               2 packed factor without the ring, 3 row-by-row CSR sweep,
- *               4 / 5 the pipelined sweep with 2 / 1 thread teams */
+ * CS_OPT_SWEEP: 0 automatic (the packed four-lanes-per-row sweep when
+ *               the mesh allows it), 1 row-by-row CSR sweep */
 #define CS_OPT_SWEEP 1
 int cs_set_option(cs_cloth* c, int option, int value);
 
@@ -166,6 +164,10 @@ int cs_set_option(cs_cloth* c, int option, int value);
  * 5 assembly, 6 contact pass, 7 other.  Enabling resets the totals. */
 int cs_profile_enable(cs_cloth* c, int on);
 int cs_profile_read(cs_cloth* c, int kind, double* total_ms, long long* launches);
+/* counters of the exact PCG reductions while profiling is on: out[5] =
+ * reductions, tiles, tiles applied from their summary, binade-walker steps,
+ * elements added one by one */
+int cs_reduction_stats(cs_cloth* c, long long* out);
 
 /* ---- lower-level entry points (host arrays unless noted) ---- */
 int cs_get_pattern(const cs_cloth* c, int* rowptr, int* colind, int* diaind);

@@ -192,9 +192,7 @@ def test_velocity_update_and_host_obstacle_values(sim_mod, oracle_mod):
 
 @pytest.mark.parametrize("name,steps,variant", [("drape32", 100, 0), ("refdemo0", 60, 0),
                                                 ("refdemo1", 60, 0), ("fold24", 60, 0),
-                                                ("drape32", 60, 1), ("fold24", 40, 2),
-                                                ("fold24", 40, 3), ("drape32", 40, 4),
-                                                ("fold24", 30, 5)])
+                                                ("drape32", 60, 1), ("fold24", 40, 1)])
 def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant):
     cl = _scenes()[name]
     o = oracle_mod.Oracle(cl, "port")
