@@ -90,6 +90,45 @@ def cpu_baseline(cloth: scene.Cloth, gx1: np.ndarray):
             "sample": f"1 step of {cloth.name} from the initial state ({dt_s:.1f} s)"}, err
 
 
+def batch_measure(count: int, steps: int, seed: int = 0) -> dict:
+    """The batch workload (independent cloths of seeded sizes 24..128 dropped
+    onto their own sphere or box): the same cloths stepped one after another
+    with their own launches ("plain") and together by cs_batch_step."""
+    from cloth_sim_self_collision_amd.sim import DeviceBatch, DeviceCloth
+    cls = scene.batch_drop(count, seed=seed)
+    nv = sum(c.np_ for c in cls)
+    devs = [DeviceCloth(c) for c in cls]
+    try:
+        for d in devs:  # warm-up: one step each
+            d.step(1)
+        t0 = time.perf_counter()
+        for d in devs:
+            d.step(1)
+        t_plain = time.perf_counter() - t0
+        print(f"[bench] batch: plain round {t_plain:.2f} s", file=sys.stderr, flush=True)
+        with DeviceBatch(devs) as b:
+            b.step(1)  # warm-up of the batched kernels
+            lat = []
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                t1 = time.perf_counter()
+                st = b.step(1)
+                lat.append(time.perf_counter() - t1)
+            t_batch = time.perf_counter() - t0
+        print(f"[bench] batch: {steps} batched steps {t_batch:.2f} s", file=sys.stderr, flush=True)
+        lat_ms = np.array(lat) * 1e3
+        plain = nv / t_plain
+        batched = nv * steps / t_batch
+        return {"workload": f"batch_drop{count}", "cloths": count, "vertices": nv,
+                "plain_vertex_steps_per_s": plain, "batched_vertex_steps_per_s": batched,
+                "speedup": batched / plain, "batched_p50_step_ms": float(np.percentile(lat_ms, 50)),
+                "batched_p99_step_ms": float(np.percentile(lat_ms, 99)),
+                "pcg_iters_max": max(s["pcg_iters"] for s in st)}
+    finally:
+        for d in devs:
+            d.close()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +141,9 @@ def main() -> int:
                     help="PCG iteration cap override (short profiling runs only)")
     ap.add_argument("--sweep-variant", type=int, default=0,
                     help="ILU sweep implementation (CS_OPT_SWEEP; results are identical)")
+    ap.add_argument("--batch-count", type=int, default=64,
+                    help="cloths of the batch workload measured beside the headline (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--copy-mb", type=float, default=512.0,
                     help="size of the streaming-copy probe that measures the copy peak")
     args = ap.parse_args()
@@ -166,6 +208,8 @@ def main() -> int:
                 "share_of_gpu_time": ms / sum(v[0] for v in kinds.values()),
                 "stream_copy_gbs": copy_gbs,
                 "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None}
+    d.close()
+    batch = batch_measure(args.batch_count, args.batch_steps) if args.batch_count > 0 else None
     cpu, err = (None, None) if args.no_cpu else cpu_baseline(workload(args.workload), first_x)
     out = {
         "metric": "vertex-steps/s + p99 step latency, served vs plain GPU, at <=1e-5 m pos err",
@@ -186,9 +230,9 @@ def main() -> int:
         "reduction_stats": red,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out))
-    d.close()
     return 0
 
 

@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("gravity", C.c_double * 3), ("mass_point", C.c_double),
                 ("stiff_contact", C.c_double), ("contact_clearance", C.c_double),
                 ("dt", C.c_double), ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int),
-                ("obstacle_kind", C.c_int), ("obstacle", C.c_double * 4)]
+                ("obstacle_kind", C.c_int), ("obstacle", C.c_double * 8)]
 
 
 class StepStats(C.Structure):
@@ -66,6 +66,7 @@ SIGNATURES = {
     "cs_set_step_start": (C.c_int, [_V, _D]),
     "cs_set_obstacle_values": (C.c_int, [_V, _D, _D]),
     "cs_update_velocity": (C.c_int, [_V]),
+    "cs_cloth_memory": (C.c_int, [_V, C.POINTER(C.c_longlong)]),
     "cs_reduction_stats": (C.c_int, [_V, C.POINTER(C.c_longlong)]),
     "cs_batch_create": (C.c_int, [C.c_int, C.POINTER(_V), C.POINTER(_V)]),
     "cs_batch_destroy": (C.c_int, [_V]),

@@ -97,7 +97,7 @@ Params to_params(const cs_params* p) {
   P.pcg_tol = p->pcg_tol;
   P.pcg_maxit = p->pcg_maxit;
   P.obstacle_kind = p->obstacle_kind;
-  for (int k = 0; k < 4; k++) P.obstacle[k] = p->obstacle[k];
+  for (int k = 0; k < 8; k++) P.obstacle[k] = p->obstacle[k];
   return P;
 }
 
@@ -110,7 +110,7 @@ int check_params(const cs_params* p) {
     set_error("params need dt > 0, mass_point > 0, pcg_maxit >= 0, pcg_tol >= 0");
     return kBadArg;
   }
-  if (p->obstacle_kind < 0 || p->obstacle_kind > 3) {
+  if (p->obstacle_kind < 0 || p->obstacle_kind > 4) {
     set_error("unknown obstacle kind");
     return kBadArg;
   }
@@ -207,9 +207,11 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&brc, kl ? 2 * (size_t)np : 0);
   A.reserve(&bcol, kl ? (size_t)col_groups(std::max(kl, ku)) * 4 * np : 0);
   const int kq = kl > 0 ? std::max(kl, ku) : 0;
-  A.reserve(&d.f_blk, (size_t)kq * 12 * np);
+  // value pairs per lane: (3 kq + 1) / 2 pairs of 3 lanes x 2 doubles per slot
+  const size_t pk = kq ? (size_t)(3 * kq + 1) / 2 * 6 * np : 0;
+  A.reserve(&d.f_blk, pk);
   A.reserve(&d.f_dinv, kq ? 12 * (size_t)np : 0);
-  A.reserve(&d.b_blk, (size_t)kq * 12 * np);
+  A.reserve(&d.b_blk, pk);
   int *qfp, *qbp;
   A.reserve(&qfp, S.qf_ptr.size());
   A.reserve(&qbp, S.qb_ptr.size());
@@ -412,6 +414,18 @@ int cs_cloth_set_params(cs_cloth* p, const cs_params* params) {
     if (int st = launch_bend_setup(c->d, c->P, c->stream)) return st;
     return sync(c);
   }
+  return kOk;
+}
+
+int cs_cloth_memory(const cs_cloth* p, long long* bytes) {
+  if (!p || !bytes) return kBadArg;
+  const Cloth* c = H(p);
+  const ContactState& k = c->contact;
+  // the arena, the contact records at their current capacity (keys, tests,
+  // leaf pairs, two sort buffers, groups, the set, impulse links), sort
+  // scratch and the per-vertex contact arrays
+  *bytes = (long long)c->arena.bytes() + (long long)k.cap * 88 + (long long)k.tmp_bytes +
+           (long long)c->S.np * 64;
   return kOk;
 }
 

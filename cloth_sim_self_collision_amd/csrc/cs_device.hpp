@@ -42,7 +42,7 @@ struct Params {
   double pcg_tol;
   int pcg_maxit;
   int obstacle_kind;
-  double obstacle[4];
+  double obstacle[8];
 };
 
 // device scalar slots of the linear solve
@@ -137,10 +137,10 @@ struct DevCloth {
   const int* fslot;  // forward slot of each row
   double* sf;        // sweep right-hand side in forward-slot order [3 * slot + r]
   double* sb;        // forward result in backward-slot order
-  // quad layout: entry (r, j) of block k of a slot's row, r the output
-  // component, at [((k * 3 + j) * np + slot) * 4 + r] (r = 3 is padding)
+  // packed layout: value v = k * 3 + j (entry (r, j) of block k of a slot's
+  // row, r the output component) at [(((v / 2) * np + slot) * 3 + r) * 2 + v % 2]
   double* f_blk;
-  double* f_dinv;  // D^-1 entry (r, j) at [(j * np + slot) * 4 + r]
+  double* f_dinv;  // D^-1 entry (r, j) the same way with v = j (4 values, one padding)
   double* b_blk;
   int* ising;      // per-row singular-pivot flags of the last factorisation
 };

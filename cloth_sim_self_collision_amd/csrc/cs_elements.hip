@@ -42,12 +42,15 @@ __global__ void k_elem_bend(int n, const double* C, const double* x, double stif
     for (int i = 0; i < 3; i++) dW[12 * (size_t)e + a * 3 + i] = bend_grad(kab, x + 12 * (size_t)e, a, i);
 }
 
+struct Obst8 {
+  double v[8];
+};
+
 __global__ void k_elem_contact(int n, const double* p, double k, double clearance, int kind,
-                               double o0, double o1, double o2, double o3, double* dW,
-                               double* ddW) {
+                               Obst8 o, double* dW, double* ddW) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  const double q[4] = {o0, o1, o2, o3};
+  const double* q = o.v;
   double pd, nrm[3];
   obstacle_depth(kind, q, p + 3 * (size_t)e, &pd, nrm);
   contact_penalty(pd, nrm, k, clearance, dW + 3 * (size_t)e, ddW + 9 * (size_t)e);
@@ -119,15 +122,17 @@ extern "C" int cs_element_bend(int n, const double* C, const double* c, double s
 
 extern "C" int cs_element_contact(int n, const double* p, double stiff, double clearance,
                                   int kind, const double* obstacle, double* dW, double* ddW) {
-  if (n < 0 || (n > 0 && (!p || !obstacle || !dW || !ddW)) || kind < 0 || kind > 3)
+  if (n < 0 || (n > 0 && (!p || !obstacle || !dW || !ddW)) || kind < 0 || kind > 4)
     return kBadArg;
   if (n == 0) return kOk;
   Scratch s;
   double *dp, *dg, *dh;
   if (s.get(&dp, 3L * n) || s.get(&dg, 3L * n) || s.get(&dh, 9L * n)) return kHipError;
   CS_HIP(hipMemcpy(dp, p, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  Obst8 o8;
+  for (int k = 0; k < 8; k++) o8.v[k] = obstacle[k];
   hipLaunchKernelGGL(k_elem_contact, dim3(blocks(n)), dim3(256), 0, 0, n, dp, stiff, clearance,
-                     kind, obstacle[0], obstacle[1], obstacle[2], obstacle[3], dg, dh);
+                     kind, o8, dg, dh);
   CS_HIP(hipGetLastError());
   CS_HIP(hipMemcpy(dW, dg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
   CS_HIP(hipMemcpy(ddW, dh, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));

@@ -319,6 +319,27 @@ CS_HD void obstacle_depth(int kind, const double* q, const double* p, double* pd
       n[2] *= -1;
       *pd *= -1;
     }
+  } else if (kind == 4) {
+    // axis-aligned box: center q[0..2], half-extents q[3..5]
+    double s[3], d[3];
+    for (int i = 0; i < 3; i++) {
+      s[i] = p[i] - q[i];
+      d[i] = fabs(s[i]) - q[3 + i];
+    }
+    if (d[0] <= 0.0 && d[1] <= 0.0 && d[2] <= 0.0) {
+      int m = 0;
+      if (d[1] > d[m]) m = 1;
+      if (d[2] > d[m]) m = 2;
+      n[0] = n[1] = n[2] = 0.0;
+      n[m] = s[m] < 0.0 ? -1.0 : 1.0;
+      *pd = -d[m];
+    } else {
+      double o[3];
+      for (int i = 0; i < 3; i++) o[i] = d[i] > 0.0 ? d[i] : 0.0;
+      const double len = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+      for (int i = 0; i < 3; i++) n[i] = (s[i] < 0.0 ? -o[i] : o[i]) / len;
+      *pd = -len;
+    }
   } else {
     *pd = -100;
     n[0] = n[1] = n[2] = 0;

@@ -352,11 +352,19 @@ __global__ __launch_bounds__(1024) void k_ilu_apply(Sel sel, const double* src, 
 // packed sweeps.  The rows of a sub-level (at most kQuadRows rows of one
 // level) are consecutive "slots".  Each row is handled by four lanes: lane r
 // (0..2) computes output component r, lane 3 idles.  The packed factor gives
-// every lane its own entries, [((k * 3 + j) * np + slot) * 4 + r] for entry
-// (r, j) of block k, so one load instruction of a wave reads 512 contiguous
-// bytes.  The forward sweep reads its right-hand side in forward-slot order
+// every lane its own entries as 16-byte pairs: value v = k * 3 + j (entry
+// (r, j) of block k) at [(((v / 2) * np + slot) * 3 + r) * 2 + v % 2] (lane 3
+// reads lane 2's), so one load instruction of a wave reads 768 contiguous
+// bytes and a row component costs ceil(3K / 2) block loads.  The load
+// instructions issued per sub-level, not HBM, bound the sweep.  The forward sweep reads its right-hand side in forward-slot order
 // (sf) and writes its result in backward-slot order (sb), so that no load of
 // either sweep depends on another load.
+
+// 16-byte pairs of a slot's values per lane (see above)
+__host__ __device__ constexpr int pack_pairs(int K) { return (3 * K + 1) / 2; }
+__device__ __forceinline__ size_t pack_index(int v, int sl, int r, int np) {
+  return ((((size_t)(v >> 1) * np + sl) * 3 + r) << 1) + (v & 1);
+}
 
 template <class Sel>
 __global__ void k_pack_factor(Sel sel) {
@@ -367,24 +375,23 @@ __global__ void k_pack_factor(Sel sel) {
   {
     const int i = c.f_rci[4 * sl], cnt = c.f_rci[4 * sl + 1];
     const double* src = c.fcrs + (size_t)c.rowptr[i] * 9;
-    for (int k = 0; k < c.kpack; k++)
-      for (int j = 0; j < 3; j++)
-        for (int r = 0; r < 4; r++)
-          c.f_blk[(((size_t)(k * 3 + j) * np + sl) << 2) + r] =
-              (k < cnt && r < 3) ? src[k * 9 + r * 3 + j] : 0.0;
+    for (int v = 0; v < 2 * pack_pairs(c.kpack); v++)
+      for (int r = 0; r < 3; r++) {
+        const int k = v / 3, j = v % 3;
+        c.f_blk[pack_index(v, sl, r, np)] = (k < cnt && k < c.kpack) ? src[k * 9 + r * 3 + j] : 0.0;
+      }
     const double* d = c.fdia + (size_t)i * 9;
-    for (int j = 0; j < 3; j++)
-      for (int r = 0; r < 4; r++)
-        c.f_dinv[(((size_t)j * np + sl) << 2) + r] = r < 3 ? d[r * 3 + j] : 0.0;
+    for (int v = 0; v < 4; v++)
+      for (int r = 0; r < 3; r++) c.f_dinv[pack_index(v, sl, r, np)] = v < 3 ? d[r * 3 + v] : 0.0;
   }
   {
     const int i = c.b_rc[2 * sl], cnt = c.b_rc[2 * sl + 1];
     const double* src = c.fcrs + (size_t)c.diaind[i] * 9;
-    for (int k = 0; k < c.kpack; k++)
-      for (int j = 0; j < 3; j++)
-        for (int r = 0; r < 4; r++)
-          c.b_blk[(((size_t)(k * 3 + j) * np + sl) << 2) + r] =
-              (k < cnt && r < 3) ? src[k * 9 + r * 3 + j] : 0.0;
+    for (int v = 0; v < 2 * pack_pairs(c.kpack); v++)
+      for (int r = 0; r < 3; r++) {
+        const int k = v / 3, j = v % 3;
+        c.b_blk[pack_index(v, sl, r, np)] = (k < cnt && k < c.kpack) ? src[k * 9 + r * 3 + j] : 0.0;
+      }
   }
 }
 
@@ -451,7 +458,8 @@ __device__ __forceinline__ void quad_load(const DevCloth& c, const int* qptr, in
     q.i = valid ? v.x : -1;
     q.cnt = v.y;
   }
-  q.t = rhs[3 * sl + (lane < 3 ? lane : 2)];
+  const int lr = lane < 3 ? lane : 2;  // lane 3 shadows lane 2 (same cache line)
+  q.t = rhs[3 * sl + lr];
   const int4* col = reinterpret_cast<const int4*>(kFwd ? c.f_col : c.b_col);
 #pragma unroll
   for (int g = 0; g < col_groups(K); g++) {
@@ -461,14 +469,20 @@ __device__ __forceinline__ void quad_load(const DevCloth& c, const int* qptr, in
     q.col[4 * g + 2] = v.z;
     q.col[4 * g + 3] = v.w;
   }
-  const double* blk = kFwd ? c.f_blk : c.b_blk;
+  const double2* blk = reinterpret_cast<const double2*>(kFwd ? c.f_blk : c.b_blk);
 #pragma unroll
-  for (int k = 0; k < K; k++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) q.a[k * 3 + j] = blk[(((size_t)(k * 3 + j) * np + sl) << 2) + lane];
+  for (int p = 0; p < pack_pairs(K); p++) {
+    const double2 w = blk[((size_t)p * np + sl) * 3 + lr];
+    q.a[2 * p] = w.x;
+    if (2 * p + 1 < 3 * K) q.a[2 * p + 1] = w.y;
+  }
   if (kFwd) {
-#pragma unroll
-    for (int j = 0; j < 3; j++) q.d[j] = c.f_dinv[(((size_t)j * np + sl) << 2) + lane];
+    const double2* dd = reinterpret_cast<const double2*>(c.f_dinv);
+    const double2 w0 = dd[(size_t)sl * 3 + lr];
+    const double2 w1 = dd[((size_t)np + sl) * 3 + lr];
+    q.d[0] = w0.x;
+    q.d[1] = w0.y;
+    q.d[2] = w1.x;
   }
 }
 

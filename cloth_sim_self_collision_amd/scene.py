@@ -32,6 +32,7 @@ OBST_NONE = 0
 OBST_PLANE = 1    # pd = d - dot(n, p), normal n (unit)
 OBST_SPHERE = 2   # pd = r - |p - c|, n = (p - c)/|p - c|
 OBST_SPHERE_INV = 3  # self_contact_sparse/main.cpp:117-133 sign-flipped sphere
+OBST_BOX = 4         # axis-aligned box (center, half-extents); not in the reference
 
 _MASK64 = (1 << 64) - 1
 
@@ -82,6 +83,22 @@ class Obstacle:
             d = float(self.params[3])
             pd = d - (p[:, 0] * n[0] + p[:, 1] * n[1] + p[:, 2] * n[2])
             return pd, np.broadcast_to(n, p.shape).copy()
+        if self.kind == OBST_BOX:
+            s = p - np.array(self.params[:3], dtype=np.float64)
+            d = np.abs(s) - np.array(self.params[3:6], dtype=np.float64)
+            inside = (d <= 0.0).all(axis=1)
+            m = np.argmax(d, axis=1)  # first axis on ties
+            pd = np.empty(len(p))
+            n = np.zeros_like(p)
+            rows = np.arange(len(p))
+            pd[inside] = -d[rows, m][inside]
+            n[rows[inside], m[inside]] = np.where(s[rows, m] < 0.0, -1.0, 1.0)[inside]
+            o = np.maximum(d, 0.0)
+            ln = np.sqrt(o[:, 0] * o[:, 0] + o[:, 1] * o[:, 1] + o[:, 2] * o[:, 2])
+            out = ~inside
+            pd[out] = -ln[out]
+            n[out] = (np.where(s < 0.0, -o, o) / np.where(ln > 0, ln, 1.0)[:, None])[out]
+            return pd, n
         c = np.array(self.params[:3], dtype=np.float64)
         r = float(self.params[3])
         v = p - c
@@ -95,6 +112,11 @@ class Obstacle:
 
 def sphere(center, radius) -> Obstacle:
     return Obstacle(OBST_SPHERE, (center[0], center[1], center[2], radius))
+
+
+def box(center, half) -> Obstacle:
+    """Axis-aligned box; inside pd = distance to the nearest face."""
+    return Obstacle(OBST_BOX, (center[0], center[1], center[2], half[0], half[1], half[2]))
 
 
 def plane(normal, offset) -> Obstacle:
@@ -320,6 +342,41 @@ def drape_over_sphere(n: int = 32, *, seed: int = 0, jitter: float = 1.0e-3,
                  pcg_tol=pcg_tol, pcg_maxit=pcg_maxit, name=f"drape{n}")
 
 
+def drop_on_obstacle(n: int, height: float, obstacle: Obstacle, *, ny: Optional[int] = None,
+                     seed: int = 0, jitter: float = 1.0e-3, dt: float = 0.01,
+                     pcg_tol: float = 1.0e-6, pcg_maxit: int = 1000, name: str = "") -> Cloth:
+    """Cloth of n x ny vertices (ny = n: square), 1 m along x with square
+    cells, alternating diagonals, flat at z=height with U(+-jitter) z-jitter,
+    falling onto `obstacle`; clearance 0.01."""
+    ny = n if ny is None else ny
+    xyz, tri, quad, area = grid_cloth(n, ny, 1.0, (ny - 1) / (n - 1), z=height, alternating=True,
+                                      jitter=jitter, seed=seed)
+    np_ = len(xyz) // 3
+    return Cloth(xyz0=xyz.copy(), xyz=xyz.copy(), uvw=np.zeros_like(xyz), tri=tri, quad=quad,
+                 bc=np.zeros(np_, dtype=np.int32), area=area, material=material_default(),
+                 obstacle=obstacle, dt=dt, clearance=0.01, pcg_tol=pcg_tol, pcg_maxit=pcg_maxit,
+                 name=name or f"drop{n}")
+
+
+def batch_drop(count: int = 64, *, seed: int = 0, side_range=(24, 128),
+               height_range=(0.3, 0.8), pcg_tol: float = 1.0e-6, pcg_maxit: int = 1000) -> list:
+    """Independent cloths of seeded sizes dropped from seeded heights, each
+    onto its own sphere (r 0.3) or box (half-extents 0.25) at the origin;
+    dt 0.01.  Draw order per cloth: side, height, obstacle kind, jitter seed."""
+    rng = SplitMix64(seed)
+    out = []
+    lo, hi = side_range
+    for k in range(count):
+        n = lo + int(rng.uniform() * (hi - lo + 1))
+        h = height_range[0] + (height_range[1] - height_range[0]) * rng.uniform()
+        ob = sphere((0.0, 0.0, 0.0), 0.3) if rng.uniform() < 0.5 else \
+            box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25))
+        js = int(rng.next_u64() & 0x7FFFFFFF)
+        out.append(drop_on_obstacle(n, h, ob, seed=js, pcg_tol=pcg_tol, pcg_maxit=pcg_maxit,
+                                    name=f"drop{n}_{k}"))
+    return out
+
+
 def fold_sweep(n: int = 256, *, size: float = 2.0, seed: int = 0, jitter: float = 1.0e-3,
                pcg_tol: float = 1.0e-6, pcg_maxit: int = 1000) -> Cloth:
     """Single large cloth folded onto itself.
@@ -377,5 +434,6 @@ __all__ = [
     "SplitMix64", "Obstacle", "Material", "Sweep", "Cloth", "sphere", "plane", "NO_OBSTACLE",
     "bending_quads", "grid_cloth", "reference_square", "material_default",
     "drape_over_sphere", "fold_sweep", "reference_demo", "apply_sweep",
-    "OBST_NONE", "OBST_PLANE", "OBST_SPHERE", "OBST_SPHERE_INV",
+    "OBST_NONE", "OBST_PLANE", "OBST_SPHERE", "OBST_SPHERE_INV", "OBST_BOX", "box",
+    "drop_on_obstacle", "batch_drop",
 ]

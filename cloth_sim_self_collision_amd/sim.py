@@ -47,7 +47,7 @@ def make_params(cloth: Cloth) -> _lib.Params:
     p.pcg_tol = cloth.pcg_tol
     p.pcg_maxit = cloth.pcg_maxit
     p.obstacle_kind = cloth.obstacle.kind
-    for k, v in enumerate(cloth.obstacle.params[:4]):
+    for k, v in enumerate(cloth.obstacle.params[:8]):
         p.obstacle[k] = v
     return p
 
@@ -75,6 +75,12 @@ class DeviceCloth:
             delta = f64(cloth.sweep.delta)
             check(lib.cs_set_sweep(self.h, len(verts), ip(verts), dp(delta),
                                    float(cloth.sweep.duration)))
+
+    def memory(self) -> int:
+        """Device bytes held by this cloth (arena + contact buffers)."""
+        b = C.c_longlong()
+        check(_lib.load().cs_cloth_memory(self.h, C.byref(b)))
+        return b.value
 
     # -- lifetime
     def close(self):
@@ -300,7 +306,7 @@ def element_bend(C_, c_, stiff):
 def element_contact(p, stiff, clearance, obstacle):
     p = f64(p).reshape(-1)
     n = len(p) // 3
-    o = f64(list(obstacle.params[:4]) + [0.0] * (4 - len(obstacle.params[:4])))
+    o = f64(list(obstacle.params[:8]) + [0.0] * (8 - len(obstacle.params[:8])))
     dW = np.zeros(3 * n)
     ddW = np.zeros(9 * n)
     check(_lib.load().cs_element_contact(n, dp(p), stiff, clearance, obstacle.kind, dp(o),

@@ -45,6 +45,9 @@ extern "C" {
 #define CS_OBST_PLANE 1      /* params (nx, ny, nz, d): pd = d - n.p        */
 #define CS_OBST_SPHERE 2     /* params (cx, cy, cz, r): pd = r - |p - c|    */
 #define CS_OBST_SPHERE_IN 3  /* sign-flipped sphere of self_contact_sparse/main.cpp:117-133 */
+#define CS_OBST_BOX 4        /* params (cx, cy, cz, hx, hy, hz), axis-aligned, half-extents h:
+                              * inside pd = distance to the nearest face (n its outward
+                              * normal, the first axis on ties), outside pd = -distance */
 
 typedef struct cs_cloth cs_cloth;
 
@@ -69,7 +72,7 @@ typedef struct {
   double pcg_tol;             /* the reference hard-codes 1e-4 ... */
   int pcg_maxit;              /* ... and 100 (solve_internal_sparse.h:318-319) */
   int obstacle_kind;
-  double obstacle[4];
+  double obstacle[8];
 } cs_params;
 
 typedef struct {
@@ -96,6 +99,9 @@ int cs_cloth_create(const cs_mesh* mesh, const cs_params* params, cs_cloth** out
 int cs_cloth_destroy(cs_cloth* c);
 int cs_cloth_set_params(cs_cloth* c, const cs_params* params);
 int cs_cloth_sizes(const cs_cloth* c, int* np, int* nt, int* nq, int* nnz, int* nnode);
+/* device memory held by the cloth now (its arena plus the contact buffers at
+ * their current capacity); what a server admits against its HBM budget */
+int cs_cloth_memory(const cs_cloth* c, long long* bytes);
 
 /* state (host arrays of 3*np doubles) */
 int cs_set_state(cs_cloth* c, const double* xyz, const double* uvw);
@@ -209,7 +215,8 @@ int cs_element_cst(int n, const double* C, const double* c, double lambda, doubl
 /* WdWddW_Bend for n stencils: C, c 12 doubles each; dW[12n], ddW[144n] */
 int cs_element_bend(int n, const double* C, const double* c, double stiff, double* dW,
                     double* ddW);
-/* WdWddW_Contact for n points against an obstacle: dW[3n], ddW[9n] */
+/* WdWddW_Contact for n points against an obstacle (parameters as in
+ * cs_params, 8 doubles): dW[3n], ddW[9n] */
 int cs_element_contact(int n, const double* p, double stiff, double clearance, int kind,
                        const double* obstacle, double* dW, double* ddW);
 

@@ -370,6 +370,29 @@ static void obstacle_depth(const double* obst, const double* p, double* pd, doub
       n[2] *= -1;
       *pd *= -1;
     }
+  } else if (kind == 4) {
+    /* axis-aligned box (center obst[1..3], half-extents obst[4..6]); not in
+     * the reference, which has only plane and sphere callbacks: the scene
+     * convention pd > 0 inside, n = push-out direction */
+    double s[3], d[3];
+    for (int i = 0; i < 3; i++) {
+      s[i] = p[i] - obst[1 + i];
+      d[i] = fabs(s[i]) - obst[4 + i];
+    }
+    if (d[0] <= 0.0 && d[1] <= 0.0 && d[2] <= 0.0) {
+      int m = 0;
+      if (d[1] > d[m]) m = 1;
+      if (d[2] > d[m]) m = 2;
+      n[0] = n[1] = n[2] = 0.0;
+      n[m] = s[m] < 0.0 ? -1.0 : 1.0;
+      *pd = -d[m];
+    } else {
+      double o[3];
+      for (int i = 0; i < 3; i++) o[i] = d[i] > 0.0 ? d[i] : 0.0;
+      const double len = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+      for (int i = 0; i < 3; i++) n[i] = (s[i] < 0.0 ? -o[i] : o[i]) / len;
+      *pd = -len;
+    }
   } else {
     *pd = -100;
     n[0] = n[1] = n[2] = 0;

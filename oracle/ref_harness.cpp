@@ -76,6 +76,30 @@ void pd_sphere_inv(double& pd, double* n, const double* p) {
   pd *= -1;
 }
 
+// axis-aligned box (center g_obst[1..3], half-extents g_obst[4..6]); the
+// scene convention of the other callbacks (pd > 0 inside, n push-out)
+void pd_box(double& pd, double* n, const double* p) {
+  double s[3], d[3];
+  for (int i = 0; i < 3; i++) {
+    s[i] = p[i] - g_obst[1 + i];
+    d[i] = std::fabs(s[i]) - g_obst[4 + i];
+  }
+  if (d[0] <= 0.0 && d[1] <= 0.0 && d[2] <= 0.0) {
+    int m = 0;
+    if (d[1] > d[m]) m = 1;
+    if (d[2] > d[m]) m = 2;
+    n[0] = n[1] = n[2] = 0.0;
+    n[m] = s[m] < 0.0 ? -1.0 : 1.0;
+    pd = -d[m];
+  } else {
+    double o[3];
+    for (int i = 0; i < 3; i++) o[i] = d[i] > 0.0 ? d[i] : 0.0;
+    const double len = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    for (int i = 0; i < 3; i++) n[i] = (s[i] < 0.0 ? -o[i] : o[i]) / len;
+    pd = -len;
+  }
+}
+
 typedef void (*PdFn)(double&, double*, const double*);
 PdFn pick_pd(const double* obst) {
   for (int i = 0; i < 8; i++) g_obst[i] = obst[i];
@@ -83,6 +107,7 @@ PdFn pick_pd(const double* obst) {
     case 1: return pd_plane;
     case 2: return pd_sphere;
     case 3: return pd_sphere_inv;
+    case 4: return pd_box;
     default: return pd_none;
   }
 }

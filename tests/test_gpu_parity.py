@@ -29,6 +29,8 @@ def _scenes():
         "refdemo0": scene.reference_demo(0),
         "refdemo1": scene.reference_demo(1),
         "fold24": scene.fold_sweep(24, size=0.3),
+        "box24": scene.drop_on_obstacle(24, 0.3, scene.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25)),
+                                        seed=5),
     }
 
 
@@ -118,7 +120,8 @@ def test_element_routines_bitwise(sim_mod, oracle_mod):
         assert np.array_equal(dWb[e], g) and np.array_equal(ddWb[e], h)
     p = rng.standard_normal((n, 3)) * 0.4
     for ob in (scene.sphere((0.0, 0.0, 0.0), 0.3), scene.plane((0.0, 0.0, 1.0), -0.1),
-               scene.Obstacle(scene.OBST_SPHERE_INV, (0.1, 0.0, 0.0, 0.35))):
+               scene.Obstacle(scene.OBST_SPHERE_INV, (0.1, 0.0, 0.0, 0.35)),
+               scene.box((0.05, 0.0, -0.1), (0.2, 0.25, 0.15))):
         g, h = sim_mod.element_contact(p, 1e3, 0.01, ob)
         for e in range(0, n, 17):
             _, g0, h0 = oracle_mod.element_contact(p[e], 1e3, 0.01, ob.as_array())
@@ -192,7 +195,8 @@ def test_velocity_update_and_host_obstacle_values(sim_mod, oracle_mod):
 
 @pytest.mark.parametrize("name,steps,variant", [("drape32", 100, 0), ("refdemo0", 60, 0),
                                                 ("refdemo1", 60, 0), ("fold24", 60, 0),
-                                                ("drape32", 60, 1), ("fold24", 40, 1)])
+                                                ("drape32", 60, 1), ("fold24", 40, 1),
+                                                ("box24", 60, 0)])
 def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant):
     cl = _scenes()[name]
     o = oracle_mod.Oracle(cl, "port")
@@ -271,3 +275,35 @@ def test_batch_matches_each_cloth_alone(sim_mod, oracle_mod):
     finally:
         for d in devs:
             d.close()
+
+
+def test_served_results_equal_alone(sim_mod, oracle_mod):
+    """Continuous batching: requests of different sizes arriving while others
+    run end bit-identical to the same simulations stepped alone, with a
+    generous HBM budget (all overlap) and a tight one (requests queue)."""
+    from cloth_sim_self_collision_amd import serve
+    sph = scene.sphere((0.0, 0.0, 0.0), 0.3)
+    bx = scene.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25))
+    reqs = [serve.Request(0, scene.drop_on_obstacle(20, 0.35, sph, seed=1), steps=12, arrival=0.0),
+            serve.Request(1, scene.drop_on_obstacle(16, 0.32, bx, ny=24, seed=2), steps=8,
+                          arrival=0.02),
+            serve.Request(2, scene.drape_over_sphere(16), steps=10, arrival=0.04)]
+    alone = {}
+    for r in reqs:
+        with sim_mod.DeviceCloth(r.cloth) as d:
+            d.step(r.steps)
+            alone[r.rid] = d.get_state()
+    # absolute parity of one request against the oracle
+    o = oracle_mod.Oracle(reqs[1].cloth, "port")
+    x, v = reqs[1].cloth.xyz.copy(), reqs[1].cloth.uvw.copy()
+    for _ in range(reqs[1].steps):
+        x, v, *_ = o.step(x, v)
+    assert np.array_equal(alone[1][0], x) and np.array_equal(alone[1][1], v)
+    biggest = max(r.cloth.np_ for r in reqs) * serve.BYTES_PER_VERTEX_ESTIMATE
+    for budget, realtime in ((1 << 36, True), (biggest + 1, False)):
+        out = serve.Server(budget).run(reqs, realtime=realtime)
+        assert [r.rid for r in out["results"]] == [0, 1, 2]
+        for res in out["results"]:
+            assert len(res.step_ms) == reqs[res.rid].steps
+            ax, av = alone[res.rid]
+            assert np.array_equal(res.xyz, ax) and np.array_equal(res.uvw, av), (budget, res.rid)

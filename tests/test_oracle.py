@@ -247,6 +247,39 @@ def test_contact_penalty_closed_form(oracle_mod):
     assert W == 0 and not dW.any() and not ddW.any()
 
 
+def test_box_obstacle_closed_form(oracle_mod):
+    """Axis-aligned box (center (0.1,0,0), half-extents (0.2,0.3,0.1)): depth
+    and normal from the geometry, in the oracle, the reference harness (the
+    callback handed to the reference) and the scene layer alike."""
+    ob = scene.box((0.1, 0.0, 0.0), (0.2, 0.3, 0.1))
+    k, delta = 1e3, 0.01
+    cases = [
+        # inside, nearest face +z at distance 0.02
+        ((0.15, 0.05, 0.08), 0.02, (0, 0, 1)),
+        # inside, nearest face -x at distance 0.03
+        ((-0.07, 0.0, 0.0), 0.03, (-1, 0, 0)),
+        # outside beyond the +y face by 0.004
+        ((0.0, 0.304, 0.0), -0.004, (0, 1, 0)),
+        # outside near the (+x, -z) edge: offsets (0.003, 0, -0.004) -> 0.005
+        ((0.303, 0.1, -0.104), -0.005, (0.6, 0, -0.8)),
+    ]
+    pts = np.array([c[0] for c in cases])
+    pd, n = ob.depth(pts)
+    for (p, want_pd, want_n), got_pd, got_n in zip(cases, pd, n):
+        assert np.isclose(got_pd, want_pd, atol=1e-12), (p, got_pd)
+        assert np.allclose(got_n, want_n, atol=1e-9), (p, got_n)
+        backends = ["port"] + (["reference"] if oracle_mod.available("reference") else [])
+        for be in backends:
+            W, dW, ddW = oracle_mod.element_contact(np.array(p), k, delta, ob.as_array(), backend=be)
+            a = got_pd + delta
+            if a > 0:  # penalty 1/2 k (pd + delta)^2, force along n
+                assert np.isclose(W, 0.5 * k * a * a)
+                assert np.allclose(dW, -k * a * np.asarray(want_n), atol=1e-9)
+                assert np.allclose(ddW.reshape(3, 3), k * np.outer(want_n, want_n), atol=1e-6)
+            else:
+                assert W == 0 and not dW.any() and not ddW.any()
+
+
 def test_coplanarity_time_known_answer(oracle_mod):
     # a vertex crossing the static plane z=0 at t* = 0.37
     s = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0.2, 0.2, 0.37], float)
