@@ -63,6 +63,7 @@ struct Cloth {
   int* hflags = nullptr;  // pinned
   double* hscal = nullptr;
   int* ising = nullptr;   // per-row singular-pivot flags
+  PcgAux aux;             // convergence-test reduction beside the sweep
   unsigned long long* sum_stats = nullptr;  // reduction counters (while profiling)
   ContactState contact;
   bool have_x1 = false;
@@ -192,6 +193,13 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&d.sum.pmin, ntile);
   A.reserve(&d.sum.pmax, ntile);
   A.reserve(&d.sum.ticket, 1);
+  A.reserve(&d.prod2, 3 * (size_t)np);
+  A.reserve(&d.sum2.tsum, ntile);
+  A.reserve(&d.sum2.code, ntile);
+  A.reserve(&d.sum2.tot, ntile);
+  A.reserve(&d.sum2.pmin, ntile);
+  A.reserve(&d.sum2.pmax, ntile);
+  A.reserve(&d.sum2.ticket, 1);
   A.reserve(&c->sum_stats, kNumSumStat);
   A.reserve(&d.scal, kNumScal);
   A.reserve(&d.flags, kNumFlag);
@@ -260,6 +268,7 @@ int build(Cloth* c, const cs_mesh* m) {
   d.nqb = S.nqb;
   hipStream_t s = c->stream;
   CS_HIP(hipMemsetAsync(d.sum.ticket, 0, sizeof(unsigned int), s));
+  CS_HIP(hipMemsetAsync(d.sum2.ticket, 0, sizeof(unsigned int), s));
   CS_HIP(hipMemcpyAsync(tri, m->tri, sizeof(int) * 3 * nt, hipMemcpyHostToDevice, s));
   if (nq) CS_HIP(hipMemcpyAsync(quad, m->quad, sizeof(int) * 4 * nq, hipMemcpyHostToDevice, s));
   std::vector<int> bcv(np, 0);
@@ -329,7 +338,8 @@ int internal_step(Cloth* c, double* conv, int* iters) {
   Prof* prof = c->prof.on ? &c->prof : nullptr;
   if (int st = launch_assemble(d, c->P, s, prof)) return st;
   if (int st = launch_ilu(d, s, prof)) return st;
-  if (int st = run_pcg(d, c->P.pcg_tol, c->P.pcg_maxit, c->hflags, s, c->poll, prof)) return st;
+  if (int st = run_pcg(d, c->P.pcg_tol, c->P.pcg_maxit, c->hflags, s, c->poll, prof, &c->aux))
+    return st;
   if (int st = launch_finish_internal(d, c->P, s)) return st;
   CS_HIP(hipMemcpyAsync(c->hflags, d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
   CS_HIP(hipMemcpyAsync(c->hscal, d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
@@ -375,6 +385,9 @@ int cs_cloth_create(const cs_mesh* mesh, const cs_params* params, cs_cloth** out
   c->P = to_params(params);
   int st = kOk;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->aux.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->aux.updated, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->aux.tested, hipEventDisableTiming) != hipSuccess ||
       hipHostMalloc((void**)&c->hflags, sizeof(int) * kNumFlag) != hipSuccess ||
       hipHostMalloc((void**)&c->hscal, sizeof(double) * kNumScal) != hipSuccess) {
     set_error("stream or pinned buffer creation failed");
@@ -400,6 +413,12 @@ int cs_cloth_destroy(cs_cloth* p) {
   if (c->hflags) (void)hipHostFree(c->hflags);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->aux.stream) {
+    (void)hipStreamSynchronize(c->aux.stream);
+    (void)hipStreamDestroy(c->aux.stream);
+  }
+  if (c->aux.updated) (void)hipEventDestroy(c->aux.updated);
+  if (c->aux.tested) (void)hipEventDestroy(c->aux.tested);
   delete c;
   return kOk;
 }
@@ -612,6 +631,7 @@ int cs_profile_enable(cs_cloth* p, int on) {
   c->prof.on = on != 0;
   CS_HIP(hipMemsetAsync(c->sum_stats, 0, sizeof(unsigned long long) * kNumSumStat, c->stream));
   c->d.sum.stats = on ? c->sum_stats : nullptr;
+  c->d.sum2.stats = c->d.sum.stats;
   return sync(c);
 }
 
@@ -738,7 +758,7 @@ int cs_pcg_solve(cs_cloth* p, const double* b, double tol, int maxit, double* x,
   hipStream_t s = c->stream;
   const size_t n = sizeof(double) * 3 * c->S.np;
   CS_HIP(hipMemcpyAsync(c->d.b, b, n, hipMemcpyHostToDevice, s));
-  if (int st = run_pcg(c->d, tol, maxit, c->hflags, s, c->poll)) return st;
+  if (int st = run_pcg(c->d, tol, maxit, c->hflags, s, c->poll, nullptr, &c->aux)) return st;
   if (x) CS_HIP(hipMemcpyAsync(x, c->d.dx, n, hipMemcpyDeviceToHost, s));
   if (r) CS_HIP(hipMemcpyAsync(r, c->d.r, n, hipMemcpyDeviceToHost, s));
   CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));

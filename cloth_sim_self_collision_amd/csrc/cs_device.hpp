@@ -121,6 +121,8 @@ struct DevCloth {
   double* p[2];
   double* prod;  // element-wise products feeding the next reduction
   SumScratch sum;  // tile summaries of the ordered reductions over prod
+  double* prod2;   // r.*r of the convergence test, reduced on the auxiliary stream
+  SumScratch sum2;
   double* scal;
   int* flags;
   // BVH boxes (6 per node)
@@ -185,12 +187,13 @@ struct Prof {
   hipStream_t stream = nullptr;
   double total_ms[kNumProf] = {0};
   long count[kNumProf] = {0};
-  // fold the recorded pairs into the totals (waits for the last event)
+  // fold the recorded pairs into the totals (waits for every pair: scopes
+  // may sit on more than one stream)
   void harvest() {
     if (used == 0) return;
-    (void)hipEventSynchronize(pool[used - 1]);
     for (size_t k = 0; k < kinds.size(); k++) {
       float ms = 0.f;
+      (void)hipEventSynchronize(pool[2 * k + 1]);
       if (hipEventElapsedTime(&ms, pool[2 * k], pool[2 * k + 1]) == hipSuccess) {
         total_ms[kinds[k]] += ms;
         count[kinds[k]] += 1;
@@ -212,7 +215,7 @@ struct Prof {
     used = 0;
     kinds.clear();
   }
-  int begin(int kind) {
+  int begin(int kind, hipStream_t on_stream = nullptr) {
     if (!on) return -1;
     if (used + 2 > 4096) harvest();
     if (used + 2 > pool.size()) {
@@ -225,21 +228,31 @@ struct Prof {
     const int id = (int)used;
     used += 2;
     kinds.push_back(kind);
-    (void)hipEventRecord(pool[id], stream);
+    (void)hipEventRecord(pool[id], on_stream ? on_stream : stream);
     return id;
   }
-  void end(int id) {
-    if (id >= 0) (void)hipEventRecord(pool[id + 1], stream);
+  void end(int id, hipStream_t on_stream = nullptr) {
+    if (id >= 0) (void)hipEventRecord(pool[id + 1], on_stream ? on_stream : stream);
   }
 };
 
 struct ProfScope {
   Prof* p;
   int id;
-  ProfScope(Prof* pr, int kind) : p(pr), id(pr ? pr->begin(kind) : -1) {}
+  hipStream_t s;
+  ProfScope(Prof* pr, int kind, hipStream_t on_stream = nullptr)
+      : p(pr), id(pr ? pr->begin(kind, on_stream) : -1), s(on_stream) {}
   ~ProfScope() {
-    if (p) p->end(id);
+    if (p) p->end(id, s);
   }
+};
+
+// a second stream for the convergence-test reduction, which runs beside the
+// ILU sweep of the same PCG iteration (null: everything on one stream)
+struct PcgAux {
+  hipStream_t stream = nullptr;
+  hipEvent_t updated = nullptr;   // r and r.*r written
+  hipEvent_t tested = nullptr;    // (r, r) reduced and tested
 };
 
 struct Cloth;  // host handle (cs_api.cpp)

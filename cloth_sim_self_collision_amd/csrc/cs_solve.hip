@@ -706,7 +706,7 @@ __global__ void k_update_rx(Sel sel, int it) {
   const double r = c.r[e] + -alpha * c.ap[e];
   c.r[e] = r;
   c.dx[e] = c.dx[e] + alpha * p[e];
-  c.prod[e] = r * r;
+  c.prod2[e] = r * r;
   if (c.fslot) {
     const int i = e / 3;
     c.sf[3 * c.fslot[i] + (e - 3 * i)] = r;  // the next sweep's input, slot order
@@ -751,6 +751,9 @@ struct DotSrc {
   int iter;
   __device__ SumJob job() const {
     const DevCloth c = sel.cloth();
+    // the convergence test reduces its own products with its own scratch: it
+    // runs beside the sweep and the (r, z) reduction
+    if (mode == kDotRr) return SumJob{c.prod2, 3 * c.np, c.sum2, c.flags + kDone};
     return SumJob{c.prod, 3 * c.np, c.sum, mode == kDotRr0 ? nullptr : c.flags + kDone};
   }
   __device__ void finish(double s) const {
@@ -896,7 +899,7 @@ int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStrea
 // Result in c.dx; conv/iters in the device scalars.  `host_flags` is pinned
 // memory used to poll for convergence every `poll` iterations.
 int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream_t s, int poll,
-            Prof* prof) {
+            Prof* prof, const PcgAux* aux) {
   const long n3 = 3L * c.np;
   {
     ProfScope ps(prof, kProfOther);
@@ -930,7 +933,18 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
       ProfScope ps(prof, kProfAxpy);
       hipLaunchKernelGGL(k_update_rx<One>, dim3(grid_for(n3)), dim3(kTB), 0, s, One{c, {}}, it);
     }
-    {
+    if (aux) {
+      // (r, r) and the convergence test beside the sweep; every later kernel
+      // of the solve checks the flag it sets, and x, r and the reported
+      // ratio and count do not depend on z
+      CS_HIP(hipEventRecord(aux->updated, s));
+      CS_HIP(hipStreamWaitEvent(aux->stream, aux->updated, 0));
+      {
+        ProfScope ps(prof, kProfReduce, aux->stream);
+        launch_dot(c, kDotRr, it, aux->stream);
+      }
+      CS_HIP(hipEventRecord(aux->tested, aux->stream));
+    } else {
       ProfScope ps(prof, kProfReduce);
       launch_dot(c, kDotRr, it, s);
     }
@@ -942,6 +956,7 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
       ProfScope ps(prof, kProfReduce);
       launch_dot(c, kDotRz, it, s);
     }
+    if (aux) CS_HIP(hipStreamWaitEvent(s, aux->tested, 0));
     CS_HIP(hipGetLastError());
     if ((it + 1) % poll == 0 || it + 1 == maxit) {
       CS_HIP(hipMemcpyAsync(host_flags, c.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));

@@ -24,7 +24,7 @@ int launch_matvec(const DevCloth& c, double alpha, const double* x, double beta,
 int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStream_t s);
 // Solve_PCG from dx = 0 with r = b; poll convergence every `poll` iterations
 int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream_t s, int poll,
-            Prof* prof = nullptr);
+            Prof* prof = nullptr, const PcgAux* aux = nullptr);
 // x[v] = rest[v] + off for the swept vertices
 int launch_apply_sweep(const DevCloth& c, const int* verts, int n, const double off[3],
                        hipStream_t s);
