@@ -129,6 +129,32 @@ def batch_measure(count: int, steps: int, seed: int = 0) -> dict:
             d.close()
 
 
+def serve_measure(count: int, rate: float, seed: int = 0) -> dict:
+    """A bounded sample of the serving workload (Poisson arrivals, rectangular
+    cloths of log-uniform triangle counts and aspect 1..4, sphere or box):
+    served with continuous batching, then the same requests one after another
+    with the plain step."""
+    from cloth_sim_self_collision_amd import serve
+    reqs = serve.poisson_stream(count, rate, seed=seed, tri_range=(1.0e3, 6.0e3),
+                                step_range=(5, 15))
+    served = serve.Server(hbm_budget_bytes=64 << 30).run(reqs, realtime=True)
+    print(f"[bench] served {count} requests in {served['wall_s']:.2f} s", file=sys.stderr, flush=True)
+    plain = serve.run_plain(reqs)
+    print(f"[bench] plain {count} requests in {plain['wall_s']:.2f} s", file=sys.stderr, flush=True)
+    same = all(np.array_equal(a.xyz, b[1]) and np.array_equal(a.uvw, b[2])
+               for a, b in zip(served["results"], plain["results"]))
+    return {"workload": f"poisson{count}", "requests": count, "rate_per_s": rate,
+            "triangles": [int(r.cloth.nt) for r in reqs], "steps": [r.steps for r in reqs],
+            "served_vertex_steps_per_s": served["vertex_steps_per_s"],
+            "served_p50_step_ms": served["p50_step_ms"], "served_p99_step_ms": served["p99_step_ms"],
+            "served_wall_s": served["wall_s"],
+            "plain_vertex_steps_per_s": plain["vertex_steps_per_s"],
+            "plain_p50_step_ms": plain["p50_step_ms"], "plain_p99_step_ms": plain["p99_step_ms"],
+            "plain_wall_s": plain["wall_s"],
+            "served_vs_plain": served["vertex_steps_per_s"] / plain["vertex_steps_per_s"],
+            "results_identical": bool(same)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +170,9 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=64,
                     help="cloths of the batch workload measured beside the headline (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--serve-count", type=int, default=12,
+                    help="requests of the serving sample measured beside the headline (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, 1/s")
     ap.add_argument("--copy-mb", type=float, default=512.0,
                     help="size of the streaming-copy probe that measures the copy peak")
     args = ap.parse_args()
@@ -210,6 +239,7 @@ def main() -> int:
                 "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None}
     d.close()
     batch = batch_measure(args.batch_count, args.batch_steps) if args.batch_count > 0 else None
+    served = serve_measure(args.serve_count, args.serve_rate) if args.serve_count > 0 else None
     cpu, err = (None, None) if args.no_cpu else cpu_baseline(workload(args.workload), first_x)
     out = {
         "metric": "vertex-steps/s + p99 step latency, served vs plain GPU, at <=1e-5 m pos err",
@@ -231,6 +261,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "batch": batch,
+        "served": served,
     }
     print(json.dumps(out))
     return 0
