@@ -846,6 +846,7 @@ struct Batch {
   int* dgather = nullptr;
   int* hgather = nullptr;  // pinned
   hipStream_t stream = nullptr;
+  PcgAux aux;  // convergence-test reductions beside the sweeps
   BatchView view{};
   double tol = 0.0;
   int maxit = 0;
@@ -860,6 +861,12 @@ Batch* HB(cs_batch* p) { return reinterpret_cast<Batch*>(p); }
 
 void batch_free(Batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->aux.stream) {
+    (void)hipStreamSynchronize(b->aux.stream);
+    (void)hipStreamDestroy(b->aux.stream);
+  }
+  if (b->aux.updated) (void)hipEventDestroy(b->aux.updated);
+  if (b->aux.tested) (void)hipEventDestroy(b->aux.tested);
   if (b->dtab) (void)hipFree(b->dtab);
   if (b->ptab) (void)hipFree(b->ptab);
   if (b->dgather) (void)hipFree(b->dgather);
@@ -928,6 +935,9 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
   b->htab.resize(n);
   b->hptab.resize(n);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->aux.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->aux.updated, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->aux.tested, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void**)&b->dtab, sizeof(DevCloth) * n) != hipSuccess ||
       hipMalloc((void**)&b->ptab, sizeof(Params) * n) != hipSuccess ||
       hipMalloc((void**)&b->dgather, sizeof(int) * n) != hipSuccess ||
@@ -977,7 +987,8 @@ extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
                                c->sweep_delta[2] * sv};
         if (int st = launch_apply_sweep(c->d, c->sweep_verts, c->n_sweep, off, s)) return st;
       }
-    if (int st = batch_internal_step(b->view, b->tol, b->maxit, b->poll, b->dgather, b->hgather, s))
+    if (int st = batch_internal_step(b->view, b->tol, b->maxit, b->poll, b->dgather, b->hgather, s,
+                                     nullptr, &b->aux))
       return st;
     for (Cloth* c : b->cl) {
       CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));

@@ -133,9 +133,9 @@ struct DevCloth {
   int wmax;           // widest level of either sweep
   int sweep_variant;  // 0 automatic (quad sweep), 1 row-by-row
   const int* f_rci;  // {row, block count, backward slot, 0} per forward slot
-  const int* f_col;  // [((k / 4) * np + slot) * 4 + k % 4]
+  const int* f_col;  // column's forward slot, [((k / 4) * np + slot) * 4 + k % 4]
   const int* b_rc;   // {row, block count} per backward slot
-  const int* b_col;
+  const int* b_col;  // column's backward slot, the same layout
   const int* fslot;  // forward slot of each row
   double* sf;        // sweep right-hand side in forward-slot order [3 * slot + r]
   double* sb;        // forward result in backward-slot order
@@ -151,6 +151,13 @@ struct DevCloth {
 // (single-cloth launches).  Many: entry blockIdx.y of device tables, one grid
 // row per cloth (batched launches); grids are sized for the largest cloth and
 // blocks past a cloth's own size return at once.
+// search direction of PCG iteration `it` (p_new = pcg_dir(c, it), p_old =
+// pcg_dir(c, it + 1)).  A select, not c.p[it & 1]: a dynamic index into the
+// struct would keep a per-thread copy of a batch table entry in scratch memory.
+__device__ __forceinline__ double* pcg_dir(const DevCloth& c, int it) {
+  return (it & 1) ? c.p[1] : c.p[0];
+}
+
 struct One {
   DevCloth c;
   Params P;

@@ -319,16 +319,18 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
         s.f_rci[4 * sl] = i;
         s.f_rci[4 * sl + 1] = fc;
         s.f_rci[4 * sl + 2] = bslot[i];
-        for (int k = 0; k < fc; k++) s.f_col[cidx(k, sl)] = s.colind[s.rowptr[i] + k];
+        for (int k = 0; k < fc; k++) s.f_col[cidx(k, sl)] = s.fslot[s.colind[s.rowptr[i] + k]];
         const int j = s.lb_rows[sl];
         const int bc = s.rowptr[j + 1] - s.diaind[j];
         s.b_rc[2 * sl] = j;
         s.b_rc[2 * sl + 1] = bc;
-        for (int k = 0; k < bc; k++) s.b_col[cidx(k, sl)] = s.colind[s.diaind[j] + k];
+        for (int k = 0; k < bc; k++) s.b_col[cidx(k, sl)] = bslot[s.colind[s.diaind[j] + k]];
       }
       // live interval of a row in a sweep: [level written, last level read];
-      // rows that share a ring slot must have disjoint intervals
-      auto check = [&](const std::vector<int>& lvl, bool lower) {
+      // rows whose sweep slots share a ring position must have disjoint
+      // intervals.  Slots run in level order, so the ring is a FIFO whose
+      // window only has to cover the rows live across a few levels.
+      auto check = [&](const std::vector<int>& lvl, bool lower, const std::vector<int>& slot_of) {
         std::vector<int> last(np);
         for (int i = 0; i < np; i++) last[i] = lvl[i];
         for (int i = 0; i < np; i++) {
@@ -337,7 +339,7 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
           for (int k = b; k < e; k++) last[s.colind[k]] = std::max(last[s.colind[k]], lvl[i]);
         }
         std::vector<std::vector<std::pair<int, int>>> slot(kRingRows);
-        for (int i = 0; i < np; i++) slot[i & (kRingRows - 1)].push_back({lvl[i], last[i]});
+        for (int i = 0; i < np; i++) slot[slot_of[i] & (kRingRows - 1)].push_back({lvl[i], last[i]});
         for (auto& v : slot) {
           std::sort(v.begin(), v.end());
           for (size_t q = 1; q < v.size(); q++)
@@ -350,7 +352,7 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
         for (int q = s.lf_ptr[l]; q < s.lf_ptr[l + 1]; q++) lf[s.lf_rows[q]] = l;
       for (int l = 0; l < s.nlb; l++)
         for (int q = s.lb_ptr[l]; q < s.lb_ptr[l + 1]; q++) lb[s.lb_rows[q]] = l;
-      s.ring_ok = check(lf, true) && check(lb, false);
+      s.ring_ok = check(lf, true, s.fslot) && check(lb, false, bslot);
     } else {
       s.kl = s.ku = 0;
     }

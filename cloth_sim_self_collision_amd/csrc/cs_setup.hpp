@@ -31,11 +31,12 @@ struct MeshSetup {
   // packed sweeps: rows in level order ("slots"), at most kl lower / ku upper
   // blocks per row.  f_rci[4*slot] = {row, block count, the row's backward
   // slot, 0}, b_rc[2*slot] = {row, block count}, fslot[row] = forward slot;
-  // column indices in groups of four per slot: col[((k / 4) * np + slot) * 4 + k % 4]
+  // per block, the slot of its column row in the same sweep (the LDS ring is
+  // addressed by slot), in groups of four: col[((k / 4) * np + slot) * 4 + k % 4]
   int kl = 0, ku = 0;
   std::vector<int> f_rci, b_rc, fslot, f_col, b_col;
-  // the LDS ring of kRingRows rows can hold the sweep dependencies: no two rows
-  // sharing a ring slot are live at the same level
+  // the LDS ring of kRingRows entries (entry = slot mod kRingRows) can hold the
+  // sweep dependencies: no two rows sharing an entry are live at the same level
   bool ring_ok = false;
   // gather lists in element order (assembly without atomics, in the
   // reference's summation order)

@@ -427,10 +427,11 @@ __device__ __forceinline__ void lds_barrier() {
 
 struct QuadRow {
   int i;     // row, -1: none for this quad in this sub-level
+  int s;     // slot in this sweep (the row's ring entry is s mod kRingRows)
   int cnt;   // blocks of the row
   int bs;    // forward: the row's backward slot
   double t;  // this lane's component of the right-hand side
-  int col[kMaxBlocksPerRow];
+  int col[kMaxBlocksPerRow];  // slots of the column rows (ring entries)
   double a[kMaxBlocksPerRow * 3];
   double d[3];
 };
@@ -448,6 +449,7 @@ __device__ __forceinline__ void quad_load(const DevCloth& c, const int* qptr, in
   const bool valid = l < nq && s0 < qptr[lc + 1];
   const int sl = valid ? s0 : 0;
   const int np = c.np;
+  q.s = sl;
   if (kFwd) {
     const int4 v = reinterpret_cast<const int4*>(c.f_rci)[sl];
     q.i = valid ? v.x : -1;
@@ -518,7 +520,7 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], doub
     z = q.d[0] * t0 + q.d[1] * t1 + q.d[2] * t2;
   }
   const bool own = has && lane < 3;
-  ring[lane < 3 ? lane : 2][own ? (q.i & kMask) : kRingRows] = z;
+  ring[lane < 3 ? lane : 2][own ? (q.s & kMask) : kRingRows] = z;
   if (own) {
     if (kFwd)
       out[3 * q.bs + lane] = z;  // backward-slot order
@@ -644,8 +646,8 @@ template <class Sel>
 __global__ void k_spmv_dir(Sel sel, int it) {
   const DevCloth c = sel.cloth();
   const int first = it == 0;
-  const double* __restrict__ pold = c.p[(it + 1) & 1];
-  double* __restrict__ pnew = c.p[it & 1];
+  const double* __restrict__ pold = pcg_dir(c, it + 1);
+  double* __restrict__ pnew = pcg_dir(c, it);
   if (c.flags[kDone]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.np) return;
@@ -698,7 +700,7 @@ __global__ void k_spmv_dir(Sel sel, int it) {
 template <class Sel>
 __global__ void k_update_rx(Sel sel, int it) {
   const DevCloth c = sel.cloth();
-  const double* __restrict__ p = c.p[it & 1];
+  const double* __restrict__ p = pcg_dir(c, it);
   if (c.flags[kDone]) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
@@ -1026,7 +1028,7 @@ static void launch_dot_batch(const BatchView& b, int mode, int iter, hipStream_t
 }
 
 int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int* dgather,
-                        int* hgather, hipStream_t s, Prof* prof) {
+                        int* hgather, hipStream_t s, Prof* prof, const PcgAux* aux) {
   const Many m{b.tab, b.ptab};
   const int B = b.n;
   const unsigned gv = grid_for(3L * b.max_np), gp = grid_for(b.max_np);
@@ -1078,7 +1080,16 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
       ProfScope ps(prof, kProfAxpy);
       hipLaunchKernelGGL(k_update_rx<Many>, dim3(gv, B), dim3(kTB), 0, s, m, it);
     }
-    {
+    if (aux) {
+      // as in run_pcg: the convergence test of every cloth beside the sweep
+      CS_HIP(hipEventRecord(aux->updated, s));
+      CS_HIP(hipStreamWaitEvent(aux->stream, aux->updated, 0));
+      {
+        ProfScope ps(prof, kProfReduce, aux->stream);
+        launch_dot_batch(b, kDotRr, it, aux->stream);
+      }
+      CS_HIP(hipEventRecord(aux->tested, aux->stream));
+    } else {
       ProfScope ps(prof, kProfReduce);
       launch_dot_batch(b, kDotRr, it, s);
     }
@@ -1090,6 +1101,7 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
       ProfScope ps(prof, kProfReduce);
       launch_dot_batch(b, kDotRz, it, s);
     }
+    if (aux) CS_HIP(hipStreamWaitEvent(s, aux->tested, 0));
     CS_HIP(hipGetLastError());
     if ((it + 1) % poll == 0 || it + 1 == maxit) {
       hipLaunchKernelGGL(k_gather_done, dim3(grid_for(B)), dim3(kTB), 0, s, b.tab, B, dgather);

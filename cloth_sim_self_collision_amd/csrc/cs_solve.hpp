@@ -43,9 +43,11 @@ struct BatchView {
   int kpack;   // ... with this many packed blocks per row
 };
 // StepTime_InternalDynamicsILU for every cloth of the batch (start positions
-// saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); gather/hgather hold n
-// ints (device / pinned host) for the convergence poll
+// saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); dgather/hgather hold n
+// ints (device / pinned host) for the convergence poll; with aux, the (r, r)
+// reductions and tests run on aux->stream beside the sweeps
 int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int* dgather,
-                        int* hgather, hipStream_t s, Prof* prof = nullptr);
+                        int* hgather, hipStream_t s, Prof* prof = nullptr,
+                        const PcgAux* aux = nullptr);
 
 }  // namespace cs
