@@ -162,7 +162,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fold256")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--no-profile", action="store_true", help="skip the profiled steps")
+    ap.add_argument("--profile-steps", type=int, default=1,
+                    help="steps after the timed ones with per-kernel events (kernel_ms, roofline)")
     ap.add_argument("--maxit", type=int, default=None,
                     help="PCG iteration cap override (short profiling runs only)")
     ap.add_argument("--sweep-variant", type=int, default=0,
@@ -204,8 +206,7 @@ def main() -> int:
               flush=True)
         if k == 0:
             first_x, _ = d.get_state()
-    if not args.no_profile:
-        d.profile(True)
+    # timed steps without per-kernel events (their harvest synchronises)
     lat = []
     stats = []
     t_start = time.perf_counter()
@@ -216,9 +217,19 @@ def main() -> int:
         print(f"[bench] step {len(lat) - 1}: {lat[-1]:.3f} s, pcg {stats[-1]['pcg_iters']}",
               file=sys.stderr, flush=True)
     t_total = time.perf_counter() - t_start
-    prof = d.profile_read() if not args.no_profile else {}
-    red = d.reduction_stats() if not args.no_profile else None
-    d.profile(False)
+    # per-kernel-class times and reduction counters from further steps,
+    # outside the timed region
+    prof, red = {}, None
+    if not args.no_profile and args.profile_steps > 0:
+        d.profile(True)
+        for k in range(args.profile_steps):
+            t0 = time.perf_counter()
+            d.step(1)
+            print(f"[bench] profiled step {k}: {time.perf_counter() - t0:.3f} s", file=sys.stderr,
+                  flush=True)
+        prof = d.profile_read()
+        red = d.reduction_stats()
+        d.profile(False)
 
     value = cloth.np_ * args.steps / t_total
     lat_ms = np.array(lat) * 1e3
@@ -257,6 +268,7 @@ def main() -> int:
         "contacts": {"n_prox": stats[-1]["n_prox"], "n_ccd": stats[-1]["n_ccd"],
                      "n_riz_pass": stats[-1]["n_riz_pass"]},
         "kernel_ms": {k: round(v[0], 3) for k, v in prof.items()} if prof else None,
+        "kernel_ms_steps": args.profile_steps if prof else 0,
         "reduction_stats": red,
         "roofline": roof,
         "cpu_baseline": cpu,

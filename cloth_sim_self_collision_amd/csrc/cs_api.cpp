@@ -641,7 +641,13 @@ int cs_reduction_stats(cs_cloth* p, long long* out) {
   unsigned long long h[kNumSumStat];
   CS_HIP(hipMemcpyAsync(h, c->sum_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   if (int st = sync(c)) return st;
-  for (int k = 0; k < kNumSumStat; k++) out[k] = (long long)h[k];
+  for (int k = 0; k < kStatChainTicks; k++) out[k] = (long long)h[k];
+  // wall-clock ticks to nanoseconds (the rate is in kHz)
+  int dev = 0, khz = 0;
+  CS_HIP(hipGetDevice(&dev));
+  CS_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  for (int k = kStatChainTicks; k < kNumSumStat; k++)
+    out[k] = khz > 0 ? (long long)((double)h[k] * 1.0e6 / khz) : 0;
   return kOk;
 }
 

@@ -133,9 +133,9 @@ struct DevCloth {
   int wmax;           // widest level of either sweep
   int sweep_variant;  // 0 automatic (quad sweep), 1 row-by-row
   const int* f_rci;  // {row, block count, backward slot, 0} per forward slot
-  const int* f_col;  // column's forward slot, [((k / 4) * np + slot) * 4 + k % 4]
+  const int* f_col;  // ring byte offset of the column's forward slot, [((k / 4) * np + slot) * 4 + k % 4]
   const int* b_rc;   // {row, block count} per backward slot
-  const int* b_col;  // column's backward slot, the same layout
+  const int* b_col;  // the same for the backward slots
   const int* fslot;  // forward slot of each row
   double* sf;        // sweep right-hand side in forward-slot order [3 * slot + r]
   double* sb;        // forward result in backward-slot order

@@ -20,22 +20,25 @@
 // inclusive integer prefixes, or "no summary" if some addend is a tie or too
 // large.  None of this depends on the exact running sum.
 //
-// Chain (the last block to finish).  One thread walks the tiles in order with
-// the exact running sum s.  When s lies in the guessed binade and
-// S + min >= 2^52 + 1 and S + max <= 2^53 - 1 (mirrored for negative s),
-// every addition of the tile stays in the binade, so the tile is applied as
-// S += total.  Otherwise the tile is summed exactly by the whole block with
+// Chain (the first wave of the last block to finish).  The wave walks the
+// tiles in order with the exact running sum s, held by every lane.  When s
+// lies in a tile's guessed binade and S + min >= 2^52 + 1 and
+// S + max <= 2^53 - 1 (mirrored for negative s), every addition of the tile
+// stays in the binade and the tile is applied as S += total.  Sixty-four
+// tiles are tested per step: a prefix scan of their totals gives each tile
+// its S, a ballot the first that fails.  A failing tile is summed exactly by
 // the binade walker below.  A wrong guess costs time, never accuracy.
 //
-// Binade walker (ordered_sum_walk).  Per step the block forms the integer
-// prefix for the binade of the current s over the rest of the tile and finds
-// the first addend where the reasoning above does not apply (tie,
-// out-of-range prefix, huge or non-finite addend, s zero/subnormal/
-// non-finite).  Everything before it is exact integer work.  Where binade
-// changes cluster (the running sum swinging through zero under heavy
-// cancellation) one thread continues with plain IEEE additions from an LDS
-// copy of the tile (a "burst") until the running sum has kept its binade for
-// kBurstQuiet elements.  Both paths perform exactly the sequential additions.
+// Binade walker (wave_walk).  Per step the wave forms the integer prefix for
+// the binade of the current s over the rest of the tile, 256 addends (four
+// per lane) at a time, and finds the first addend where the reasoning above
+// does not apply (tie, out-of-range prefix, huge or non-finite addend, s
+// zero/subnormal/non-finite).  Everything before it is exact integer work.
+// Where binade changes cluster (the running sum swinging through zero under
+// heavy cancellation) the wave continues with plain IEEE additions from an
+// LDS copy of the tile (a "burst") until the running sum has kept its binade
+// for kBurstQuiet elements.  Both paths perform exactly the sequential
+// additions.  The chain uses no block barrier.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,12 +51,12 @@ namespace cs {
 constexpr int kSumBlock = 256;                  // threads per block
 constexpr int kSumPer = 4;                      // consecutive elements per thread
 constexpr int kSumTile = kSumBlock * kSumPer;   // elements per tile
-constexpr int kSumBuf = kSumTile + kSumTile / 16;
+constexpr int kSumBuf = kSumTile + 256;         // a tile plus the overreach of the last chunk
 constexpr int kSumWaves = kSumBlock / 64;
 constexpr int kChainWin = 1024;   // tile summaries staged in LDS at a time
-constexpr int kBurstMin = 64;     // elements a burst always covers
-constexpr int kBurstQuiet = 64;   // binade-stable elements that end a burst
-constexpr int kBurstFinish = 256; // a burst this close to the tile end finishes it
+constexpr int kBurstMin = 32;     // elements a burst always covers
+constexpr int kBurstQuiet = 32;   // binade-stable elements that end a burst
+constexpr int kBurstFinish = 64;  // a burst this close to the tile end finishes it
 
 // device scratch of the reduction, one entry per tile
 struct SumScratch {
@@ -64,10 +67,14 @@ struct SumScratch {
   long long* pmax;       // maximum inclusive integer prefix
   unsigned int* ticket;  // blocks finished; zero between launches
   // optional counters (null: off): sums, tiles, tiles applied in O(1),
-  // walker steps, burst elements
+  // walker steps, burst elements; device wall-clock ticks spent in the
+  // chain, in the walker (bursts included) and in bursts
   unsigned long long* stats;
 };
-enum SumStat : int { kStatSums = 0, kStatTiles, kStatFast, kStatSteps, kStatBurst, kNumSumStat };
+enum SumStat : int {
+  kStatSums = 0, kStatTiles, kStatFast, kStatSteps, kStatBurst,
+  kStatChainTicks, kStatWalkTicks, kStatBurstTicks, kNumSumStat
+};
 
 __host__ __device__ inline int sum_tiles(int n) { return n > 0 ? (n + kSumTile - 1) / kSumTile : 1; }
 
@@ -76,18 +83,13 @@ struct SumShared {
   long long wv2[kSumWaves];
   double wd[kSumWaves];
   int wi[kSumWaves];
-  int pos;
-  int tile;
   int last;
-  double s;
-  double buf[kSumBuf];
+  alignas(16) double buf[kSumBuf];
   int code[kChainWin];
   long long tot[kChainWin];
   long long pmin[kChainWin];
   long long pmax[kChainWin];
 };
-
-__device__ __forceinline__ int sum_pad(int i) { return i + (i >> 4); }
 
 __device__ __forceinline__ int sign_exp(double x) {
   return (int)((unsigned long long)__double_as_longlong(x) >> 52);
@@ -120,19 +122,6 @@ __device__ __forceinline__ long long block_excl_scan64(long long x, SumShared& s
   total = tot;
   __syncthreads();
   return base + inc - x;
-}
-
-__device__ __forceinline__ int block_min_int(int x, SumShared& sh) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
-  if (lane == 0) sh.wi[wid] = x;
-  __syncthreads();
-  int m = sh.wi[0];
-#pragma unroll
-  for (int w = 1; w < kSumWaves; w++) m = min(m, sh.wi[w]);
-  __syncthreads();
-  return m;
 }
 
 __device__ __forceinline__ void block_minmax64(long long& mn, long long& mx, SumShared& sh) {
@@ -171,138 +160,223 @@ __device__ __forceinline__ double block_sum_approx(double x, SumShared& sh) {
   return s;
 }
 
-// Sequential IEEE additions of buf[m - base ...] onto s by one thread;
-// returns the first element not yet added.
-__device__ inline int ordered_sum_burst(const double* buf, int base, int m, int end, double& s) {
+// ---------------------------------------------------------------------------
+// one-wave chain
+
+// LDS written by some lanes of the wave and read by others: the fences keep
+// the compiler from moving accesses across (one wave's LDS operations
+// execute in order)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ long long dpp_add64(long long x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), kCtrl, kRowMask, 0xf, false);
+  return x + (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// inclusive prefix sum over the 64 lanes: Hillis-Steele inside each row of
+// 16 lanes (row_shr 1, 2, 4, 8), then the row totals (row_bcast 15 into rows
+// 1 and 3, row_bcast 31 into rows 2 and 3); source lanes outside a row or
+// rows outside the mask contribute 0
+__device__ __forceinline__ long long wave_scan_dpp64(long long x) {
+  x = dpp_add64<0x111, 0xf>(x);
+  x = dpp_add64<0x112, 0xf>(x);
+  x = dpp_add64<0x114, 0xf>(x);
+  x = dpp_add64<0x118, 0xf>(x);
+  x = dpp_add64<0x142, 0xa>(x);
+  x = dpp_add64<0x143, 0xc>(x);
+  return x;
+}
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), kCtrl, kRowMask, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), kCtrl, kRowMask, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// exclusive prefix sum over the 64 lanes in doubles: shift by one lane
+// (wave_shr 1, lane 0 reads +0), then the inclusive scan above
+__device__ __forceinline__ double wave_excl_scan_f64(double x) {
+  x = dpp_f64<0x138, 0xf>(x);
+  x = x + dpp_f64<0x111, 0xf>(x);
+  x = x + dpp_f64<0x112, 0xf>(x);
+  x = x + dpp_f64<0x114, 0xf>(x);
+  x = x + dpp_f64<0x118, 0xf>(x);
+  x = x + dpp_f64<0x142, 0xa>(x);
+  x = x + dpp_f64<0x143, 0xc>(x);
+  return x;
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                          __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+
+__device__ __forceinline__ long long readlane64(long long x, int lane) {
+  const int lo = __builtin_amdgcn_readlane((int)x, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// s is a normal double (its integer view exists)
+__device__ __forceinline__ bool sum_normal(double s) {
+  return fabs(s) >= DBL_MIN && fabs(s) <= DBL_MAX;
+}
+
+// Sequential IEEE additions of buf[m ...] onto s; every lane performs the
+// same additions (uniform LDS reads).  Ends once the running sum kept its
+// binade for kBurstQuiet elements (after kBurstMin at least), or at len when
+// that is within kBurstFinish; returns the first element not yet added.
+// Reads up to buf[len + 31].
+__device__ __forceinline__ void burst_load(const double* buf, int k, double (&w)[16]) {
+#pragma unroll
+  for (int j = 0; j < 16; j++) w[j] = buf[k + j];
+}
+
+// adds w[0 .. min(16, len - k)) onto s (k < len) and loads the next 16
+// into wn behind the first addition: waiting for w then waits only for loads
+// issued a whole block earlier.  True when the burst ends after these.
+__device__ __forceinline__ bool burst_block(const double (&w)[16], double (&wn)[16],
+                                            const double* buf, int m, int len, int& k,
+                                            int& quiet, double& s) {
+  const int hb = sign_exp(s);
+  s = s + w[0];
+  __builtin_amdgcn_sched_barrier(0);
+  burst_load(buf, k + 16, wn);
+  __builtin_amdgcn_sched_barrier(0);
+  if (len - k >= 16) {
+#pragma unroll
+    for (int j = 1; j < 16; j++) s = s + w[j];
+    k += 16;
+  } else {
+#pragma unroll
+    for (int j = 1; j < 16; j++)
+      if (k + j < len) s = s + w[j];
+    k = len;
+  }
+  if (sign_exp(s) != hb) quiet = k;
+  return k >= len || (k - m >= kBurstMin && k - quiet >= kBurstQuiet && len - k > kBurstFinish);
+}
+
+__device__ inline int wave_burst(const double* buf, int m, int len, double& s) {
   int k = m, quiet = m;
-  while (k < end) {
-    const int hb = sign_exp(s);
-    if (end - k >= 16) {
-      double w[16];
-#pragma unroll
-      for (int j = 0; j < 16; j++) w[j] = buf[sum_pad(k - base + j)];
-#pragma unroll
-      for (int j = 0; j < 16; j++) s = s + w[j];
-      k += 16;
-    } else {
-      for (; k < end; k++) s = s + buf[sum_pad(k - base)];
-    }
-    if (sign_exp(s) != hb) quiet = k;
-    if (k - m >= kBurstMin && k - quiet >= kBurstQuiet && end - k > kBurstFinish) break;
+  double wa[16], wb[16];  // the block being added and the next, in turn
+  burst_load(buf, k, wa);
+  while (true) {
+    if (burst_block(wa, wb, buf, m, len, k, quiet, s)) break;
+    if (burst_block(wb, wa, buf, m, len, k, quiet, s)) break;
   }
   return k;
 }
 
-// Adds t[begin, end) (at most one tile) onto sh.s exactly; called by all
-// threads of the block, the result is in sh.s.
-__device__ inline void ordered_sum_walk(const double* __restrict__ t, int begin, int end,
-                                        SumShared& sh, unsigned long long* stats) {
-  unsigned long long steps = 0, burst = 0;
-  const int tid = threadIdx.x;
-  const double two62 = 4611686018427387904.0;  // 2^62
-  const long long lo52 = (1LL << 52) + 1, hi53 = (1LL << 53) - 1;
-  double v[kSumPer];
-#pragma unroll
-  for (int j = 0; j < kSumPer; j++) {
-    const int idx = begin + tid * kSumPer + j;
-    v[j] = idx < end ? t[idx] : 0.0;
-    sh.buf[sum_pad(tid * kSumPer + j)] = v[j];
-  }
-  __syncthreads();
-  int pos = begin;
-  while (pos < end) {
-    const double s = sh.s;
+// Adds buf[0, len) (a tile, len <= kSumTile) onto s exactly and returns the
+// result; one wave, s the same in every lane.  Reads up to buf[kSumBuf - 1].
+__device__ inline double wave_walk(const double* buf, int len, double s,
+                                   unsigned long long* stats) {
+  const int lane = threadIdx.x & 63;
+  const double two52 = 4503599627370496.0;  // 2^52
+  const double lo52 = two52 + 1.0, hi53 = 2.0 * two52 - 1.0;
+  unsigned long long steps = 0, burst = 0, burst_ticks = 0;
+  const unsigned long long walk_t0 = stats ? wall_clock64() : 0;
+  int pos = 0;
+  while (pos < len) {
     if (isnan(s)) break;  // NaN absorbs every later addend
     steps++;
-    // modes: 0 zero, 1 subnormal, 2 normal, 3 infinite
-    int mode;
-    int scale = 0;
-    long long S = 0;
-    if (s == 0.0) {
-      mode = 0;
-    } else if (isinf(s)) {
-      mode = 3;
-    } else if (fabs(s) < DBL_MIN) {
-      mode = 1;
-    } else {
-      mode = 2;
-      scale = 52 - ilogb(s);
-      S = (long long)ldexp(s, scale);
-    }
-    long long pref[kSumPer];
-    long long run = 0;
-    int bad = INT_MAX;
+    int m = len;  // first addend the integer view (or the s-keeping test) rejects
+    if (sum_normal(s)) {
+      // the integer view in doubles: every value used is an integer of
+      // magnitude below 2^53 (exact) up to the first rejected addend
+      const int scale = 52 - ilogb(s);
+      const double sg = s < 0.0 ? -1.0 : 1.0;
+      double S = ldexp(s, scale);
+      double Sm = 0.0;
+      for (int c0 = pos & ~3; c0 < len; c0 += 4 * 64) {
+        const int i0 = c0 + 4 * lane;
+        const double2 xa = *reinterpret_cast<const double2*>(buf + i0);
+        const double2 xb = *reinterpret_cast<const double2*>(buf + i0 + 2);
+        const double x[4] = {xa.x, xa.y, xb.x, xb.y};
+        double inc[4];
+        double run = 0.0;
+        unsigned usem = 0, failm = 0;  // bit j: addend j in [pos, len) / rejected
+        // branch-free per addend; |r| >= 2^52 always leaves the binade, so it
+        // is rejected with the ties
 #pragma unroll
-    for (int j = 0; j < kSumPer; j++) {
-      const int idx = begin + tid * kSumPer + j;
-      pref[j] = run;
-      if (idx < pos || idx >= end || bad != INT_MAX) continue;
-      const double x = v[j];
-      if (mode == 2) {
-        const double q = ldexp(x, scale);
-        const double r = rint(q);
-        if (!(fabs(q) < two62) || fabs(q - r) == 0.5) {
-          bad = idx;
-          continue;
+        for (int j = 0; j < 4; j++) {
+          const int i = i0 + j;
+          const bool use = i >= pos && i < len;
+          const double q = ldexp(x[j], scale);
+          const double r = rint(q);
+          const bool conv = fabs(q) < two52 && fabs(q - r) != 0.5;
+          run = run + ((use && conv) ? r : 0.0);
+          inc[j] = run;
+          usem |= use ? (1u << j) : 0u;
+          failm |= (use && !conv) ? (1u << j) : 0u;
         }
-        run += (long long)r;
-        pref[j] = run;
-      } else if (mode == 0) {
-        if (x != 0.0) bad = idx;  // zeros keep s = +0
-      } else if (mode == 3) {
-        if (isnan(x) || (isinf(x) && ((x < 0) != (s < 0)))) bad = idx;
-      } else {
-        bad = idx;  // subnormal running sum: handled by the burst
-      }
-    }
-    long long total;
-    const long long off = block_excl_scan64(run, sh, total);
-    int fail = bad;
-    if (mode == 2) {
-      const bool neg = s < 0;
+        // exclusive prefix of the lane sums: exact up to the first lane
+        // holding a rejected addend, which is all that is used
+        const double off = S + wave_excl_scan_f64(run);
 #pragma unroll
-      for (int j = 0; j < kSumPer; j++) {
-        const int idx = begin + tid * kSumPer + j;
-        if (idx < pos || idx >= end || idx >= fail) continue;
-        const long long P = S + off + pref[j];
-        const long long a = neg ? -P : P;
-        if (a < lo52 || a > hi53) fail = idx;
+        for (int j = 0; j < 4; j++) {
+          const double a = sg * (off + inc[j]);
+          failm |= (((usem >> j) & 1u) && (a < lo52 || a > hi53)) ? (1u << j) : 0u;
+        }
+        const int jf = failm ? __builtin_ctz(failm) : 4;
+        const unsigned long long bal = __ballot(jf < 4);
+        if (bal) {
+          const int f = __ffsll((long long)bal) - 1;
+          double before = off;
+          before = jf == 1 ? off + inc[0] : before;
+          before = jf == 2 ? off + inc[1] : before;
+          before = jf == 3 ? off + inc[2] : before;
+          Sm = readlane_f64(before, f);
+          m = __builtin_amdgcn_readlane(i0 + jf, f);
+          break;
+        }
+        S = readlane_f64(off + run, 63);
       }
-    }
-    const int m = block_min_int(fail, sh);
-    if (m == INT_MAX) {
-      // no failure up to the end of the range
-      if (mode == 2 && tid == 0) sh.s = ldexp((double)(S + total), -scale);
-      __syncthreads();
-      pos = end;
+      s = ldexp(m < len ? Sm : S, -scale);
+    } else if (s == 0.0 || isinf(s)) {
+      // zero addends keep s = +0; an infinite s stays unless the addend is
+      // NaN or the opposite infinity
+      for (int c0 = pos & ~3; c0 < len; c0 += 4 * 64) {
+        const int i0 = c0 + 4 * lane;
+        int jf = 4;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int i = i0 + j;
+          const double x = buf[i];
+          const bool keep = s == 0.0 ? x == 0.0
+                                     : !(isnan(x) || (isinf(x) && ((x < 0.0) != (s < 0.0))));
+          if (jf == 4 && i >= pos && i < len && !keep) jf = j;
+        }
+        const unsigned long long bal = __ballot(jf < 4);
+        if (bal) {
+          m = __builtin_amdgcn_readlane(i0 + jf, __ffsll((long long)bal) - 1);
+          break;
+        }
+      }
     } else {
-      // the running sum just before element m is exact: S + off + (this
-      // thread's prefix before m), or s itself in modes 0, 1 and 3
-      const int owner = (m - begin) / kSumPer;
-      if (tid == owner) {
-        const int j = (m - begin) % kSumPer;
-        long long pj = 0;
-#pragma unroll
-        for (int k = 0; k < kSumPer; k++)
-          if (k + 1 == j) pj = pref[k];
-        sh.s = mode == 2 ? ldexp((double)(S + off + pj), -scale) : s;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        double sb = sh.s;
-        sh.pos = ordered_sum_burst(sh.buf, begin, m, end, sb);
-        sh.s = sb;
-      }
-      __syncthreads();
-      burst += sh.pos - m;
-      pos = sh.pos;
+      m = pos;  // subnormal running sum: one by one
     }
+    if (m >= len) break;
+    const unsigned long long b0 = stats ? wall_clock64() : 0;
+    pos = wave_burst(buf, m, len, s);
+    burst += pos - m;
+    if (stats) burst_ticks += wall_clock64() - b0;
   }
-  if (stats && tid == 0) {
+  if (stats && lane == 0) {
     atomicAdd(stats + kStatSteps, steps);
     atomicAdd(stats + kStatBurst, burst);
+    atomicAdd(stats + kStatBurstTicks, burst_ticks);
+    atomicAdd(stats + kStatWalkTicks, wall_clock64() - walk_t0);
   }
-  __syncthreads();  // buf and sh.s are reused by the caller
+  return s;
 }
 
 // One reduction: the addends, their count, the scratch and an optional flag
@@ -400,64 +474,92 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
       w.pmax[b] = mx;
     }
   }
-  // the last block to finish chains the tiles
+  // the last block to finish chains the tiles, on its first wave
   __threadfence();
   if (tid == 0) sh.last = atomicAdd(w.ticket, 1u) == (unsigned)(nt - 1);
   __syncthreads();
-  if (!sh.last) return;
+  if (!sh.last || tid >= 64) return;
   __threadfence();
-  if (tid == 0) {
-    *w.ticket = 0u;
-    sh.s = 0.0;
-  }
-  __syncthreads();
+  const unsigned long long chain_t0 = w.stats ? wall_clock64() : 0;
+  const int lane = tid;
+  if (lane == 0) *w.ticket = 0u;
+  double s = 0.0;
   int fallback = 0;
-  for (int b0 = 0; b0 < nt; b0 += kChainWin) {
+  for (int b0 = 0; b0 < nt && !isnan(s); b0 += kChainWin) {
     const int wend = min(nt, b0 + kChainWin);
-    for (int j = b0 + tid; j < wend; j += kSumBlock) {
-      sh.code[j - b0] = w.code[j];
-      sh.tot[j - b0] = w.tot[j];
-      sh.pmin[j - b0] = w.pmin[j];
-      sh.pmax[j - b0] = w.pmax[j];
-    }
-    __syncthreads();
-    int bb = b0;
-    while (bb < wend) {
-      if (tid == 0) {
-        double s = sh.s;
-        int k = bb;
-        for (; k < wend; k++) {
-          if (isnan(s)) {
-            k = wend;  // NaN absorbs everything that follows
-            break;
-          }
-          const int code = sh.code[k - b0];
-          if (code < 0 || code != sign_exp(s)) break;
-          const int e = ilogb(s);
-          const long long S = (long long)ldexp(s, 52 - e);
-          const long long lo = S + sh.pmin[k - b0], hi = S + sh.pmax[k - b0];
-          const bool inside = s > 0 ? (lo >= lo52 && hi <= hi53) : (lo >= -hi53 && hi <= -lo52);
-          if (!inside) break;
-          s = ldexp((double)(S + sh.tot[k - b0]), e - 52);
-        }
-        sh.s = s;
-        sh.tile = k;
+    wave_lds_sync();  // the previous window is no longer read
+    // 256 summaries per pass, every load issued before the first store
+    for (int j0 = b0; j0 < wend; j0 += 256) {
+      int cd[4];
+      long long tt[4], mn[4], mx[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = min(j0 + r * 64 + lane, wend - 1);
+        cd[r] = w.code[j];
+        tt[r] = w.tot[j];
+        mn[r] = w.pmin[j];
+        mx[r] = w.pmax[j];
       }
-      __syncthreads();
-      bb = sh.tile;
-      if (bb >= wend) break;
-      ordered_sum_walk(t, bb * kSumTile, min(n, (bb + 1) * kSumTile), sh, w.stats);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = j0 + r * 64 + lane;
+        if (j < wend) {
+          sh.code[j - b0] = cd[r];
+          sh.tot[j - b0] = tt[r];
+          sh.pmin[j - b0] = mn[r];
+          sh.pmax[j - b0] = mx[r];
+        }
+      }
+    }
+    wave_lds_sync();
+    int bb = b0;
+    while (bb < wend && !isnan(s)) {
+      // tiles applied from their summaries, 64 per step
+      while (bb < wend && sum_normal(s)) {
+        const int e = ilogb(s);
+        const long long S = (long long)ldexp(s, 52 - e);
+        const int k = bb + lane;
+        const bool in = k < wend;
+        const int kk = min(k, wend - 1) - b0;
+        const long long tot = in ? sh.tot[kk] : 0;
+        const long long incl = wave_scan_dpp64(tot);
+        const long long base = S + incl - tot;  // S at the start of tile k
+        const long long lo = base + sh.pmin[kk], hi = base + sh.pmax[kk];
+        const bool inside = s > 0.0 ? (lo >= lo52 && hi <= hi53) : (lo >= -hi53 && hi <= -lo52);
+        const bool ok = in && sh.code[kk] == sign_exp(s) && inside;
+        const unsigned long long bad = __ballot(!ok);
+        const int f = bad ? __ffsll((long long)bad) - 1 : 64;
+        if (f > 0) s = ldexp((double)(S + readlane64(incl, f - 1)), e - 52);
+        bb += f;
+        if (f < 64) break;
+      }
+      if (bb >= wend || isnan(s)) break;
+      // tile bb by the walker, from an LDS copy; len is 0 only for the one
+      // tile of an empty sum, which adds nothing (and must not load)
+      const int begin = bb * kSumTile, len = min(n, begin + kSumTile) - begin;
+      if (len > 0) {
+        // every load issued before the first store (past len: copies of the
+        // last addend, never added)
+#pragma unroll
+        for (int r = 0; r < kSumTile / 64; r++) {
+          const int i = r * 64 + lane;
+          sh.buf[i] = t[begin + min(i, len - 1)];
+        }
+        wave_lds_sync();
+        s = wave_walk(sh.buf, len, s, w.stats);
+        wave_lds_sync();  // the next walk rewrites buf
+      }
       fallback++;
       bb++;
     }
-    __syncthreads();  // the window arrays are refilled
   }
-  if (tid == 0) {
-    src.finish(sh.s);
+  if (lane == 0) {
+    src.finish(s);
     if (w.stats) {
       atomicAdd(w.stats + kStatSums, 1ull);
       atomicAdd(w.stats + kStatTiles, (unsigned long long)nt);
       atomicAdd(w.stats + kStatFast, (unsigned long long)(nt - fallback));
+      atomicAdd(w.stats + kStatChainTicks, wall_clock64() - chain_t0);
     }
   }
 }

@@ -313,18 +313,20 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
       s.f_col.assign((size_t)col_groups(kq) * 4 * np, 0);
       s.b_col.assign((size_t)col_groups(kq) * 4 * np, 0);
       auto cidx = [np](int k, int sl) { return ((size_t)(k / 4) * np + sl) * 4 + k % 4; };
+      // byte offset of a slot's entry in a row of the sweep's LDS ring
+      auto ring_byte = [](int slot) { return (slot & (kRingRows - 1)) * (int)sizeof(double); };
       for (int sl = 0; sl < np; sl++) {
         const int i = s.lf_rows[sl];
         const int fc = s.diaind[i] - s.rowptr[i];
         s.f_rci[4 * sl] = i;
         s.f_rci[4 * sl + 1] = fc;
         s.f_rci[4 * sl + 2] = bslot[i];
-        for (int k = 0; k < fc; k++) s.f_col[cidx(k, sl)] = s.fslot[s.colind[s.rowptr[i] + k]];
+        for (int k = 0; k < fc; k++) s.f_col[cidx(k, sl)] = ring_byte(s.fslot[s.colind[s.rowptr[i] + k]]);
         const int j = s.lb_rows[sl];
         const int bc = s.rowptr[j + 1] - s.diaind[j];
         s.b_rc[2 * sl] = j;
         s.b_rc[2 * sl + 1] = bc;
-        for (int k = 0; k < bc; k++) s.b_col[cidx(k, sl)] = bslot[s.colind[s.diaind[j] + k]];
+        for (int k = 0; k < bc; k++) s.b_col[cidx(k, sl)] = ring_byte(bslot[s.colind[s.diaind[j] + k]]);
       }
       // live interval of a row in a sweep: [level written, last level read];
       // rows whose sweep slots share a ring position must have disjoint

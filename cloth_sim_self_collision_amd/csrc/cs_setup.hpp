@@ -31,8 +31,9 @@ struct MeshSetup {
   // packed sweeps: rows in level order ("slots"), at most kl lower / ku upper
   // blocks per row.  f_rci[4*slot] = {row, block count, the row's backward
   // slot, 0}, b_rc[2*slot] = {row, block count}, fslot[row] = forward slot;
-  // per block, the slot of its column row in the same sweep (the LDS ring is
-  // addressed by slot), in groups of four: col[((k / 4) * np + slot) * 4 + k % 4]
+  // per block, the byte offset of its column row's entry in a row of the LDS
+  // ring (entry = the column row's slot in the same sweep mod kRingRows), in
+  // groups of four: col[((k / 4) * np + slot) * 4 + k % 4]
   int kl = 0, ku = 0;
   std::vector<int> f_rci, b_rc, fslot, f_col, b_col;
   // the LDS ring of kRingRows entries (entry = slot mod kRingRows) can hold the

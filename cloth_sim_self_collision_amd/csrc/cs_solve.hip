@@ -431,57 +431,94 @@ struct QuadRow {
   int cnt;   // blocks of the row
   int bs;    // forward: the row's backward slot
   double t;  // this lane's component of the right-hand side
-  int col[kMaxBlocksPerRow];  // slots of the column rows (ring entries)
+  int col[kMaxBlocksPerRow];  // byte offsets of the column rows' ring entries
   double a[kMaxBlocksPerRow * 3];
   double d[3];
 };
+
+// Bases of one sweep's operands, formed once per launch: every array is
+// addressed as base[unsigned 32-bit index], which the loads take as a
+// scalar base plus a per-lane offset (no 64-bit address arithmetic per load).
+template <bool kFwd, int K>
+struct QuadSrc {
+  const int4* rci;   // forward: {row, count, backward slot, 0} per slot
+  const int2* rc;    // backward: {row, count} per slot
+  const double* rhs;
+  const int4* col[col_groups(K)];
+  const double2* blk[pack_pairs(K)];
+  const double2* d0;  // D^-1 values 0-1 and 2-3 (forward)
+  const double2* d1;
+};
+
+template <bool kFwd, int K>
+__device__ __forceinline__ QuadSrc<kFwd, K> quad_src(const DevCloth& c, const double* rhs) {
+  QuadSrc<kFwd, K> q;
+  const size_t np = (size_t)c.np;
+  q.rci = reinterpret_cast<const int4*>(c.f_rci);
+  q.rc = reinterpret_cast<const int2*>(c.b_rc);
+  q.rhs = rhs;
+  const int4* col = reinterpret_cast<const int4*>(kFwd ? c.f_col : c.b_col);
+#pragma unroll
+  for (int g = 0; g < col_groups(K); g++) q.col[g] = col + g * np;
+  const double2* blk = reinterpret_cast<const double2*>(kFwd ? c.f_blk : c.b_blk);
+#pragma unroll
+  for (int p = 0; p < pack_pairs(K); p++) q.blk[p] = blk + p * np * 3;
+  q.d0 = reinterpret_cast<const double2*>(c.f_dinv);
+  q.d1 = q.d0 + np * 3;
+  return q;
+}
+
+// base[i] for an index whose byte offset fits 32 bits: the load then takes the
+// scalar base plus a 32-bit per-lane offset
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     i * (unsigned)sizeof(T));
+}
 
 // Branch-free on purpose: a quad without a row at this sub-level loads slot 0
 // and is only marked (i = -1), lane 3 loads lane 2's right-hand side.  Loads
 // on one path only would make the compiler wait for every outstanding load
 // where the paths merge, defeating the prefetch.
 template <bool kFwd, int K>
-__device__ __forceinline__ void quad_load(const DevCloth& c, const int* qptr, int nq, int l,
-                                          const double* __restrict__ rhs, QuadRow& q) {
+__device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int* qptr, int nq,
+                                          int l, QuadRow& q) {
   const int lane = threadIdx.x & 3, qi = threadIdx.x >> 2;
   const int lc = l < nq ? l : 0;
   const int s0 = qptr[lc] + qi;
   const bool valid = l < nq && s0 < qptr[lc + 1];
-  const int sl = valid ? s0 : 0;
-  const int np = c.np;
-  q.s = sl;
+  const unsigned sl = valid ? (unsigned)s0 : 0u;
+  q.s = (int)sl;
   if (kFwd) {
-    const int4 v = reinterpret_cast<const int4*>(c.f_rci)[sl];
+    const int4 v = ld32(src.rci, sl);
     q.i = valid ? v.x : -1;
     q.cnt = v.y;
     q.bs = v.z;
   } else {
-    const int2 v = reinterpret_cast<const int2*>(c.b_rc)[sl];
+    const int2 v = ld32(src.rc, sl);
     q.i = valid ? v.x : -1;
     q.cnt = v.y;
   }
-  const int lr = lane < 3 ? lane : 2;  // lane 3 shadows lane 2 (same cache line)
-  q.t = rhs[3 * sl + lr];
-  const int4* col = reinterpret_cast<const int4*>(kFwd ? c.f_col : c.b_col);
+  const unsigned lr = lane < 3 ? lane : 2;  // lane 3 shadows lane 2 (same cache line)
+  const unsigned e = 3 * sl + lr;
+  q.t = ld32(src.rhs, e);
 #pragma unroll
   for (int g = 0; g < col_groups(K); g++) {
-    const int4 v = col[(size_t)g * np + sl];
+    const int4 v = ld32(src.col[g], sl);
     q.col[4 * g] = v.x;
     q.col[4 * g + 1] = v.y;
     q.col[4 * g + 2] = v.z;
     q.col[4 * g + 3] = v.w;
   }
-  const double2* blk = reinterpret_cast<const double2*>(kFwd ? c.f_blk : c.b_blk);
 #pragma unroll
   for (int p = 0; p < pack_pairs(K); p++) {
-    const double2 w = blk[((size_t)p * np + sl) * 3 + lr];
+    const double2 w = ld32(src.blk[p], e);
     q.a[2 * p] = w.x;
     if (2 * p + 1 < 3 * K) q.a[2 * p + 1] = w.y;
   }
   if (kFwd) {
-    const double2* dd = reinterpret_cast<const double2*>(c.f_dinv);
-    const double2 w0 = dd[(size_t)sl * 3 + lr];
-    const double2 w1 = dd[((size_t)np + sl) * 3 + lr];
+    const double2 w0 = ld32(src.d0, e);
+    const double2 w1 = ld32(src.d1, e);
     q.d[0] = w0.x;
     q.d[1] = w0.y;
     q.d[2] = w1.x;
@@ -500,12 +537,14 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], doub
   const int lane = threadIdx.x & 3;
   const bool has = q.i >= 0;
   double y[K][3];
+  const char* rb = reinterpret_cast<const char*>(&ring[0][0]);
+  constexpr int kRow = (kRingRows + 1) * (int)sizeof(double);  // bytes per ring row
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    const int j = q.col[k] & kMask;
-    y[k][0] = ring[0][j];
-    y[k][1] = ring[1][j];
-    y[k][2] = ring[2][j];
+    const char* at = rb + q.col[k];
+    y[k][0] = *reinterpret_cast<const double*>(at);
+    y[k][1] = *reinterpret_cast<const double*>(at + kRow);
+    y[k][2] = *reinterpret_cast<const double*>(at + 2 * kRow);
   }
   double t = q.t;
 #pragma unroll
@@ -529,25 +568,31 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], doub
   }
 }
 
-// one sweep over the sub-levels; two register sets alternate so that the
-// loads of sub-level l + 2 are in flight while l + 1 is computed
+// one sweep over the sub-levels; three register sets rotate so that the
+// loads of sub-levels l + 1 and l + 2 are in flight while l is computed
 template <bool kFwd, int K>
 __device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, int nq,
                                            const double* rhs, double* out,
                                            double (*ring)[kRingRows + 1]) {
-  QuadRow qa, qb;
-  quad_load<kFwd, K>(c, qptr, nq, 0, rhs, qa);
-  quad_load<kFwd, K>(c, qptr, nq, 1, rhs, qb);
-  // both halves unconditional (an odd count ends with an empty sub-level):
-  // a skipped half would make the loads of qa the newest on one path, and the
-  // compiler would then wait for all loads at the top of every iteration
-  for (int l = 0; l < nq; l += 2) {
+  const QuadSrc<kFwd, K> src = quad_src<kFwd, K>(c, rhs);
+  QuadRow qa, qb, qc;
+  quad_load<kFwd, K>(src, qptr, nq, 0, qa);
+  quad_load<kFwd, K>(src, qptr, nq, 1, qb);
+  quad_load<kFwd, K>(src, qptr, nq, 2, qc);
+  // all three thirds unconditional (a count that is not a multiple of three
+  // ends with empty sub-levels): a skipped third would change which loads are
+  // the newest on one path, and the compiler would then wait for all loads at
+  // the top of every iteration
+  for (int l = 0; l < nq; l += 3) {
     quad_compute<kFwd, K>(ring, out, qa);
     lds_barrier();
-    quad_load<kFwd, K>(c, qptr, nq, l + 2, rhs, qa);
+    quad_load<kFwd, K>(src, qptr, nq, l + 3, qa);
     quad_compute<kFwd, K>(ring, out, qb);
     lds_barrier();
-    quad_load<kFwd, K>(c, qptr, nq, l + 3, rhs, qb);
+    quad_load<kFwd, K>(src, qptr, nq, l + 4, qb);
+    quad_compute<kFwd, K>(ring, out, qc);
+    lds_barrier();
+    quad_load<kFwd, K>(src, qptr, nq, l + 5, qc);
   }
 }
 

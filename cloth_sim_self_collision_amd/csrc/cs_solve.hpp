@@ -8,8 +8,11 @@ namespace cs {
 constexpr int kQuadLevels = 4096;  // sub-level pointers the quad sweep keeps in LDS
 // the packed (quad) sweep applies: packed factor, LDS ring proven safe,
 // sub-level pointers fit
+// the quad sweep addresses its operands with 32-bit byte offsets (at most
+// 3 * np 16-byte pairs per array)
 inline bool quad_ok(const DevCloth& c) {
-  return c.kpack > 0 && c.ring_ok && c.nqf <= kQuadLevels && c.nqb <= kQuadLevels;
+  return c.kpack > 0 && c.ring_ok && c.nqf <= kQuadLevels && c.nqb <= kQuadLevels &&
+         48LL * c.np < (1LL << 32);
 }
 
 int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s);

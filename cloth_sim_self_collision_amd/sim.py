@@ -177,9 +177,10 @@ class DeviceCloth:
         check(_lib.load().cs_profile_enable(self.h, 1 if on else 0))
 
     def reduction_stats(self) -> dict:
-        out = (C.c_longlong * 5)()
+        out = (C.c_longlong * 8)()
         check(_lib.load().cs_reduction_stats(self.h, out))
-        return dict(zip(("sums", "tiles", "fast_tiles", "walker_steps", "burst_elements"), out))
+        return dict(zip(("sums", "tiles", "fast_tiles", "walker_steps", "burst_elements",
+                         "chain_ns", "walker_ns", "burst_ns"), out))
 
     def profile_read(self) -> dict:
         out = {}

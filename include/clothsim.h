@@ -170,9 +170,11 @@ int cs_set_option(cs_cloth* c, int option, int value);
  * 5 assembly, 6 contact pass, 7 other.  Enabling resets the totals. */
 int cs_profile_enable(cs_cloth* c, int on);
 int cs_profile_read(cs_cloth* c, int kind, double* total_ms, long long* launches);
-/* counters of the exact PCG reductions while profiling is on: out[5] =
+/* counters of the exact PCG reductions while profiling is on: out[8] =
  * reductions, tiles, tiles applied from their summary, binade-walker steps,
- * elements added one by one */
+ * elements added one by one, then nanoseconds (device wall clock, summed
+ * over reductions) spent in the serial chain, in the binade walker (bursts
+ * included) and in one-by-one bursts */
 int cs_reduction_stats(cs_cloth* c, long long* out);
 
 /* ---- lower-level entry points (host arrays unless noted) ---- */

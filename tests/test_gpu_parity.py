@@ -86,6 +86,14 @@ def test_ordered_sum_is_the_sequential_sum(sim_mod):
         # mixed signs decaying over 600 binades: dense binade changes early,
         # then addends far below the running sum's ulp
         cases.append(rng.standard_normal(n) * np.exp2(np.linspace(300, -300, n)))
+    # more than 1024 tiles (two windows of tile summaries)
+    cases.append(rng.standard_normal(1_100_000))
+    # a running sum crossing binades at every addend (alternating signs)
+    n = 20000
+    cases.append((1.0 + 1e-3 * rng.standard_normal(n)) * np.where(np.arange(n) % 2, -1.0, 1.0))
+    # products of correlated vectors, as in the PCG reductions
+    a = rng.standard_normal(70000) * np.exp2(rng.integers(-20, 20, 70000))
+    cases.append(a * (a + 0.5 * rng.standard_normal(70000)))
     cases.append(rng.standard_normal(3000) * np.exp2(rng.integers(-1074, -1000, 3000)))
     t = rng.standard_normal(5000)
     t[100] = np.inf
