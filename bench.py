@@ -162,9 +162,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fold256")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-profile", action="store_true", help="skip the profiled steps")
-    ap.add_argument("--profile-steps", type=int, default=1,
-                    help="steps after the timed ones with per-kernel events (kernel_ms, roofline)")
+    ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--maxit", type=int, default=None,
                     help="PCG iteration cap override (short profiling runs only)")
     ap.add_argument("--sweep-variant", type=int, default=0,
@@ -189,9 +187,12 @@ def main() -> int:
         print("no HIP device", file=sys.stderr)
         return 1
 
-    from cloth_sim_self_collision_amd.sim import bandwidth_probe
+    from cloth_sim_self_collision_amd.sim import bandwidth_probe, read_probe
     n_copy = int(args.copy_mb * 2**20 / 8) // 2 * 2
     copy_gbs = bandwidth_probe(n_copy, 10) if n_copy > 0 else None
+    # the one-workgroup sweep streams its factor through one CU: that CU's
+    # read ceiling, over a buffer of the factor's size
+    one_cu_gbs = read_probe(1 << 23, 5, 1) if n_copy > 0 else None
     cloth = workload(args.workload)
     if args.maxit is not None:
         cloth.pcg_maxit = args.maxit
@@ -206,7 +207,10 @@ def main() -> int:
               flush=True)
         if k == 0:
             first_x, _ = d.get_state()
-    # timed steps without per-kernel events (their harvest synchronises)
+    # per-kernel-class HIP events on the cloth's streams over the timed steps
+    # (their cost is below the step-to-step spread)
+    if not args.no_profile:
+        d.profile(True)
     lat = []
     stats = []
     t_start = time.perf_counter()
@@ -217,19 +221,9 @@ def main() -> int:
         print(f"[bench] step {len(lat) - 1}: {lat[-1]:.3f} s, pcg {stats[-1]['pcg_iters']}",
               file=sys.stderr, flush=True)
     t_total = time.perf_counter() - t_start
-    # per-kernel-class times and reduction counters from further steps,
-    # outside the timed region
-    prof, red = {}, None
-    if not args.no_profile and args.profile_steps > 0:
-        d.profile(True)
-        for k in range(args.profile_steps):
-            t0 = time.perf_counter()
-            d.step(1)
-            print(f"[bench] profiled step {k}: {time.perf_counter() - t0:.3f} s", file=sys.stderr,
-                  flush=True)
-        prof = d.profile_read()
-        red = d.reduction_stats()
-        d.profile(False)
+    prof = d.profile_read() if not args.no_profile else {}
+    red = d.reduction_stats() if not args.no_profile else None
+    d.profile(False)
 
     value = cloth.np_ * args.steps / t_total
     lat_ms = np.array(lat) * 1e3
@@ -247,7 +241,9 @@ def main() -> int:
                 "bytes_per_launch": b, "avg_launch_us": avg_s * 1e6, "launches": n,
                 "share_of_gpu_time": ms / sum(v[0] for v in kinds.values()),
                 "stream_copy_gbs": copy_gbs,
-                "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None}
+                "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None,
+                "one_cu_read_gbs": one_cu_gbs,
+                "frac_of_one_cu_read": (ach / one_cu_gbs) if (ach and one_cu_gbs) else None}
     d.close()
     batch = batch_measure(args.batch_count, args.batch_steps) if args.batch_count > 0 else None
     served = serve_measure(args.serve_count, args.serve_rate) if args.serve_count > 0 else None
@@ -268,7 +264,6 @@ def main() -> int:
         "contacts": {"n_prox": stats[-1]["n_prox"], "n_ccd": stats[-1]["n_ccd"],
                      "n_riz_pass": stats[-1]["n_riz_pass"]},
         "kernel_ms": {k: round(v[0], 3) for k, v in prof.items()} if prof else None,
-        "kernel_ms_steps": args.profile_steps if prof else 0,
         "reduction_stats": red,
         "roofline": roof,
         "cpu_baseline": cpu,

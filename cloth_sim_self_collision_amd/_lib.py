@@ -73,6 +73,7 @@ SIGNATURES = {
     "cs_batch_size": (C.c_int, [_V, C.POINTER(C.c_int)]),
     "cs_batch_step": (C.c_int, [_V, C.c_int, _V]),
     "cs_bandwidth_probe": (C.c_int, [C.c_longlong, C.c_int, _D]),
+    "cs_read_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, _D]),
     "cs_step_internal": (C.c_int, [_V, _D, _I]),
     "cs_contact_resolve": (C.c_int, [_V, _I, C.POINTER(StepStats)]),
     "cs_post_contact_update": (C.c_int, [_V]),

@@ -179,7 +179,56 @@ __global__ void k_stream_copy(long long n2, const double2* __restrict__ x, doubl
     y[i] = x[i];
 }
 
+// Streaming reads by a few workgroups of 512 threads (one per CU): the
+// memory ceiling of a kernel that runs in one workgroup, as the ILU sweeps
+// do.  Sixteen 16-byte loads per thread are in flight per pass; n2 is a
+// multiple of 512 * 16 * blocks, each block reads its own contiguous part.
+constexpr int kProbeThreads = 512, kProbeUnroll = 16;
+__global__ __launch_bounds__(kProbeThreads) void k_read_probe(long long per_block,
+                                                              const double2* __restrict__ x,
+                                                              double* out) {
+  const double2* a = x + (long long)blockIdx.x * per_block;
+  double acc = 0.0;
+  for (long long i = threadIdx.x; i < per_block; i += kProbeThreads * kProbeUnroll) {
+    double2 v[kProbeUnroll];
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; u++) v[u] = a[i + u * kProbeThreads];
+#pragma unroll
+    for (int u = 0; u < kProbeUnroll; u++) acc += v[u].x + v[u].y;
+  }
+  if (acc == -1.0) out[blockIdx.x] = acc;  // keeps the loads; never taken for zeros
+}
+
 }  // namespace cs
+
+extern "C" int cs_read_probe(long long n, int reps, int blocks, double* gbs) {
+  using namespace cs;
+  const long long unit = 2LL * kProbeThreads * kProbeUnroll * blocks;  // doubles per pass
+  if (n <= 0 || n % unit || reps <= 0 || blocks <= 0 || blocks > 4096 || !gbs) return kBadArg;
+  Scratch s;
+  double *x, *out;
+  if (s.get(&x, (size_t)n) || s.get(&out, (size_t)blocks)) return kHipError;
+  CS_HIP(hipMemset(x, 0, sizeof(double) * n));
+  const long long per_block = n / 2 / blocks;  // double2 per block
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_read_probe, dim3(blocks), dim3(kProbeThreads), 0, 0, per_block,
+                     (const double2*)x, out);
+  CS_HIP(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_read_probe, dim3(blocks), dim3(kProbeThreads), 0, 0, per_block,
+                       (const double2*)x, out);
+  CS_HIP(hipEventRecord(e1, 0));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CS_HIP(hipGetLastError());
+  *gbs = 8.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+  return kOk;
+}
 
 extern "C" int cs_bandwidth_probe(long long n, int reps, double* gbs) {
   using namespace cs;

@@ -426,6 +426,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 struct QuadRow {
+  bool live; // some quad of this wave has a row (wave-uniform)
   int i;     // row, -1: none for this quad in this sub-level
   int s;     // slot in this sweep (the row's ring entry is s mod kRingRows)
   int cnt;   // blocks of the row
@@ -478,15 +479,19 @@ __device__ __forceinline__ T ld32(const T* base, unsigned i) {
 
 // Branch-free on purpose: a quad without a row at this sub-level loads slot 0
 // and is only marked (i = -1), lane 3 loads lane 2's right-hand side.  Loads
-// on one path only would make the compiler wait for every outstanding load
-// where the paths merge, defeating the prefetch.
+// on one path only (even a wave-uniform skip) make the compiler wait for every
+// outstanding load where the paths merge, defeating the prefetch.
 template <bool kFwd, int K>
 __device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int* qptr, int nq,
                                           int l, QuadRow& q) {
   const int lane = threadIdx.x & 3, qi = threadIdx.x >> 2;
   const int lc = l < nq ? l : 0;
-  const int s0 = qptr[lc] + qi;
-  const bool valid = l < nq && s0 < qptr[lc + 1];
+  const int first = qptr[lc], end = qptr[lc + 1];
+  const int s0 = first + qi;
+  const bool valid = l < nq && s0 < end;
+  // the quads of a wave are consecutive: the wave has a row iff its first does
+  q.live = __builtin_amdgcn_readfirstlane(
+               (int)(l < nq && first + (int)(threadIdx.x >> 6) * 16 < end)) != 0;
   const unsigned sl = valid ? (unsigned)s0 : 0u;
   q.s = (int)sl;
   if (kFwd) {
@@ -534,6 +539,7 @@ template <bool kFwd, int K>
 __device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], double* out,
                                              const QuadRow& q) {
   constexpr int kMask = kRingRows - 1;
+  if (!q.live) return;  // no row in this wave: nothing to compute (its loads are not skipped)
   const int lane = threadIdx.x & 3;
   const bool has = q.i >= 0;
   double y[K][3];

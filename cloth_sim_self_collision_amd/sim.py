@@ -358,6 +358,16 @@ def bandwidth_probe(n_doubles: int = 1 << 26, reps: int = 20) -> float:
     return g.value
 
 
+def read_probe(n_doubles: int = 1 << 23, reps: int = 5, blocks: int = 1) -> float:
+    """Measured read bandwidth in GB/s of `blocks` workgroups of 512 threads
+    (1: one CU, the ceiling of the one-workgroup ILU sweep)."""
+    unit = 16384 * blocks
+    n = max(unit, int(n_doubles) // unit * unit)
+    g = C.c_double()
+    check(_lib.load().cs_read_probe(n, int(reps), int(blocks), C.byref(g)))
+    return g.value
+
+
 def selftest_divsqrt(a, b):
     a = f64(a)
     b = f64(b)

@@ -239,6 +239,10 @@ int cs_selftest_ordered_sum(int n, const double* t, double* out);
  * (n even) between two fresh buffers; the last reps are timed with events.
  * *gbs = 16 n reps / seconds / 1e9 (read + write) */
 int cs_bandwidth_probe(long long n, int reps, double* gbs);
+/* read-only streaming of n doubles by `blocks` workgroups of 512 threads
+ * (blocks = 1: one CU, the ceiling of the one-workgroup ILU sweep), reps
+ * times; n a multiple of 16384 * blocks; GB/s of reads */
+int cs_read_probe(long long n, int reps, int blocks, double* gbs);
 
 #ifdef __cplusplus
 }
