@@ -589,16 +589,18 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, i
   // ends with empty sub-levels): a skipped third would change which loads are
   // the newest on one path, and the compiler would then wait for all loads at
   // the top of every iteration
+  // a set is reloaded right after its compute, before the barrier: waves
+  // that finish computing early issue their loads while others still compute
   for (int l = 0; l < nq; l += 3) {
     quad_compute<kFwd, K>(ring, out, qa);
-    lds_barrier();
     quad_load<kFwd, K>(src, qptr, nq, l + 3, qa);
+    lds_barrier();
     quad_compute<kFwd, K>(ring, out, qb);
-    lds_barrier();
     quad_load<kFwd, K>(src, qptr, nq, l + 4, qb);
-    quad_compute<kFwd, K>(ring, out, qc);
     lds_barrier();
+    quad_compute<kFwd, K>(ring, out, qc);
     quad_load<kFwd, K>(src, qptr, nq, l + 5, qc);
+    lds_barrier();
   }
 }
 
