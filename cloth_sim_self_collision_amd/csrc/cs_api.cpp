@@ -205,15 +205,16 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&d.flags, kNumFlag);
   A.reserve(&d.bb, 6 * (size_t)nn);
   A.reserve(&c->ising, np);
-  int *frc, *fcol, *brc, *bcol, *fsl;
+  int *frc, *brc, *fsl;
+  unsigned short *fmeta, *bmeta;
   const int kl = S.kl > 0 ? S.kl : 0, ku = S.ku > 0 ? S.ku : 0;
   A.reserve(&frc, kl ? 4 * (size_t)np : 0);
   A.reserve(&fsl, kl ? (size_t)np : 0);
   A.reserve(&d.sf, 3 * (size_t)np);
   A.reserve(&d.sb, 3 * (size_t)np);
-  A.reserve(&fcol, kl ? (size_t)col_groups(std::max(kl, ku)) * 4 * np : 0);
+  A.reserve(&fmeta, kl ? (size_t)meta_groups(std::max(kl, ku)) * 8 * np : 0);
   A.reserve(&brc, kl ? 2 * (size_t)np : 0);
-  A.reserve(&bcol, kl ? (size_t)col_groups(std::max(kl, ku)) * 4 * np : 0);
+  A.reserve(&bmeta, kl ? (size_t)meta_groups(std::max(kl, ku)) * 8 * np : 0);
   const int kq = kl > 0 ? std::max(kl, ku) : 0;
   // value pairs per lane: (3 kq + 1) / 2 pairs of 3 lanes x 2 doubles per slot
   const size_t pk = kq ? (size_t)(3 * kq + 1) / 2 * 6 * np : 0;
@@ -258,9 +259,9 @@ int build(Cloth* c, const cs_mesh* m) {
   d.ring_ok = S.ring_ok ? 1 : 0;
   d.f_rci = frc;
   d.fslot = kl ? fsl : nullptr;  // null: no packed sweep, no slot-order copy of r
-  d.f_col = fcol;
+  d.f_meta = fmeta;
   d.b_rc = brc;
-  d.b_col = bcol;
+  d.b_meta = bmeta;
   d.ising = c->ising;
   d.qf_ptr = qfp;
   d.qb_ptr = qbp;
@@ -303,9 +304,9 @@ int build(Cloth* c, const cs_mesh* m) {
     if (int st = upload(qbp, S.qb_ptr, s)) return st;
     if (int st = upload(frc, S.f_rci, s)) return st;
     if (int st = upload(fsl, S.fslot, s)) return st;
-    if (int st = upload(fcol, S.f_col, s)) return st;
+    if (int st = upload(fmeta, S.f_meta, s)) return st;
     if (int st = upload(brc, S.b_rc, s)) return st;
-    if (int st = upload(bcol, S.b_col, s)) return st;
+    if (int st = upload(bmeta, S.b_meta, s)) return st;
   }
   CS_HIP(hipMemcpyAsync(d.x, m->xyz, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s));
   if (m->uvw)

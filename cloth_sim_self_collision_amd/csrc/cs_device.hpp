@@ -133,9 +133,13 @@ struct DevCloth {
   int wmax;           // widest level of either sweep
   int sweep_variant;  // 0 automatic (quad sweep), 1 row-by-row
   const int* f_rci;  // {row, block count, backward slot, 0} per forward slot
-  const int* f_col;  // ring byte offset of the column's forward slot, [((k / 4) * np + slot) * 4 + k % 4]
   const int* b_rc;   // {row, block count} per backward slot
-  const int* b_col;  // the same for the backward slots
+  // sweep metadata per slot: kpack 16-bit ring byte offsets of the column
+  // rows (padded blocks: the ring's zero entry), then the forward slot's
+  // backward slot / the backward slot's row as two 16-bit halves; word h at
+  // [((h / 8) * np + slot) * 8 + h % 8]
+  const unsigned short* f_meta;
+  const unsigned short* b_meta;
   const int* fslot;  // forward slot of each row
   double* sf;        // sweep right-hand side in forward-slot order [3 * slot + r]
   double* sb;        // forward result in backward-slot order

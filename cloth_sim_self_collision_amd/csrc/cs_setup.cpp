@@ -310,23 +310,35 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
         bslot[s.lb_rows[sl]] = sl;
       }
       const int kq = std::max(std::max(s.kl, s.ku), 1);  // both sweeps padded alike
-      s.f_col.assign((size_t)col_groups(kq) * 4 * np, 0);
-      s.b_col.assign((size_t)col_groups(kq) * 4 * np, 0);
-      auto cidx = [np](int k, int sl) { return ((size_t)(k / 4) * np + sl) * 4 + k % 4; };
+      const size_t mw = (size_t)meta_groups(kq) * 8;    // metadata words per slot
+      s.f_meta.assign(mw * np, 0);
+      s.b_meta.assign(mw * np, 0);
+      auto midx = [np](int h, int sl) { return ((size_t)(h / 8) * np + sl) * 8 + h % 8; };
       // byte offset of a slot's entry in a row of the sweep's LDS ring
-      auto ring_byte = [](int slot) { return (slot & (kRingRows - 1)) * (int)sizeof(double); };
+      auto ring_byte = [](int slot) {
+        return (unsigned short)((slot & (kRingRows - 1)) * (int)sizeof(double));
+      };
+      const unsigned short zero_byte = (unsigned short)(kRingZero * (int)sizeof(double));
+      auto put32 = [&](std::vector<unsigned short>& m, int sl, unsigned v) {
+        m[midx(kq, sl)] = (unsigned short)(v & 0xffffu);
+        m[midx(kq + 1, sl)] = (unsigned short)(v >> 16);
+      };
       for (int sl = 0; sl < np; sl++) {
         const int i = s.lf_rows[sl];
         const int fc = s.diaind[i] - s.rowptr[i];
         s.f_rci[4 * sl] = i;
         s.f_rci[4 * sl + 1] = fc;
         s.f_rci[4 * sl + 2] = bslot[i];
-        for (int k = 0; k < fc; k++) s.f_col[cidx(k, sl)] = ring_byte(s.fslot[s.colind[s.rowptr[i] + k]]);
+        for (int k = 0; k < kq; k++)
+          s.f_meta[midx(k, sl)] = k < fc ? ring_byte(s.fslot[s.colind[s.rowptr[i] + k]]) : zero_byte;
+        put32(s.f_meta, sl, (unsigned)bslot[i]);
         const int j = s.lb_rows[sl];
         const int bc = s.rowptr[j + 1] - s.diaind[j];
         s.b_rc[2 * sl] = j;
         s.b_rc[2 * sl + 1] = bc;
-        for (int k = 0; k < bc; k++) s.b_col[cidx(k, sl)] = ring_byte(bslot[s.colind[s.diaind[j] + k]]);
+        for (int k = 0; k < kq; k++)
+          s.b_meta[midx(k, sl)] = k < bc ? ring_byte(bslot[s.colind[s.diaind[j] + k]]) : zero_byte;
+        put32(s.b_meta, sl, (unsigned)j);
       }
       // live interval of a row in a sweep: [level written, last level read];
       // rows whose sweep slots share a ring position must have disjoint

@@ -30,12 +30,17 @@ struct MeshSetup {
   int nqf = 0, nqb = 0;
   // packed sweeps: rows in level order ("slots"), at most kl lower / ku upper
   // blocks per row.  f_rci[4*slot] = {row, block count, the row's backward
-  // slot, 0}, b_rc[2*slot] = {row, block count}, fslot[row] = forward slot;
-  // per block, the byte offset of its column row's entry in a row of the LDS
-  // ring (entry = the column row's slot in the same sweep mod kRingRows), in
-  // groups of four: col[((k / 4) * np + slot) * 4 + k % 4]
+  // slot, 0}, b_rc[2*slot] = {row, block count}, fslot[row] = forward slot.
+  // Sweep metadata per slot (f_meta, b_meta), 16-bit words in 16-byte groups,
+  // word h at [((h / 8) * np + slot) * 8 + h % 8]: for each of the
+  // kq = max(kl, ku) blocks the byte offset of its column row's entry in a
+  // row of the LDS ring (entry = that row's slot in the same sweep mod
+  // kRingRows; blocks past the row's count: the ring's zero entry), then a
+  // 32-bit value as low and high word (forward: the row's backward slot;
+  // backward: the row)
   int kl = 0, ku = 0;
-  std::vector<int> f_rci, b_rc, fslot, f_col, b_col;
+  std::vector<int> f_rci, b_rc, fslot;
+  std::vector<unsigned short> f_meta, b_meta;
   // the LDS ring of kRingRows entries (entry = slot mod kRingRows) can hold the
   // sweep dependencies: no two rows sharing an entry are live at the same level
   bool ring_ok = false;
@@ -61,10 +66,14 @@ struct MeshSetup {
 };
 
 constexpr int kRingRows = 4096;
+// a ring row holds kRingRows entries, then a write-only dummy entry (lanes
+// without a row) and an entry that always holds +0.0 (read by padded blocks)
+constexpr int kRingDummy = kRingRows, kRingZero = kRingRows + 1, kRingLen = kRingRows + 2;
 constexpr int kQuadRows = 128;  // rows per sub-level of the quad sweep (4 lanes each)
 constexpr int kMaxBlocksPerRow = 8;
 // packed layout: column indices in groups of four per slot
-inline constexpr int col_groups(int k) { return (k + 3) / 4; }
+// 16-byte groups of sweep metadata per slot for k blocks per row
+inline constexpr int meta_groups(int k) { return (k + 2 + 7) / 8; }
 
 // Returns an empty string on success, else a description of the problem.
 std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,

@@ -427,10 +427,9 @@ __device__ __forceinline__ void lds_barrier() {
 
 struct QuadRow {
   bool live; // some quad of this wave has a row (wave-uniform)
-  int i;     // row, -1: none for this quad in this sub-level
+  bool has;  // this quad has a row at this sub-level
   int s;     // slot in this sweep (the row's ring entry is s mod kRingRows)
-  int cnt;   // blocks of the row
-  int bs;    // forward: the row's backward slot
+  int out;   // output row: forward, the row's backward slot; backward, the row
   double t;  // this lane's component of the right-hand side
   int col[kMaxBlocksPerRow];  // byte offsets of the column rows' ring entries
   double a[kMaxBlocksPerRow * 3];
@@ -442,10 +441,8 @@ struct QuadRow {
 // scalar base plus a per-lane offset (no 64-bit address arithmetic per load).
 template <bool kFwd, int K>
 struct QuadSrc {
-  const int4* rci;   // forward: {row, count, backward slot, 0} per slot
-  const int2* rc;    // backward: {row, count} per slot
+  const int4* meta[meta_groups(K)];  // sweep metadata, 16-byte groups per slot
   const double* rhs;
-  const int4* col[col_groups(K)];
   const double2* blk[pack_pairs(K)];
   const double2* d0;  // D^-1 values 0-1 and 2-3 (forward)
   const double2* d1;
@@ -455,12 +452,10 @@ template <bool kFwd, int K>
 __device__ __forceinline__ QuadSrc<kFwd, K> quad_src(const DevCloth& c, const double* rhs) {
   QuadSrc<kFwd, K> q;
   const size_t np = (size_t)c.np;
-  q.rci = reinterpret_cast<const int4*>(c.f_rci);
-  q.rc = reinterpret_cast<const int2*>(c.b_rc);
-  q.rhs = rhs;
-  const int4* col = reinterpret_cast<const int4*>(kFwd ? c.f_col : c.b_col);
+  const int4* meta = reinterpret_cast<const int4*>(kFwd ? c.f_meta : c.b_meta);
 #pragma unroll
-  for (int g = 0; g < col_groups(K); g++) q.col[g] = col + g * np;
+  for (int g = 0; g < meta_groups(K); g++) q.meta[g] = meta + g * np;
+  q.rhs = rhs;
   const double2* blk = reinterpret_cast<const double2*>(kFwd ? c.f_blk : c.b_blk);
 #pragma unroll
   for (int p = 0; p < pack_pairs(K); p++) q.blk[p] = blk + p * np * 3;
@@ -494,27 +489,23 @@ __device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int
                (int)(l < nq && first + (int)(threadIdx.x >> 6) * 16 < end)) != 0;
   const unsigned sl = valid ? (unsigned)s0 : 0u;
   q.s = (int)sl;
-  if (kFwd) {
-    const int4 v = ld32(src.rci, sl);
-    q.i = valid ? v.x : -1;
-    q.cnt = v.y;
-    q.bs = v.z;
-  } else {
-    const int2 v = ld32(src.rc, sl);
-    q.i = valid ? v.x : -1;
-    q.cnt = v.y;
+  q.has = valid;
+  unsigned mw[4 * meta_groups(K)];  // metadata as 32-bit words (two 16-bit entries each)
+#pragma unroll
+  for (int g = 0; g < meta_groups(K); g++) {
+    const int4 v = ld32(src.meta[g], sl);
+    mw[4 * g] = (unsigned)v.x;
+    mw[4 * g + 1] = (unsigned)v.y;
+    mw[4 * g + 2] = (unsigned)v.z;
+    mw[4 * g + 3] = (unsigned)v.w;
   }
+  auto word16 = [&](int h) { return (mw[h / 2] >> (16 * (h % 2))) & 0xffffu; };
+#pragma unroll
+  for (int k = 0; k < K; k++) q.col[k] = (int)word16(k);
+  q.out = (int)(word16(K) | (word16(K + 1) << 16));
   const unsigned lr = lane < 3 ? lane : 2;  // lane 3 shadows lane 2 (same cache line)
   const unsigned e = 3 * sl + lr;
   q.t = ld32(src.rhs, e);
-#pragma unroll
-  for (int g = 0; g < col_groups(K); g++) {
-    const int4 v = ld32(src.col[g], sl);
-    q.col[4 * g] = v.x;
-    q.col[4 * g + 1] = v.y;
-    q.col[4 * g + 2] = v.z;
-    q.col[4 * g + 3] = v.w;
-  }
 #pragma unroll
   for (int p = 0; p < pack_pairs(K); p++) {
     const double2 w = ld32(src.blk[p], e);
@@ -530,21 +521,23 @@ __device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int
   }
 }
 
-// one row component; every lane of the block runs it (the DPP exchange needs
-// whole quads).  Branch-free up to the global store: blocks past the row's
-// count are selected away, and lanes without a row (and lane 3) write their
-// value to a dummy ring slot.  Everything loaded is thus consumed on every
-// path, which keeps the compiler from waiting for the next sub-level's loads.
+// one row component; every lane of a wave with rows runs it (the DPP
+// exchange needs whole quads).  Branch-free up to the global store: a block
+// past the row's count has zero coefficients and reads the ring's zero
+// entry, so its update t - ((0*0 + 0*0) + 0*0) = t - (+0) leaves t unchanged
+// bit for bit (also -0, infinities and NaN); lanes without a row (and lane 3)
+// write their value to the ring's dummy entry.  Everything loaded is thus
+// consumed on every path, which keeps the compiler from waiting for the next
+// sub-level's loads.
 template <bool kFwd, int K>
-__device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], double* out,
+__device__ __forceinline__ void quad_compute(double (*ring)[kRingLen], double* out,
                                              const QuadRow& q) {
   constexpr int kMask = kRingRows - 1;
   if (!q.live) return;  // no row in this wave: nothing to compute (its loads are not skipped)
   const int lane = threadIdx.x & 3;
-  const bool has = q.i >= 0;
   double y[K][3];
   const char* rb = reinterpret_cast<const char*>(&ring[0][0]);
-  constexpr int kRow = (kRingRows + 1) * (int)sizeof(double);  // bytes per ring row
+  constexpr int kRow = kRingLen * (int)sizeof(double);  // bytes per ring row
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const char* at = rb + q.col[k];
@@ -556,22 +549,16 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], doub
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const double* a = q.a + k * 3;
-    const double u = t - (a[0] * y[k][0] + a[1] * y[k][1] + a[2] * y[k][2]);
-    t = (k < q.cnt) ? u : t;
+    t = t - (a[0] * y[k][0] + a[1] * y[k][1] + a[2] * y[k][2]);
   }
   double z = t;
   if (kFwd) {
     const double t0 = quad_bcast<0>(t), t1 = quad_bcast<1>(t), t2 = quad_bcast<2>(t);
     z = q.d[0] * t0 + q.d[1] * t1 + q.d[2] * t2;
   }
-  const bool own = has && lane < 3;
-  ring[lane < 3 ? lane : 2][own ? (q.s & kMask) : kRingRows] = z;
-  if (own) {
-    if (kFwd)
-      out[3 * q.bs + lane] = z;  // backward-slot order
-    else
-      out[3 * q.i + lane] = z;   // row order
-  }
+  const bool own = q.has && lane < 3;
+  ring[lane < 3 ? lane : 2][own ? (q.s & kMask) : kRingDummy] = z;
+  if (own) out[3 * q.out + lane] = z;  // forward: backward-slot order; backward: row order
 }
 
 // one sweep over the sub-levels; three register sets rotate so that the
@@ -579,7 +566,7 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingRows + 1], doub
 template <bool kFwd, int K>
 __device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, int nq,
                                            const double* rhs, double* out,
-                                           double (*ring)[kRingRows + 1]) {
+                                           double (*ring)[kRingLen]) {
   const QuadSrc<kFwd, K> src = quad_src<kFwd, K>(c, rhs);
   QuadRow qa, qb, qc;
   quad_load<kFwd, K>(src, qptr, nq, 0, qa);
@@ -618,11 +605,12 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
     dst = c.z;
     prod = c.prod;
   }
-  __shared__ double ring[3][kRingRows + 1];  // + a dummy slot
+  __shared__ double ring[3][kRingLen];  // + the dummy and the zero entry
   __shared__ int qfp[kQuadLevels + 1];
   __shared__ int qbp[kQuadLevels + 1];
   for (int l = threadIdx.x; l <= c.nqf; l += blockDim.x) qfp[l] = c.qf_ptr[l];
   for (int l = threadIdx.x; l <= c.nqb; l += blockDim.x) qbp[l] = c.qb_ptr[l];
+  if (threadIdx.x < 3) ring[threadIdx.x][kRingZero] = 0.0;  // never written again
   __syncthreads();
   quad_sweep<true, K>(c, qfp, c.nqf, c.sf, c.sb, ring);
   __syncthreads();  // the backward sweep reads the complete forward result
