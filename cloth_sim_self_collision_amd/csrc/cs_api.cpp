@@ -210,7 +210,6 @@ int build(Cloth* c, const cs_mesh* m) {
   const int kl = S.kl > 0 ? S.kl : 0, ku = S.ku > 0 ? S.ku : 0;
   A.reserve(&frc, kl ? 4 * (size_t)np : 0);
   A.reserve(&fsl, kl ? (size_t)np : 0);
-  A.reserve(&d.sf, 3 * (size_t)np);
   A.reserve(&d.sb, 3 * (size_t)np);
   A.reserve(&fmeta, kl ? (size_t)meta_groups(std::max(kl, ku)) * 8 * np : 0);
   A.reserve(&brc, kl ? 2 * (size_t)np : 0);

@@ -141,12 +141,11 @@ struct DevCloth {
   const unsigned short* f_meta;
   const unsigned short* b_meta;
   const int* fslot;  // forward slot of each row
-  double* sf;        // sweep right-hand side in forward-slot order [3 * slot + r]
   double* sb;        // forward result in backward-slot order
   // packed layout: value v = k * 3 + j (entry (r, j) of block k of a slot's
   // row, r the output component) at [(((v / 2) * np + slot) * 3 + r) * 2 + v % 2]
   double* f_blk;
-  double* f_dinv;  // D^-1 entry (r, j) the same way with v = j (4 values, one padding)
+  double* f_dinv;  // D^-1 entry (r, j) the same way with v = j; v = 3: the forward sweep's right-hand side
   double* b_blk;
   int* ising;      // per-row singular-pivot flags of the last factorisation
 };

@@ -356,8 +356,9 @@ __global__ __launch_bounds__(1024) void k_ilu_apply(Sel sel, const double* src, 
 // (r, j) of block k) at [(((v / 2) * np + slot) * 3 + r) * 2 + v % 2] (lane 3
 // reads lane 2's), so one load instruction of a wave reads 768 contiguous
 // bytes and a row component costs ceil(3K / 2) block loads.  The load
-// instructions issued per sub-level, not HBM, bound the sweep.  The forward sweep reads its right-hand side in forward-slot order
-// (sf) and writes its result in backward-slot order (sb), so that no load of
+// instructions issued per sub-level, not HBM, bound the sweep.  The forward
+// sweep reads its right-hand side in forward-slot order (inside the packed
+// D^-1) and writes its result in backward-slot order (sb), so that no load of
 // either sweep depends on another load.
 
 // 16-byte pairs of a slot's values per lane (see above)
@@ -395,14 +396,21 @@ __global__ void k_pack_factor(Sel sel) {
   }
 }
 
-// x (row order) -> sf (forward-slot order)
+// The forward sweep's right-hand side rides in the free fourth value of each
+// slot's D^-1 pairs (value 3 of the packed D^-1): one load gives a lane its
+// last D^-1 entry and its right-hand side.  Written after k_pack_factor.
+__device__ __forceinline__ size_t fwd_rhs_index(int np, int slot, int r) {
+  return pack_index(3, slot, r, np);
+}
+
+// x (row order) -> the forward sweep's right-hand side (forward-slot order)
 template <class Sel>
 __global__ void k_gather_fslots(Sel sel, const double* x) {
   const DevCloth c = sel.cloth();
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
   const int sl = e / 3, r = e - 3 * sl;
-  c.sf[e] = x[3 * c.f_rci[4 * sl] + r];
+  c.f_dinv[fwd_rhs_index(c.np, sl, r)] = x[3 * c.f_rci[4 * sl] + r];
 }
 
 constexpr int kQuadThreads = 4 * kQuadRows;  // 512: one sub-level of 128 rows per pass
@@ -505,7 +513,7 @@ __device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int
   q.out = (int)(word16(K) | (word16(K + 1) << 16));
   const unsigned lr = lane < 3 ? lane : 2;  // lane 3 shadows lane 2 (same cache line)
   const unsigned e = 3 * sl + lr;
-  q.t = ld32(src.rhs, e);
+  if (!kFwd) q.t = ld32(src.rhs, e);  // forward: with the last D^-1 entry below
 #pragma unroll
   for (int p = 0; p < pack_pairs(K); p++) {
     const double2 w = ld32(src.blk[p], e);
@@ -518,6 +526,7 @@ __device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int
     q.d[0] = w0.x;
     q.d[1] = w0.y;
     q.d[2] = w1.x;
+    q.t = w1.y;
   }
 }
 
@@ -591,8 +600,9 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, i
   }
 }
 
-// M^-1 (L+D)(I+D^-1 U) solve of the factor: forward sweep from c.sf (the
-// right-hand side in forward-slot order) into c.sb, backward sweep into dst
+// M^-1 (L+D)(I+D^-1 U) solve of the factor: forward sweep from the
+// right-hand side in forward-slot order (fwd_rhs_index) into c.sb, backward
+// sweep into dst
 // (row order); prod = src_row .* dst when prod is given
 // K: the packed blocks per row (DevCloth::kpack)
 template <int K, class Sel>
@@ -612,7 +622,7 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
   for (int l = threadIdx.x; l <= c.nqb; l += blockDim.x) qbp[l] = c.qb_ptr[l];
   if (threadIdx.x < 3) ring[threadIdx.x][kRingZero] = 0.0;  // never written again
   __syncthreads();
-  quad_sweep<true, K>(c, qfp, c.nqf, c.sf, c.sb, ring);
+  quad_sweep<true, K>(c, qfp, c.nqf, nullptr, c.sb, ring);
   __syncthreads();  // the backward sweep reads the complete forward result
   quad_sweep<false, K>(c, qbp, c.nqb, c.sb, dst, ring);
   __syncthreads();
@@ -638,13 +648,13 @@ static void launch_quad(int kq, dim3 grid, const Sel& sel, const double* src, do
 
 // one forward+backward substitution launch (variants give identical results):
 // 0 automatic (the quad sweep when the mesh allows it), 1 row-by-row CSR
-// sweep.  `sf_ready`: c.sf already holds src in forward-slot order (the PCG
-// producers write it).
+// sweep.  `rhs_ready`: the forward sweep's right-hand side already holds src
+// (the PCG producers write it).
 static void launch_sweep(const DevCloth& c, const double* src, double* dst, double* prod,
-                         hipStream_t s, bool sf_ready = false) {
+                         hipStream_t s, bool rhs_ready = false) {
   const One one{c, {}};
   if (quad_ok(c) && c.sweep_variant != 1) {
-    if (!sf_ready)
+    if (!rhs_ready)
       hipLaunchKernelGGL(k_gather_fslots<One>, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, one, src);
     launch_quad(c.kpack, dim3(1), one, src, dst, prod, s);
   } else {
@@ -752,7 +762,7 @@ __global__ void k_update_rx(Sel sel, int it) {
   c.prod2[e] = r * r;
   if (c.fslot) {
     const int i = e / 3;
-    c.sf[3 * c.fslot[i] + (e - 3 * i)] = r;  // the next sweep's input, slot order
+    c.f_dinv[fwd_rhs_index(c.np, c.fslot[i], e - 3 * i)] = r;  // the next sweep's input
   }
 }
 
@@ -773,7 +783,7 @@ __global__ void k_pcg_init(Sel sel, double tol2) {
   c.prod[e] = b * b;
   if (c.fslot) {
     const int i = e / 3;
-    c.sf[3 * c.fslot[i] + (e - 3 * i)] = b;
+    c.f_dinv[fwd_rhs_index(c.np, c.fslot[i], e - 3 * i)] = b;
   }
 }
 
