@@ -413,7 +413,7 @@ __global__ void k_gather_fslots(Sel sel, const double* x) {
   c.f_dinv[fwd_rhs_index(c.np, sl, r)] = x[3 * c.f_rci[4 * sl] + r];
 }
 
-constexpr int kQuadThreads = 4 * kQuadRows;  // 512: one sub-level of 128 rows per pass
+constexpr int kQuadThreads = 4 * kQuadRows;  // at most 512: four lanes per row of a sub-level
 
 // value of lane j of this lane's quad
 template <int j>
@@ -630,10 +630,17 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
     for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src_row[e] * dst[e];
 }
 
+// threads of a quad sweep: four per row of the widest sub-level (at most
+// kQuadRows rows), in whole waves; waves past it would only load and wait
+inline int quad_threads(int wmax) {
+  const int rows = std::min(std::max(wmax, 1), kQuadRows);
+  return 4 * ((rows + 15) / 16 * 16);
+}
+
 template <class Sel>
-static void launch_quad(int kq, dim3 grid, const Sel& sel, const double* src, double* dst,
-                        double* prod, hipStream_t s) {
-  const dim3 tb(kQuadThreads);
+static void launch_quad(int kq, int wmax, dim3 grid, const Sel& sel, const double* src,
+                        double* dst, double* prod, hipStream_t s) {
+  const dim3 tb(quad_threads(wmax));
   switch (kq) {
     case 1: hipLaunchKernelGGL((k_ilu_apply_quad<1, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
     case 2: hipLaunchKernelGGL((k_ilu_apply_quad<2, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
@@ -656,7 +663,7 @@ static void launch_sweep(const DevCloth& c, const double* src, double* dst, doub
   if (quad_ok(c) && c.sweep_variant != 1) {
     if (!rhs_ready)
       hipLaunchKernelGGL(k_gather_fslots<One>, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, one, src);
-    launch_quad(c.kpack, dim3(1), one, src, dst, prod, s);
+    launch_quad(c.kpack, c.wmax, dim3(1), one, src, dst, prod, s);
   } else {
     hipLaunchKernelGGL(k_ilu_apply<One>, dim3(1), dim3(1024), 0, s, one, src, dst, prod);
   }
@@ -1068,7 +1075,7 @@ static void launch_sweep_batch(const BatchView& b, hipStream_t s) {
   const Many m{b.tab, b.ptab};
   if (b.piped) {
     // every cloth packed: r is already in slot order (k_pcg_init / k_update_rx)
-    launch_quad(b.kpack, dim3(1, b.n), m, nullptr, nullptr, nullptr, s);
+    launch_quad(b.kpack, b.max_wmax, dim3(1, b.n), m, nullptr, nullptr, nullptr, s);
   } else {
     hipLaunchKernelGGL(k_ilu_apply<Many>, dim3(1, b.n), dim3(1024), 0, s, m, nullptr, nullptr, nullptr);
   }
