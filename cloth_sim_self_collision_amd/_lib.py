@@ -74,6 +74,8 @@ SIGNATURES = {
     "cs_batch_step": (C.c_int, [_V, C.c_int, _V]),
     "cs_bandwidth_probe": (C.c_int, [C.c_longlong, C.c_int, _D]),
     "cs_read_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, _D]),
+    "cs_set_setup_cache": (C.c_int, [C.c_char_p]),
+    "cs_setup_cache_stats": (C.c_int, [C.POINTER(C.c_longlong)] * 3),
     "cs_step_internal": (C.c_int, [_V, _D, _I]),
     "cs_contact_resolve": (C.c_int, [_V, _I, C.POINTER(StepStats)]),
     "cs_post_contact_update": (C.c_int, [_V]),
@@ -153,6 +155,19 @@ def device_count() -> int:
     n = C.c_int(0)
     load().cs_device_count(C.byref(n))
     return n.value
+
+
+def set_setup_cache(directory) -> None:
+    """Keep per-mesh setup products in `directory` (None: no cache)."""
+    lib = load()
+    check(lib.cs_set_setup_cache(None if directory is None else str(directory).encode()))
+
+
+def setup_cache_stats() -> dict:
+    lib = load()
+    h, m, w = C.c_longlong(), C.c_longlong(), C.c_longlong()
+    check(lib.cs_setup_cache_stats(C.byref(h), C.byref(m), C.byref(w)))
+    return {"hits": h.value, "misses": m.value, "writes": w.value}
 
 
 def inspect_setup(cloth) -> dict:

@@ -120,7 +120,7 @@ int check_params(const cs_params* p) {
 
 int build(Cloth* c, const cs_mesh* m) {
   MeshSetup& S = c->S;
-  std::string err = build_setup(S, m->np, m->nt, m->nq, m->tri, m->quad, m->xyz);
+  std::string err = build_setup_cached(S, m->np, m->nt, m->nq, m->tri, m->quad, m->xyz);
   if (!err.empty()) {
     set_error(err);
     return kBadArg;
@@ -782,7 +782,7 @@ int cs_inspect_setup(const cs_mesh* m, int* sizes, int* rowptr, int* colind, int
     return kBadArg;
   }
   MeshSetup S;
-  std::string err = build_setup(S, m->np, m->nt, m->nq, m->tri, m->quad, m->xyz);
+  std::string err = build_setup_cached(S, m->np, m->nt, m->nq, m->tri, m->quad, m->xyz);
   if (!err.empty()) {
     set_error(err);
     return kBadArg;
@@ -1021,4 +1021,14 @@ extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
   }
   CS_HIP(hipStreamSynchronize(s));
   return kOk;
+}
+
+extern "C" int cs_set_setup_cache(const char* dir) {
+  cs::set_setup_cache_dir(dir);
+  return cs::kOk;
+}
+
+extern "C" int cs_setup_cache_stats(long long* hits, long long* misses, long long* writes) {
+  cs::setup_cache_stats(hits, misses, writes);
+  return cs::kOk;
 }

@@ -71,12 +71,18 @@ constexpr int kRingRows = 4096;
 constexpr int kRingDummy = kRingRows, kRingZero = kRingRows + 1, kRingLen = kRingRows + 2;
 constexpr int kQuadRows = 128;  // rows per sub-level of the quad sweep (4 lanes each)
 constexpr int kMaxBlocksPerRow = 8;
-// packed layout: column indices in groups of four per slot
 // 16-byte groups of sweep metadata per slot for k blocks per row
 inline constexpr int meta_groups(int k) { return (k + 2 + 7) / 8; }
 
 // Returns an empty string on success, else a description of the problem.
 std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,
                         const double* xyz_init);
+
+// build_setup through the on-disk cache of setup products (cs_setup_cache.cpp;
+// no cache directory set: plain build_setup)
+std::string build_setup_cached(MeshSetup& s, int np, int nt, int nq, const int* tri,
+                               const int* quad, const double* xyz_init);
+void set_setup_cache_dir(const char* dir);  // null or "": no cache
+void setup_cache_stats(long long* hits, long long* misses, long long* writes);
 
 }  // namespace cs

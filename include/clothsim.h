@@ -222,6 +222,17 @@ int cs_element_bend(int n, const double* C, const double* c, double stiff, doubl
 int cs_element_contact(int n, const double* p, double stiff, double clearance, int kind,
                        const double* obstacle, double* dW, double* ddW);
 
+/* On-disk cache of the per-mesh setup (pattern, level schedules, sweep
+ * slots, gather lists, BVH topology), the work cs_cloth_create and
+ * cs_inspect_setup do on the host before anything reaches the device
+ * (self_contact_sparse/main.cpp:561-583 plus this library's schedules; 2 s at
+ * 512^2).  Files are keyed by the mesh (connectivity and initial positions)
+ * and the layout version; a file that is missing, truncated or for another
+ * mesh is rebuilt.  dir NULL or "": no cache (the default).  Process-wide;
+ * safe to call from any thread. */
+int cs_set_setup_cache(const char* dir);
+/* cache lookups that hit, missed, and files written since the process began */
+int cs_setup_cache_stats(long long* hits, long long* misses, long long* writes);
 /* host-only: the per-mesh setup of cs_cloth_create without a device.
  * sizes[10] = np, nt, nq, nnz, nnode, forward levels, backward levels, max
  * lower / upper blocks per row of the packed sweeps (0 = not packed), 1 if

@@ -315,3 +315,27 @@ def test_served_results_equal_alone(sim_mod, oracle_mod):
             assert len(res.step_ms) == reqs[res.rid].steps
             ax, av = alone[res.rid]
             assert np.array_equal(res.xyz, ax) and np.array_equal(res.uvw, av), (budget, res.rid)
+
+
+def test_cloth_from_setup_cache_steps_identically(sim_mod, tmp_path):
+    """A cloth created from the on-disk setup cache (pattern, schedules,
+    sweep slots, gather lists, BVH all loaded, none rebuilt) steps exactly
+    like one whose setup was built."""
+    from cloth_sim_self_collision_amd import _lib
+    cl = scene.drape_over_sphere(32)
+    steps = 20
+    with sim_mod.DeviceCloth(cl) as d:
+        d.step(steps)
+        want = d.get_state()
+    try:
+        _lib.set_setup_cache(tmp_path)
+        with sim_mod.DeviceCloth(cl):
+            pass  # builds and writes the file
+        s0 = _lib.setup_cache_stats()
+        with sim_mod.DeviceCloth(cl) as d:
+            assert _lib.setup_cache_stats()["hits"] == s0["hits"] + 1
+            d.step(steps)
+            got = d.get_state()
+    finally:
+        _lib.set_setup_cache(None)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])

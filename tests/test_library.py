@@ -87,3 +87,53 @@ def test_level_schedules_are_dependency_closed(lib):
         assert np.all(lo < i) and np.all(ci[di[i]:rp[i + 1]] > i)
         lvl[i] = 1 + lvl[lo].max() if len(lo) else 0
     assert lvl.max() + 1 == s["levels_forward"]
+
+
+def _same_setup(a, b):
+    for k in ("rowptr", "colind", "diaind", "nodes"):
+        if not np.array_equal(a[k], b[k]):
+            return False
+    return all(a[k] == b[k] for k in ("levels_forward", "levels_backward", "max_lower",
+                                      "max_upper", "ring_ok"))
+
+
+def test_setup_cache_hits_rebuilds_and_keys_by_mesh(lib, tmp_path):
+    """The on-disk setup cache: a cold build writes a file, the next build of
+    the same mesh loads it, a truncated file is rebuilt, another mesh (same
+    connectivity, other positions) misses; every result equals the uncached
+    setup."""
+    cl = scene.drop_on_obstacle(20, 0.5, scene.sphere((0.0, 0.0, 0.0), 0.3), seed=3)
+    other = scene.drop_on_obstacle(20, 0.5, scene.sphere((0.0, 0.0, 0.0), 0.3), seed=4)
+    plain = lib.inspect_setup(cl)
+    plain_other = lib.inspect_setup(other)
+    try:
+        lib.set_setup_cache(tmp_path)
+        # inspect_setup calls cs_inspect_setup twice (sizes, then arrays): two
+        # lookups per call, the second of a cold build already a hit
+        s0 = lib.setup_cache_stats()
+        cold = lib.inspect_setup(cl)
+        s1 = lib.setup_cache_stats()
+        assert (s1["misses"] - s0["misses"], s1["writes"] - s0["writes"],
+                s1["hits"] - s0["hits"]) == (1, 1, 1)
+        files = list(tmp_path.glob("*.cssetup"))
+        assert len(files) == 1
+        warm = lib.inspect_setup(cl)
+        s2 = lib.setup_cache_stats()
+        assert s2["hits"] - s1["hits"] == 2 and s2["misses"] == s1["misses"]
+        assert _same_setup(cold, plain) and _same_setup(warm, plain)
+        # a truncated file is not trusted
+        data = files[0].read_bytes()
+        files[0].write_bytes(data[: len(data) // 2])
+        again = lib.inspect_setup(cl)
+        s3 = lib.setup_cache_stats()
+        assert s3["misses"] - s2["misses"] == 1 and s3["writes"] - s2["writes"] == 1
+        assert _same_setup(again, plain)
+        assert files[0].read_bytes() == data  # rewritten whole
+        # other positions, other key
+        o = lib.inspect_setup(other)
+        s4 = lib.setup_cache_stats()
+        assert s4["misses"] - s3["misses"] == 1
+        assert _same_setup(o, plain_other)
+        assert len(list(tmp_path.glob("*.cssetup"))) == 2
+    finally:
+        lib.set_setup_cache(None)
