@@ -111,7 +111,7 @@ int check_params(const cs_params* p) {
     set_error("params need dt > 0, mass_point > 0, pcg_maxit >= 0, pcg_tol >= 0");
     return kBadArg;
   }
-  if (p->obstacle_kind < 0 || p->obstacle_kind > 4) {
+  if (p->obstacle_kind < 0 || p->obstacle_kind > CS_OBST_MAX) {
     set_error("unknown obstacle kind");
     return kBadArg;
   }

@@ -122,7 +122,7 @@ extern "C" int cs_element_bend(int n, const double* C, const double* c, double s
 
 extern "C" int cs_element_contact(int n, const double* p, double stiff, double clearance,
                                   int kind, const double* obstacle, double* dW, double* ddW) {
-  if (n < 0 || (n > 0 && (!p || !obstacle || !dW || !ddW)) || kind < 0 || kind > 4)
+  if (n < 0 || (n > 0 && (!p || !obstacle || !dW || !ddW)) || kind < 0 || kind > CS_OBST_MAX)
     return kBadArg;
   if (n == 0) return kOk;
   Scratch s;

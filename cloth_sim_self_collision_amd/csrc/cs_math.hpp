@@ -340,6 +340,33 @@ CS_HD void obstacle_depth(int kind, const double* q, const double* p, double* pd
       for (int i = 0; i < 3; i++) n[i] = (s[i] < 0.0 ? -o[i] : o[i]) / len;
       *pd = -len;
     }
+  } else if (kind == 5 || kind == 6) {
+    // bar: the capsule around the segment q[0..2]-q[3..5] with radius q[6],
+    // pd = r - |p - c| with c the closest point of the segment, n = (p - c)/|p - c|
+    double e[3], w[3];
+    for (int i = 0; i < 3; i++) {
+      e[i] = q[3 + i] - q[i];
+      w[i] = p[i] - q[i];
+    }
+    const double ee = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    double t = (w[0] * e[0] + w[1] * e[1] + w[2] * e[2]) / ee;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    for (int i = 0; i < 3; i++) n[i] = w[i] - t * e[i];
+    const double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    n[0] /= len;
+    n[1] /= len;
+    n[2] /= len;
+    *pd = q[6] - len;
+    if (kind == 6) {
+      // and the ground z = q[7] (normal +z): the deeper of the two, the bar on ties
+      const double pg = q[7] - p[2];
+      if (pg > *pd) {
+        n[0] = 0.0;
+        n[1] = 0.0;
+        n[2] = 1.0;
+        *pd = pg;
+      }
+    }
   } else {
     *pd = -100;
     n[0] = n[1] = n[2] = 0;

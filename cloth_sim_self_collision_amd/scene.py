@@ -23,7 +23,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List, Optional
+from typing import Optional
 
 import numpy as np
 
@@ -33,6 +33,8 @@ OBST_PLANE = 1    # pd = d - dot(n, p), normal n (unit)
 OBST_SPHERE = 2   # pd = r - |p - c|, n = (p - c)/|p - c|
 OBST_SPHERE_INV = 3  # self_contact_sparse/main.cpp:117-133 sign-flipped sphere
 OBST_BOX = 4         # axis-aligned box (center, half-extents); not in the reference
+OBST_BAR = 5         # capsule around a segment (a, b, radius); not in the reference
+OBST_BAR_GROUND = 6  # the bar and the ground z = h, the deeper of the two
 
 _MASK64 = (1 << 64) - 1
 
@@ -60,14 +62,30 @@ class SplitMix64:
         """U(-a, a) as a*(2u-1)."""
         return a * (2.0 * self.uniform() - 1.0)
 
+    def symmetric_array(self, a: float, count: int) -> np.ndarray:
+        """The next `count` values of symmetric(a), computed together (the
+        same doubles: wrapping uint64 arithmetic, the same float operations)."""
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + k * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        self.state = (self.state + count * 0x9E3779B97F4A7C15) & _MASK64
+        u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return a * (2.0 * u - 1.0)
+
 
 @dataclasses.dataclass
 class Obstacle:
     kind: int = OBST_NONE
-    params: tuple = ()  # plane: (nx,ny,nz,d); sphere: (cx,cy,cz,r)
+    # plane: (nx,ny,nz,d); sphere: (cx,cy,cz,r); box: (cx,cy,cz,hx,hy,hz);
+    # bar: (ax,ay,az,bx,by,bz,r); bar over ground: (ax,ay,az,bx,by,bz,r,h)
+    params: tuple = ()
 
     def as_array(self) -> np.ndarray:
-        a = np.zeros(8, dtype=np.float64)
+        """[kind, params..., 0 padding]: the oracle's obstacle record (9 values)."""
+        a = np.zeros(9, dtype=np.float64)
         a[0] = float(self.kind)
         for i, v in enumerate(self.params):
             a[1 + i] = float(v)
@@ -99,6 +117,22 @@ class Obstacle:
             pd[out] = -ln[out]
             n[out] = (np.where(s < 0.0, -o, o) / np.where(ln > 0, ln, 1.0)[:, None])[out]
             return pd, n
+        if self.kind in (OBST_BAR, OBST_BAR_GROUND):
+            a = np.array(self.params[:3], dtype=np.float64)
+            e = np.array(self.params[3:6], dtype=np.float64) - a
+            w = p - a
+            ee = e[0] * e[0] + e[1] * e[1] + e[2] * e[2]
+            t = np.clip((w[:, 0] * e[0] + w[:, 1] * e[1] + w[:, 2] * e[2]) / ee, 0.0, 1.0)
+            v = w - t[:, None] * e
+            ln = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+            n = v / ln[:, None]
+            pd = float(self.params[6]) - ln
+            if self.kind == OBST_BAR_GROUND:
+                pg = float(self.params[7]) - p[:, 2]
+                g = pg > pd
+                pd = np.where(g, pg, pd)
+                n[g] = (0.0, 0.0, 1.0)
+            return pd, n
         c = np.array(self.params[:3], dtype=np.float64)
         r = float(self.params[3])
         v = p - c
@@ -117,6 +151,16 @@ def sphere(center, radius) -> Obstacle:
 def box(center, half) -> Obstacle:
     """Axis-aligned box; inside pd = distance to the nearest face."""
     return Obstacle(OBST_BOX, (center[0], center[1], center[2], half[0], half[1], half[2]))
+
+
+def bar(a, b, radius) -> Obstacle:
+    """Capsule of `radius` around the segment a-b."""
+    return Obstacle(OBST_BAR, (a[0], a[1], a[2], b[0], b[1], b[2], radius))
+
+
+def bar_over_ground(a, b, radius, ground_z) -> Obstacle:
+    """The bar and the ground plane z = ground_z (normal +z)."""
+    return Obstacle(OBST_BAR_GROUND, (a[0], a[1], a[2], b[0], b[1], b[2], radius, ground_z))
 
 
 def plane(normal, offset) -> Obstacle:
@@ -217,23 +261,25 @@ def bending_quads(tri: np.ndarray) -> np.ndarray:
     (edges (1,2), (2,0), (0,1)).
     """
     tri = np.asarray(tri, dtype=np.int64).reshape(-1, 3)
-    e2n = ((1, 2), (2, 0), (0, 1))
-    owner = {}
-    quads: List[tuple] = []
-    for it, t in enumerate(tri):
-        for ie, (a, b) in enumerate(e2n):
-            key = (min(t[a], t[b]), max(t[a], t[b]))
-            owner.setdefault(key, []).append((it, ie))
-    for it, t in enumerate(tri):
-        for ie, (a, b) in enumerate(e2n):
-            key = (min(t[a], t[b]), max(t[a], t[b]))
-            lst = owner[key]
-            if len(lst) != 2 or lst[0][0] != it:
-                continue
-            jt, je = lst[1]
-            opp0 = int(t[ie])
-            opp1 = int(tri[jt][je])
-            quads.append((opp0, opp1, int(t[a]), int(t[b])))
+    if len(tri) == 0:
+        return np.zeros(0, dtype=np.int32)
+    # every (triangle, local edge) in that order; an edge key occurs once per
+    # triangle that has it, in order (a stable sort keeps that order)
+    ie = np.tile(np.arange(3), len(tri))
+    it = np.repeat(np.arange(len(tri)), 3)
+    va = tri[it, np.array([1, 2, 0])[ie]]
+    vb = tri[it, np.array([2, 0, 1])[ie]]
+    key = np.minimum(va, vb) * (int(tri.max()) + 1) + np.maximum(va, vb)
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    start = np.flatnonzero(np.r_[True, ks[1:] != ks[:-1]])
+    count = np.diff(np.r_[start, len(ks)])
+    first = order[start[count == 2]]       # the lower triangle's (triangle, edge)
+    second = order[start[count == 2] + 1]  # the other triangle's
+    sel = np.argsort(first)                # by lower triangle, then its local edge
+    first, second = first[sel], second[sel]
+    quads = np.stack([tri[first // 3, first % 3], tri[second // 3, second % 3],
+                      va[first], vb[first]], axis=1)
     return np.asarray(quads, dtype=np.int32).reshape(-1)
 
 
@@ -253,32 +299,25 @@ def grid_cloth(nx: int, ny: int, size_x: float, size_y: float, *,
     hy = size_y / (ny - 1)
     x0 = center[0] - 0.5 * size_x
     y0 = center[1] - 0.5 * size_y
-    rng = SplitMix64(seed)
+    ix = np.repeat(np.arange(nx), ny)
+    iy = np.tile(np.arange(ny), nx)
     xyz = np.empty((nx * ny, 3), dtype=np.float64)
-    for ix in range(nx):
-        for iy in range(ny):
-            i = ix * ny + iy
-            xyz[i, 0] = x0 + ix * hx
-            xyz[i, 1] = y0 + iy * hy
-            xyz[i, 2] = z
+    xyz[:, 0] = x0 + ix * hx
+    xyz[:, 1] = y0 + iy * hy
+    xyz[:, 2] = z
     if jitter > 0.0:
-        for i in range(nx * ny):
-            xyz[i, 2] = z + rng.symmetric(jitter)
-    tri = np.empty((2 * (nx - 1) * (ny - 1), 3), dtype=np.int32)
-    k = 0
-    for ix in range(nx - 1):
-        for iy in range(ny - 1):
-            v00 = ix * ny + iy
-            v10 = (ix + 1) * ny + iy
-            v01 = ix * ny + iy + 1
-            v11 = (ix + 1) * ny + iy + 1
-            if alternating and ((ix + iy) & 1):
-                tri[k] = (v00, v10, v11)
-                tri[k + 1] = (v00, v11, v01)
-            else:
-                tri[k] = (v00, v10, v01)
-                tri[k + 1] = (v11, v01, v10)
-            k += 2
+        xyz[:, 2] = z + SplitMix64(seed).symmetric_array(jitter, nx * ny)
+    cx = np.repeat(np.arange(nx - 1), ny - 1)
+    cy = np.tile(np.arange(ny - 1), nx - 1)
+    v00 = cx * ny + cy
+    v10 = (cx + 1) * ny + cy
+    v01 = cx * ny + cy + 1
+    v11 = (cx + 1) * ny + cy + 1
+    alt = ((cx + cy) & 1).astype(bool) if alternating else np.zeros(len(cx), dtype=bool)
+    t = np.empty((len(cx), 2, 3), dtype=np.int32)
+    t[:, 0] = np.where(alt[:, None], np.stack([v00, v10, v11], 1), np.stack([v00, v10, v01], 1))
+    t[:, 1] = np.where(alt[:, None], np.stack([v00, v11, v01], 1), np.stack([v11, v01, v10], 1))
+    tri = t.reshape(-1, 3)
     tri = tri.reshape(-1)
     quad = bending_quads(tri)
     return xyz.reshape(-1), tri, quad, size_x * size_y
@@ -377,6 +416,36 @@ def batch_drop(count: int = 64, *, seed: int = 0, side_range=(24, 128),
     return out
 
 
+def stacked_sheets(count: int = 4, n: int = 512, *, gap: float = 0.02, size: float = 1.0,
+                   z0: float = 0.1, seed: int = 0, jitter: float = 2.0e-3, dt: float = 0.002,
+                   clearance: float = 0.005, obstacle: Optional[Obstacle] = None,
+                   pcg_tol: float = 1.0e-6, pcg_maxit: int = 1000, name: str = "") -> Cloth:
+    """`count` n x n sheets of side `size` (alternating diagonals), sheet i
+    flat at z0 + i*gap with U(+-jitter) z-jitter from SplitMix64(seed + i),
+    as one cloth whose vertices and triangles are the sheets' in order (sheet
+    on sheet contact is self-contact).  Default obstacle: a bar of radius 0.05
+    along y under the stack's middle (axis at z = 0) over the ground z = -0.3.
+    With count 4, n 512: 1,048,576 vertices, 2,088,968 triangles."""
+    if obstacle is None:
+        h = 0.75 * size
+        obstacle = bar_over_ground((0.0, -h, 0.0), (0.0, h, 0.0), 0.05, -0.3)
+    xs, ts, qs = [], [], []
+    for i in range(count):
+        xyz, tri, quad, _ = grid_cloth(n, n, size, size, z=z0 + i * gap, alternating=True,
+                                       jitter=jitter, seed=seed + i)
+        off = i * n * n
+        xs.append(xyz)
+        ts.append(tri + off)
+        qs.append(quad + off)
+    xyz = np.concatenate(xs)
+    np_ = len(xyz) // 3
+    return Cloth(xyz0=xyz.copy(), xyz=xyz.copy(), uvw=np.zeros_like(xyz),
+                 tri=np.concatenate(ts).astype(np.int32), quad=np.concatenate(qs).astype(np.int32),
+                 bc=np.zeros(np_, dtype=np.int32), area=count * size * size,
+                 material=material_default(), obstacle=obstacle, dt=dt, clearance=clearance,
+                 pcg_tol=pcg_tol, pcg_maxit=pcg_maxit, name=name or f"stack{count}x{n}")
+
+
 def fold_sweep(n: int = 256, *, size: float = 2.0, seed: int = 0, jitter: float = 1.0e-3,
                pcg_tol: float = 1.0e-6, pcg_maxit: int = 1000) -> Cloth:
     """Single large cloth folded onto itself.
@@ -430,10 +499,151 @@ def apply_sweep(cloth: Cloth, xyz: np.ndarray, t_next: float) -> None:
     x[v] = x0[v] + off
 
 
+# ---------------------------------------------------------------------------
+# scene configuration files (JSON objects)
+
+_OBSTACLE_KINDS = {
+    "none": lambda spec: NO_OBSTACLE,
+    "plane": lambda spec: plane(spec["normal"], spec["offset"]),
+    "sphere": lambda spec: sphere(spec["center"], spec["radius"]),
+    "box": lambda spec: box(spec["center"], spec["half"]),
+    "bar": lambda spec: bar(spec["a"], spec["b"], spec["radius"]),
+    "bar_over_ground": lambda spec: bar_over_ground(spec["a"], spec["b"], spec["radius"],
+                                                    spec["ground_z"]),
+}
+
+
+def obstacle_from_config(spec) -> Obstacle:
+    """{"kind": "sphere", "center": [0, 0, 0], "radius": 0.3} and the like:
+    plane (normal, offset), sphere (center, radius), box (center, half), bar
+    (a, b, radius), bar_over_ground (a, b, radius, ground_z), none."""
+    if spec is None:
+        return NO_OBSTACLE
+    kind = spec.get("kind", "none")
+    if kind not in _OBSTACLE_KINDS:
+        raise ValueError(f"unknown obstacle kind {kind!r} (known: {sorted(_OBSTACLE_KINDS)})")
+    return _OBSTACLE_KINDS[kind](spec)
+
+
+def obstacle_to_config(ob: Obstacle) -> dict:
+    p = [float(v) for v in ob.params]
+    if ob.kind == OBST_NONE:
+        return {"kind": "none"}
+    if ob.kind == OBST_PLANE:
+        return {"kind": "plane", "normal": p[:3], "offset": p[3]}
+    if ob.kind == OBST_SPHERE:
+        return {"kind": "sphere", "center": p[:3], "radius": p[3]}
+    if ob.kind == OBST_BOX:
+        return {"kind": "box", "center": p[:3], "half": p[3:6]}
+    if ob.kind == OBST_BAR:
+        return {"kind": "bar", "a": p[:3], "b": p[3:6], "radius": p[6]}
+    if ob.kind == OBST_BAR_GROUND:
+        return {"kind": "bar_over_ground", "a": p[:3], "b": p[3:6], "radius": p[6],
+                "ground_z": p[7]}
+    raise ValueError(f"obstacle kind {ob.kind} has no configuration form")
+
+
+def _mesh_cloth(cfg) -> Cloth:
+    xyz = np.asarray(cfg["xyz"], dtype=np.float64).reshape(-1)
+    tri = np.asarray(cfg["tri"], dtype=np.int32).reshape(-1)
+    np_ = len(xyz) // 3
+    if len(xyz) != 3 * np_ or len(tri) % 3 or len(tri) == 0:
+        raise ValueError("mesh: xyz needs 3 values per vertex and tri 3 indices per triangle")
+    if tri.min() < 0 or tri.max() >= np_:
+        raise ValueError("mesh: triangle index out of range")
+    bc = np.zeros(np_, dtype=np.int32)
+    for i in cfg.get("fixed", []):
+        bc[int(i)] = 1
+    if "area" in cfg:
+        area = float(cfg["area"])
+    else:  # total rest area of the triangles
+        p = xyz.reshape(-1, 3)
+        t = tri.reshape(-1, 3)
+        area = float(0.5 * np.linalg.norm(np.cross(p[t[:, 1]] - p[t[:, 0]],
+                                                   p[t[:, 2]] - p[t[:, 0]]), axis=1).sum())
+    return Cloth(xyz0=xyz.copy(), xyz=xyz.copy(), uvw=np.zeros_like(xyz), tri=tri,
+                 quad=bending_quads(tri), bc=bc, area=area, material=material_default(),
+                 obstacle=obstacle_from_config(cfg.get("obstacle")), dt=float(cfg.get("dt", 0.01)),
+                 clearance=float(cfg.get("clearance", 0.01)), name=cfg.get("name", "mesh"))
+
+
+def _material_from_config(m: dict, base: Material) -> Material:
+    kw = dict(lam=base.lam, mu=base.mu, stiff_bend=base.stiff_bend, density=base.density,
+              gravity=base.gravity, stiff_contact=base.stiff_contact)
+    if "young" in m:
+        lm = Material.from_young(float(m["young"]), float(m.get("poisson", 0.3)), stiff_bend=0.0)
+        kw["lam"], kw["mu"] = lm.lam, lm.mu
+    for k in ("lam", "mu", "stiff_bend", "density", "stiff_contact"):
+        if k in m:
+            kw[k] = float(m[k])
+    if "gravity" in m:
+        kw["gravity"] = tuple(float(g) for g in m["gravity"])
+    return Material(**kw)
+
+
+def from_config(cfg: dict):
+    """A cloth (or, for "batch", a list of cloths) from a JSON scene object.
+
+    "scene": "drape" (n, seed, jitter), "fold" (n, size, seed, jitter),
+    "drop" (n, ny, height, obstacle, seed, jitter, dt), "stack" (count, n,
+    gap, size, z0, seed, jitter, dt, clearance, obstacle), "batch" (count,
+    seed, side_range, height_range), "mesh" (xyz, tri, fixed vertex ids,
+    area, obstacle, dt, clearance) for any triangle mesh (bending stencils
+    from its interior edges).  Optional for all: pcg_tol, pcg_maxit,
+    material {lam, mu (or young, poisson), stiff_bend, density, gravity,
+    stiff_contact}, name.
+    """
+    kind = cfg.get("scene")
+    opt = {k: cfg[k] for k in ("seed", "jitter") if k in cfg}
+    if kind == "drape":
+        out = drape_over_sphere(int(cfg.get("n", 32)), **opt)
+    elif kind == "fold":
+        out = fold_sweep(int(cfg.get("n", 256)), size=float(cfg.get("size", 2.0)), **opt)
+    elif kind == "drop":
+        out = drop_on_obstacle(int(cfg["n"]), float(cfg.get("height", 0.5)),
+                               obstacle_from_config(cfg.get("obstacle",
+                                                            {"kind": "sphere", "center": [0, 0, 0],
+                                                             "radius": 0.3})),
+                               ny=cfg.get("ny"), dt=float(cfg.get("dt", 0.01)), **opt)
+    elif kind == "stack":
+        extra = {k: float(cfg[k]) for k in ("gap", "size", "z0", "dt", "clearance") if k in cfg}
+        ob = obstacle_from_config(cfg["obstacle"]) if "obstacle" in cfg else None
+        out = stacked_sheets(int(cfg.get("count", 4)), int(cfg.get("n", 512)), obstacle=ob,
+                             **extra, **opt)
+    elif kind == "batch":
+        out = batch_drop(int(cfg.get("count", 64)), seed=int(cfg.get("seed", 0)),
+                         side_range=tuple(cfg.get("side_range", (24, 128))),
+                         height_range=tuple(cfg.get("height_range", (0.3, 0.8))))
+    elif kind == "mesh":
+        out = _mesh_cloth(cfg)
+    else:
+        raise ValueError(f"unknown scene {kind!r} (drape, fold, drop, stack, batch, mesh)")
+    cloths = out if isinstance(out, list) else [out]
+    for cl in cloths:
+        if "pcg_tol" in cfg:
+            cl.pcg_tol = float(cfg["pcg_tol"])
+        if "pcg_maxit" in cfg:
+            cl.pcg_maxit = int(cfg["pcg_maxit"])
+        if "material" in cfg:
+            cl.material = _material_from_config(cfg["material"], cl.material)
+        if "name" in cfg and not isinstance(out, list):
+            cl.name = str(cfg["name"])
+    return out
+
+
+def load_config(path):
+    """from_config of a JSON file."""
+    import json
+    with open(path) as f:
+        return from_config(json.load(f))
+
+
 __all__ = [
     "SplitMix64", "Obstacle", "Material", "Sweep", "Cloth", "sphere", "plane", "NO_OBSTACLE",
     "bending_quads", "grid_cloth", "reference_square", "material_default",
     "drape_over_sphere", "fold_sweep", "reference_demo", "apply_sweep",
-    "OBST_NONE", "OBST_PLANE", "OBST_SPHERE", "OBST_SPHERE_INV", "OBST_BOX", "box",
-    "drop_on_obstacle", "batch_drop",
+    "OBST_NONE", "OBST_PLANE", "OBST_SPHERE", "OBST_SPHERE_INV", "OBST_BOX", "OBST_BAR",
+    "OBST_BAR_GROUND", "box", "bar", "bar_over_ground",
+    "drop_on_obstacle", "batch_drop", "stacked_sheets", "from_config", "load_config",
+    "obstacle_from_config", "obstacle_to_config",
 ]

@@ -48,6 +48,14 @@ extern "C" {
 #define CS_OBST_BOX 4        /* params (cx, cy, cz, hx, hy, hz), axis-aligned, half-extents h:
                               * inside pd = distance to the nearest face (n its outward
                               * normal, the first axis on ties), outside pd = -distance */
+#define CS_OBST_BAR 5        /* params (ax, ay, az, bx, by, bz, r): the capsule of radius r
+                              * around segment a-b; pd = r - |p - c|, c the segment's
+                              * closest point, n = (p - c)/|p - c| */
+#define CS_OBST_BAR_GROUND 6 /* params (ax, ay, az, bx, by, bz, r, h): the bar and the
+                              * ground z = h (normal +z); the deeper of the two, the bar
+                              * on ties.  Kinds 4-6 are not in the reference (obstacle
+                              * callbacks with its penetrationDepth convention) */
+#define CS_OBST_MAX 6
 
 typedef struct cs_cloth cs_cloth;
 

@@ -393,6 +393,33 @@ static void obstacle_depth(const double* obst, const double* p, double* pd, doub
       for (int i = 0; i < 3; i++) n[i] = (s[i] < 0.0 ? -o[i] : o[i]) / len;
       *pd = -len;
     }
+  } else if (kind == 5 || kind == 6) {
+    /* bar: the capsule around the segment obst[1..3]-obst[4..6] with radius
+     * obst[7]; kind 6 adds the ground z = obst[8] (normal +z), the deeper of
+     * the two, the bar on ties.  Not in the reference, same convention. */
+    double e[3], w[3];
+    for (int i = 0; i < 3; i++) {
+      e[i] = obst[4 + i] - obst[1 + i];
+      w[i] = p[i] - obst[1 + i];
+    }
+    const double ee = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+    double t = (w[0] * e[0] + w[1] * e[1] + w[2] * e[2]) / ee;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    for (int i = 0; i < 3; i++) n[i] = w[i] - t * e[i];
+    const double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    n[0] /= len;
+    n[1] /= len;
+    n[2] /= len;
+    *pd = obst[7] - len;
+    if (kind == 6) {
+      const double pg = obst[8] - p[2];
+      if (pg > *pd) {
+        n[0] = 0.0;
+        n[1] = 0.0;
+        n[2] = 1.0;
+        *pd = pg;
+      }
+    }
   } else {
     *pd = -100;
     n[0] = n[1] = n[2] = 0;
@@ -431,7 +458,7 @@ typedef struct {
   int *tri, *quad, *bc;
   double lam, mu, bend, grav[3], mass, kc, clear, dt, tol;
   int maxit;
-  double obst[8];
+  double obst[9];
   /* 3x3-block CSR without the diagonal + separate diagonal (ref: matrix_square_sparse.h:14-48) */
   int nnz;
   int *rowptr, *colind, *diaind;
@@ -734,7 +761,7 @@ void* orc_create(int np, const double* xyz0, const double* xyz_init, int nt, con
   c->dt = params[9];
   c->tol = params[10];
   c->maxit = (int)params[11];
-  memcpy(c->obst, obst, sizeof(double) * 8);
+  memcpy(c->obst, obst, sizeof(double) * 9);
   /* setup order of self_contact_sparse/main.cpp:574-582 */
   build_bvh(c, xyz_init);
   edge_of_elem(c->tri, nt, 3, np, &c->eidx, &c->earr);

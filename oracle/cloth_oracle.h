@@ -11,8 +11,10 @@
  *
  * params[12]: lambda, mu, stiff_bend, gx, gy, gz, mass_point, stiff_contact,
  *             clearance, dt, pcg_tol, pcg_maxit
- * obst[8]:    kind (0 none, 1 plane, 2 sphere, 3 inverted sphere), then
- *             plane (nx, ny, nz, d) or sphere (cx, cy, cz, r)
+ * obst[9]:    kind (0 none, 1 plane, 2 sphere, 3 inverted sphere, 4 box, 5 bar,
+ *             6 bar over ground), then up to 8 parameters: plane (nx, ny, nz, d),
+ *             sphere (cx, cy, cz, r), box (cx, cy, cz, hx, hy, hz), bar
+ *             (ax, ay, az, bx, by, bz, r), bar over ground (the bar's 7, h)
  * contact lists: 5 ints per element: is_fv, i0, i1, i2, i3 (canonical key)
  */
 #ifndef CLOTH_ORACLE_H

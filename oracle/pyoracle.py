@@ -113,7 +113,7 @@ class Oracle:
                       np.ascontiguousarray(cloth.tri, np.int32),
                       np.ascontiguousarray(cloth.quad, np.int32),
                       np.ascontiguousarray(cloth.bc, np.int32),
-                      cloth_params(cloth), cloth.obstacle.as_array()]
+                      cloth_params(cloth), _obst9(cloth.obstacle.as_array())]
         k = self._keep
         self.h = self.api["create"](self.np_, _dp(k[0]), _dp(k[1]), cloth.nt, _ip(k[2]), cloth.nq,
                                     _ip(k[3]), _ip(k[4]), _dp(k[5]), _dp(k[6]))
@@ -280,13 +280,21 @@ def element_bend(C_, c_, stiff, backend="port"):
     return dW, ddW
 
 
+def _obst9(a):
+    """The 9-value obstacle record the C code reads (kind, 8 parameters)."""
+    out = np.zeros(9, np.float64)
+    a = np.asarray(a, np.float64).ravel()
+    out[:min(9, len(a))] = a[:9]
+    return out
+
+
 def element_contact(c_, k, clearance, obst_array, backend="port"):
     api = _load(backend)
     W = np.zeros(1)
     dW = np.zeros(3)
     ddW = np.zeros(9)
     api["contact_elem"](_dp(np.ascontiguousarray(c_, np.float64)), k, clearance,
-                        _dp(np.ascontiguousarray(obst_array, np.float64)), _dp(W), _dp(dW), _dp(ddW))
+                        _dp(_obst9(obst_array)), _dp(W), _dp(dW), _dp(ddW))
     return float(W[0]), dW, ddW
 
 

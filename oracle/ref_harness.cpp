@@ -42,7 +42,7 @@ struct Quiet {
 };
 
 // obstacle for the penetrationDepth callback (set per call)
-double g_obst[8];
+double g_obst[9];
 
 void pd_none(double& pd, double* n, const double* p) {
   (void)p;
@@ -100,14 +100,46 @@ void pd_box(double& pd, double* n, const double* p) {
   }
 }
 
+// bar: the capsule around the segment g_obst[1..3]-g_obst[4..6] with radius
+// g_obst[7]; with the ground z = g_obst[8] (normal +z) the deeper of the two,
+// the bar on ties (same convention)
+void pd_bar(double& pd, double* n, const double* p) {
+  double e[3], w[3];
+  for (int i = 0; i < 3; i++) {
+    e[i] = g_obst[4 + i] - g_obst[1 + i];
+    w[i] = p[i] - g_obst[1 + i];
+  }
+  const double ee = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+  double t = (w[0] * e[0] + w[1] * e[1] + w[2] * e[2]) / ee;
+  t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+  for (int i = 0; i < 3; i++) n[i] = w[i] - t * e[i];
+  const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  n[0] /= len;
+  n[1] /= len;
+  n[2] /= len;
+  pd = g_obst[7] - len;
+}
+void pd_bar_ground(double& pd, double* n, const double* p) {
+  pd_bar(pd, n, p);
+  const double pg = g_obst[8] - p[2];
+  if (pg > pd) {
+    n[0] = 0.0;
+    n[1] = 0.0;
+    n[2] = 1.0;
+    pd = pg;
+  }
+}
+
 typedef void (*PdFn)(double&, double*, const double*);
 PdFn pick_pd(const double* obst) {
-  for (int i = 0; i < 8; i++) g_obst[i] = obst[i];
+  for (int i = 0; i < 9; i++) g_obst[i] = obst[i];
   switch ((int)obst[0]) {
     case 1: return pd_plane;
     case 2: return pd_sphere;
     case 3: return pd_sphere_inv;
     case 4: return pd_box;
+    case 5: return pd_bar;
+    case 6: return pd_bar_ground;
     default: return pd_none;
   }
 }
@@ -119,7 +151,7 @@ struct RefCtx {
   double lam, mu, stiff_bend, grav[3], mass_point, stiff_contact, clearance, dt;
   double tol;
   int maxit;
-  double obst[8];
+  double obst[9];
   CMatrixSquareSparse mat_A;
   CPreconditionerILU ilu_A;
   int iroot_bvh;
@@ -226,7 +258,7 @@ void* ref_create(int np, const double* xyz0, const double* xyz_init, int nt, con
   c->dt = params[9];
   c->tol = params[10];
   c->maxit = (int)params[11];
-  for (int i = 0; i < 8; i++) c->obst[i] = obst[i];
+  for (int i = 0; i < 9; i++) c->obst[i] = obst[i];
   // main.cpp:574-582
   std::vector<double> xinit(xyz_init, xyz_init + 3 * np);
   c->iroot_bvh = MakeBVHTopology_TopDown(c->tri, xinit, c->aNodeBVH);

@@ -31,6 +31,10 @@ def _scenes():
         "fold24": scene.fold_sweep(24, size=0.3),
         "box24": scene.drop_on_obstacle(24, 0.3, scene.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25)),
                                         seed=5),
+        "bar24": scene.drop_on_obstacle(
+            24, 0.2, scene.bar_over_ground((-0.6, 0.05, 0.0), (0.6, -0.05, 0.0), 0.05, -0.25),
+            seed=6),
+        "stack16": scene.stacked_sheets(2, 16, size=0.5, dt=0.005, seed=3),
     }
 
 
@@ -129,7 +133,9 @@ def test_element_routines_bitwise(sim_mod, oracle_mod):
     p = rng.standard_normal((n, 3)) * 0.4
     for ob in (scene.sphere((0.0, 0.0, 0.0), 0.3), scene.plane((0.0, 0.0, 1.0), -0.1),
                scene.Obstacle(scene.OBST_SPHERE_INV, (0.1, 0.0, 0.0, 0.35)),
-               scene.box((0.05, 0.0, -0.1), (0.2, 0.25, 0.15))):
+               scene.box((0.05, 0.0, -0.1), (0.2, 0.25, 0.15)),
+               scene.bar((-0.3, 0.1, 0.0), (0.4, -0.1, 0.05), 0.2),
+               scene.bar_over_ground((-0.3, 0.1, 0.0), (0.4, -0.1, 0.05), 0.15, -0.2)):
         g, h = sim_mod.element_contact(p, 1e3, 0.01, ob)
         for e in range(0, n, 17):
             _, g0, h0 = oracle_mod.element_contact(p[e], 1e3, 0.01, ob.as_array())
@@ -204,7 +210,8 @@ def test_velocity_update_and_host_obstacle_values(sim_mod, oracle_mod):
 @pytest.mark.parametrize("name,steps,variant", [("drape32", 100, 0), ("refdemo0", 60, 0),
                                                 ("refdemo1", 60, 0), ("fold24", 60, 0),
                                                 ("drape32", 60, 1), ("fold24", 40, 1),
-                                                ("box24", 60, 0)])
+                                                ("box24", 60, 0), ("bar24", 70, 0),
+                                                ("stack16", 60, 0)])
 def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant):
     cl = _scenes()[name]
     o = oracle_mod.Oracle(cl, "port")

@@ -137,3 +137,15 @@ def test_setup_cache_hits_rebuilds_and_keys_by_mesh(lib, tmp_path):
         assert len(list(tmp_path.glob("*.cssetup"))) == 2
     finally:
         lib.set_setup_cache(None)
+
+
+def test_stacked_sheets_setup_matches_oracle(lib, oracle_mod):
+    """Disconnected sheets in one cloth: the pattern and the BVH topology (the
+    reference's split over several components) equal the oracle's."""
+    cl = scene.stacked_sheets(3, 12, gap=0.02, size=0.5)
+    s = lib.inspect_setup(cl)
+    o = oracle_mod.Oracle(cl, "port")
+    rp, ci, di = o.pattern()
+    assert np.array_equal(s["rowptr"], rp) and np.array_equal(s["colind"], ci)
+    assert np.array_equal(s["diaind"], di)
+    assert np.array_equal(s["nodes"], o.bvh())

@@ -448,3 +448,36 @@ def test_proximity_set_equals_brute_force(oracle_mod):
     assert expect, "the scene must contain face-vertex proximities"
     assert expect <= found
     assert fv_found <= expect | unsure
+
+
+def test_bar_obstacles_closed_form(oracle_mod):
+    """Bar (capsule around (-0.5,0,0.1)-(0.5,0,0.1), radius 0.05) alone and
+    over the ground z = -0.2: depth and normal from the geometry, including a
+    point past the segment's end (closest point clamped to it) and the ground
+    taking over where it is the deeper penetration; in the scene layer, the
+    oracle and the reference harness alike."""
+    a, b, r = (-0.5, 0.0, 0.1), (0.5, 0.0, 0.1), 0.05
+    end = np.array([0.1, 0.0, 0.04])
+    cases = [
+        (scene.bar(a, b, r), (0.2, 0.03, 0.1), 0.02, (0, 1, 0)),
+        (scene.bar(a, b, r), (0.0, 0.0, 0.18), -0.03, (0, 0, 1)),
+        (scene.bar(a, b, r), (0.6, 0.0, 0.14), r - np.linalg.norm(end),
+         tuple(end / np.linalg.norm(end))),
+        (scene.bar_over_ground(a, b, r, -0.2), (0.3, 0.4, -0.21), 0.01, (0, 0, 1)),
+        (scene.bar_over_ground(a, b, r, -0.2), (0.0, 0.02, 0.1), 0.03, (0, 1, 0)),
+    ]
+    k, delta = 1e3, 0.01
+    for ob, p, want_pd, want_n in cases:
+        pd, n = ob.depth(np.array([p]))
+        assert np.isclose(pd[0], want_pd, atol=1e-12), (p, pd)
+        assert np.allclose(n[0], want_n, atol=1e-9), (p, n)
+        backends = ["port"] + (["reference"] if oracle_mod.available("reference") else [])
+        for be in backends:
+            W, dW, ddW = oracle_mod.element_contact(np.array(p), k, delta, ob.as_array(), backend=be)
+            s = want_pd + delta
+            if s > 0:
+                assert np.isclose(W, 0.5 * k * s * s), (be, p)
+                assert np.allclose(dW, -k * s * np.asarray(want_n), atol=1e-9), (be, p)
+                assert np.allclose(ddW.reshape(3, 3), k * np.outer(want_n, want_n), atol=1e-6)
+            else:
+                assert W == 0 and not dW.any() and not ddW.any()

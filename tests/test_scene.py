@@ -74,3 +74,79 @@ def test_sweep_offsets():
     assert np.all(d[np.setdiff1d(np.arange(cl.np_), v)] == 0)
     scene.apply_sweep(cl, x, 3.0)
     assert np.allclose((x - cl.xyz0).reshape(-1, 3)[v], cl.sweep.delta)
+
+
+def _same_cloth(a, b):
+    for f in ("xyz0", "xyz", "uvw", "tri", "quad", "bc"):
+        if not np.array_equal(getattr(a, f), getattr(b, f)):
+            return False
+    return (a.area, a.material, a.obstacle.kind, tuple(a.obstacle.params), a.dt, a.clearance,
+            a.pcg_tol, a.pcg_maxit) == (b.area, b.material, b.obstacle.kind,
+                                        tuple(b.obstacle.params), b.dt, b.clearance,
+                                        b.pcg_tol, b.pcg_maxit)
+
+
+def test_symmetric_array_continues_the_scalar_sequence():
+    a, b = scene.SplitMix64(11), scene.SplitMix64(11)
+    arr = a.symmetric_array(2e-3, 500)
+    assert np.array_equal(arr, [b.symmetric(2e-3) for _ in range(500)])
+    assert a.next_u64() == b.next_u64()
+
+
+def test_stack_config_sizes():
+    cl = scene.stacked_sheets(4, 512)
+    assert (cl.np_, cl.nt) == (1048576, 2088968)
+    assert cl.nq == 4 * (3 * 511 * 511 - 2 * 511)
+    z = cl.xyz[2::3].reshape(4, -1)
+    for i in range(4):  # sheet i at 0.1 + 0.02 i, jitter 2e-3
+        assert np.all(np.abs(z[i] - (0.1 + 0.02 * i)) <= 2e-3)
+    assert cl.obstacle.kind == scene.OBST_BAR_GROUND and cl.clearance == 0.005
+    assert cl.mass_point == 4.0 / cl.np_
+
+
+def test_scenes_from_config_equal_their_builders(tmp_path):
+    import json
+    box = {"kind": "box", "center": [0, 0, 0], "half": [0.25, 0.25, 0.25]}
+    pairs = [
+        ({"scene": "drape", "n": 16}, scene.drape_over_sphere(16)),
+        ({"scene": "fold", "n": 24, "size": 0.3}, scene.fold_sweep(24, size=0.3)),
+        ({"scene": "drop", "n": 20, "height": 0.3, "obstacle": box, "seed": 5},
+         scene.drop_on_obstacle(20, 0.3, scene.box((0, 0, 0), (0.25, 0.25, 0.25)), seed=5)),
+        ({"scene": "stack", "count": 2, "n": 10, "gap": 0.03, "size": 0.5},
+         scene.stacked_sheets(2, 10, gap=0.03, size=0.5)),
+    ]
+    for cfg, want in pairs:
+        assert _same_cloth(scene.from_config(cfg), want), cfg
+        path = tmp_path / "scene.json"
+        path.write_text(json.dumps(cfg))
+        assert _same_cloth(scene.load_config(path), want), cfg
+    got = scene.from_config({"scene": "batch", "count": 3, "seed": 2})
+    assert all(_same_cloth(a, b) for a, b in zip(got, scene.batch_drop(3, seed=2)))
+    # overrides
+    cl = scene.from_config({"scene": "drape", "n": 16, "pcg_tol": 1e-4, "pcg_maxit": 100,
+                            "material": {"young": 2e3, "poisson": 0.25, "gravity": [0, 0, -9.8]}})
+    m = scene.Material.from_young(2e3, 0.25, stiff_bend=0.0)
+    assert (cl.pcg_tol, cl.pcg_maxit) == (1e-4, 100)
+    assert (cl.material.lam, cl.material.mu, cl.material.gravity) == (m.lam, m.mu, (0, 0, -9.8))
+    assert cl.material.stiff_bend == scene.drape_over_sphere(16).material.stiff_bend
+
+
+def test_mesh_scene_and_obstacle_round_trip():
+    xyz, tri, quad, area = scene.grid_cloth(6, 5, 1.0, 0.8, z=0.2)
+    cl = scene.from_config({"scene": "mesh", "xyz": xyz.tolist(), "tri": tri.tolist(),
+                            "fixed": [0, 4], "obstacle": {"kind": "plane", "normal": [0, 0, 1],
+                                                          "offset": -0.1}})
+    assert np.array_equal(cl.quad, quad) and cl.bc.sum() == 2 and cl.bc[4] == 1
+    assert abs(cl.area - area) < 1e-12  # the triangles' rest area
+    for ob in (scene.NO_OBSTACLE, scene.sphere((1, 2, 3), 0.5), scene.plane((0, 0, 1), -0.5),
+               scene.box((0, 0, 0), (1, 2, 3)), scene.bar((0, 0, 0), (1, 0, 0), 0.1),
+               scene.bar_over_ground((0, -1, 0), (0, 1, 0), 0.05, -0.3)):
+        back = scene.obstacle_from_config(scene.obstacle_to_config(ob))
+        assert back.kind == ob.kind and np.allclose(back.params, ob.params)
+    import pytest
+    with pytest.raises(ValueError):
+        scene.from_config({"scene": "teapot"})
+    with pytest.raises(ValueError):
+        scene.obstacle_from_config({"kind": "torus"})
+    with pytest.raises(ValueError):
+        scene.from_config({"scene": "mesh", "xyz": [0, 0, 0, 1, 0, 0, 0, 1, 0], "tri": [0, 1, 3]})
