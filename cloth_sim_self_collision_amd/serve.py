@@ -16,12 +16,32 @@ mix" holds exactly, not only within a tolerance.
 Reported per request: the final state, the latency of each of its steps (the
 wall time of the tick that advanced it) and its queueing delay; per run:
 aggregate vertex-steps/s and p50/p99 step latency.
+
+Wire format (JSON lines, `python -m cloth_sim_self_collision_amd.serve`):
+
+  request  {"id": any JSON value, "scene": a scene configuration
+            (scene.from_config; one cloth), "steps": int >= 1,
+            "arrival": seconds after the start (optional, default 0),
+            "precision": "f64" (optional; the only precision served),
+            "return": subset of ["x", "v"] (optional, default ["x"])}
+  result   {"id", "status": "ok", "vertices", "triangles", "steps",
+            "arrival", "queued_s", "admitted_s", "finished_s",
+            "step_ms": [...], "pcg_iters": [...], "impulse_applied": [...],
+            "n_prox": [...], "n_ccd": [[...] per step],
+            "x": {"dtype": "<f8", "shape": [np, 3], "data": base64}, "v": ...}
+           or {"id", "status": "error", "error": message} for a request
+           that cannot be served (bad record, unknown scene, precision).
+
+Results are written as requests finish, one line each, in completion order.
 """
 from __future__ import annotations
 
+import base64
 import dataclasses
+import json
+import sys
 import time
-from typing import Callable, List, Optional
+from typing import Callable, Iterable, List, Optional, TextIO
 
 import numpy as np
 
@@ -36,10 +56,11 @@ BYTES_PER_VERTEX_ESTIMATE = 8192
 
 @dataclasses.dataclass
 class Request:
-    rid: int
+    rid: object  # any JSON value on the wire
     cloth: Cloth
     steps: int
     arrival: float = 0.0  # seconds after the start of the run
+    returns: tuple = ("x",)  # arrays of the final state in the wire result
 
 
 @dataclasses.dataclass
@@ -53,6 +74,9 @@ class Result:
     finished_s: float
     vertices: int
     pcg_iters: List[int]
+    impulse_applied: List[bool] = dataclasses.field(default_factory=list)
+    n_prox: List[int] = dataclasses.field(default_factory=list)
+    n_ccd: List[list] = dataclasses.field(default_factory=list)
 
 
 @dataclasses.dataclass
@@ -62,6 +86,9 @@ class _Active:
     done: int = 0
     step_ms: List[float] = dataclasses.field(default_factory=list)
     pcg_iters: List[int] = dataclasses.field(default_factory=list)
+    impulse_applied: List[bool] = dataclasses.field(default_factory=list)
+    n_prox: List[int] = dataclasses.field(default_factory=list)
+    n_ccd: List[list] = dataclasses.field(default_factory=list)
     admitted_s: float = 0.0
     memory: int = 0
 
@@ -69,9 +96,11 @@ class _Active:
 class Server:
     """Continuous-batching tick loop over resident simulations."""
 
-    def __init__(self, hbm_budget_bytes: int, clock: Callable[[], float] = time.perf_counter):
+    def __init__(self, hbm_budget_bytes: int, clock: Callable[[], float] = time.perf_counter,
+                 on_result: Optional[Callable[["Result"], None]] = None):
         self.budget = int(hbm_budget_bytes)
         self.clock = clock
+        self.on_result = on_result  # called as each request finishes
 
     def run(self, requests: List[Request], *, realtime: bool = True) -> dict:
         """Serve `requests` (arrival times in seconds from now).  realtime:
@@ -122,13 +151,20 @@ class Server:
                     a.done += 1
                     a.step_ms.append(dt_ms)
                     a.pcg_iters.append(st["pcg_iters"])
+                    a.impulse_applied.append(bool(st["impulse_applied"]))
+                    a.n_prox.append(int(st["n_prox"]))
+                    a.n_ccd.append([int(c) for c in st["n_ccd"][:st["n_ccd_pass"]]])
                     if a.done >= a.req.steps:
                         x, v = a.dev.get_state()
                         results.append(Result(
                             rid=a.req.rid, xyz=x, uvw=v, step_ms=a.step_ms,
                             queued_s=a.admitted_s - a.req.arrival if realtime else 0.0,
                             admitted_s=a.admitted_s, finished_s=self.clock() - t0,
-                            vertices=a.req.cloth.np_, pcg_iters=a.pcg_iters))
+                            vertices=a.req.cloth.np_, pcg_iters=a.pcg_iters,
+                            impulse_applied=a.impulse_applied, n_prox=a.n_prox,
+                            n_ccd=a.n_ccd))
+                        if self.on_result is not None:
+                            self.on_result(results[-1])
                         resident -= a.memory
                         a.dev.close()
                     else:
@@ -203,3 +239,133 @@ def poisson_stream(count: int, rate_per_s: float, *, seed: int = 0, tri_range=(1
                                     name=f"req{k}_{nx}x{ny}")
         out.append(Request(rid=k, cloth=cl, steps=steps, arrival=t))
     return out
+
+
+# ---------------------------------------------------------------------------
+# wire format
+
+PRECISIONS = ("f64",)
+
+
+def encode_array(a: np.ndarray) -> dict:
+    """A float64 array as {"dtype": "<f8", "shape", "data": base64 of its bytes}."""
+    a = np.ascontiguousarray(a, dtype="<f8")
+    return {"dtype": "<f8", "shape": list(a.shape), "data": base64.b64encode(a.tobytes()).decode()}
+
+
+def decode_array(obj: dict) -> np.ndarray:
+    if obj.get("dtype") != "<f8":
+        raise ValueError("only <f8 arrays")
+    a = np.frombuffer(base64.b64decode(obj["data"]), dtype="<f8")
+    return a.reshape(obj["shape"]).copy()
+
+
+def parse_request(obj: dict, index: int = 0) -> Request:
+    """A wire request (see the module docstring) as a Request; ValueError
+    with the reason if it cannot be served."""
+    if not isinstance(obj, dict):
+        raise ValueError("a request is a JSON object")
+    rid = obj.get("id", index)
+    if "scene" not in obj:
+        raise ValueError("missing 'scene'")
+    steps = obj.get("steps")
+    if not isinstance(steps, int) or isinstance(steps, bool) or steps < 1:
+        raise ValueError("'steps' must be an integer >= 1")
+    prec = obj.get("precision", "f64")
+    if prec not in PRECISIONS:
+        raise ValueError(f"precision {prec!r} is not served (served: {', '.join(PRECISIONS)})")
+    arrival = float(obj.get("arrival", 0.0))
+    if not arrival >= 0.0:
+        raise ValueError("'arrival' must be >= 0")
+    returns = tuple(obj.get("return", ["x"]))
+    if not set(returns) <= {"x", "v"}:
+        raise ValueError("'return' takes 'x' and/or 'v'")
+    cloth = scene.from_config(obj["scene"])
+    if isinstance(cloth, list):
+        raise ValueError("a request carries one cloth (use one request per cloth)")
+    return Request(rid=rid, cloth=cloth, steps=steps, arrival=arrival, returns=returns)
+
+
+def result_record(r: Result, req: Optional[Request] = None) -> dict:
+    out = {"id": r.rid, "status": "ok", "vertices": r.vertices,
+           "triangles": req.cloth.nt if req is not None else None, "steps": len(r.step_ms),
+           "arrival": req.arrival if req is not None else None,
+           "queued_s": r.queued_s, "admitted_s": r.admitted_s, "finished_s": r.finished_s,
+           "step_ms": r.step_ms, "pcg_iters": r.pcg_iters,
+           "impulse_applied": r.impulse_applied, "n_prox": r.n_prox, "n_ccd": r.n_ccd}
+    returns = req.returns if req is not None else ("x",)
+    if "x" in returns:
+        out["x"] = encode_array(r.xyz.reshape(-1, 3))
+    if "v" in returns:
+        out["v"] = encode_array(r.uvw.reshape(-1, 3))
+    return out
+
+
+def error_record(rid, message: str) -> dict:
+    return {"id": rid, "status": "error", "error": message}
+
+
+def serve_jsonl(lines: Iterable[str], out: TextIO, *, hbm_budget_bytes: int,
+                realtime: bool = True) -> dict:
+    """Serve the JSON-lines requests in `lines`, writing one result line per
+    request to `out` as each finishes (error lines first, for requests that
+    cannot be served).  Returns the run summary of Server.run."""
+    reqs, by_rid = [], {}
+    for k, line in enumerate(lines):
+        line = line.strip()
+        if not line:
+            continue
+        rid = k
+        try:
+            obj = json.loads(line)
+            rid = obj.get("id", k) if isinstance(obj, dict) else k
+            req = parse_request(obj, k)
+        except (ValueError, KeyError, TypeError) as e:
+            out.write(json.dumps(error_record(rid, str(e))) + "\n")
+            out.flush()
+            continue
+        key = len(reqs)  # internal id: wire ids need not be unique or hashable
+        by_rid[key] = req
+        reqs.append(dataclasses.replace(req, rid=key))
+
+    def emit(r: Result):
+        req = by_rid[r.rid]
+        rec = result_record(dataclasses.replace(r, rid=req.rid), req)
+        out.write(json.dumps(rec) + "\n")
+        out.flush()
+
+    if not reqs:
+        return {"results": [], "wall_s": 0.0, "ticks": 0, "vertex_steps_per_s": 0.0,
+                "p50_step_ms": 0.0, "p99_step_ms": 0.0}
+    return Server(hbm_budget_bytes, on_result=emit).run(reqs, realtime=realtime)
+
+
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(description="Serve cloth simulations on one GPU (JSON lines).")
+    ap.add_argument("--requests", default="-", help="request lines (default: stdin)")
+    ap.add_argument("--results", default="-", help="result lines (default: stdout)")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="HBM budget for resident cloths")
+    ap.add_argument("--no-realtime", action="store_true",
+                    help="admit every request at once (ignore arrival times)")
+    ap.add_argument("--setup-cache", default=None, help="directory of cached per-mesh setups")
+    args = ap.parse_args(argv)
+    if args.setup_cache:
+        from . import _lib
+        _lib.set_setup_cache(args.setup_cache)
+    fin = sys.stdin if args.requests == "-" else open(args.requests)
+    fout = sys.stdout if args.results == "-" else open(args.results, "w")
+    try:
+        summary = serve_jsonl(fin, fout, hbm_budget_bytes=int(args.budget_gb * 2**30),
+                              realtime=not args.no_realtime)
+    finally:
+        if fin is not sys.stdin:
+            fin.close()
+        if fout is not sys.stdout:
+            fout.close()
+    print(json.dumps({k: v for k, v in summary.items() if k != "results"}), file=sys.stderr)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

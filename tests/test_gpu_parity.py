@@ -346,3 +346,38 @@ def test_cloth_from_setup_cache_steps_identically(sim_mod, tmp_path):
     finally:
         _lib.set_setup_cache(None)
     assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+def test_wire_served_results_equal_alone(sim_mod):
+    """Serving over the JSON-lines wire format: each decoded final state equals
+    stepping that request's cloth alone, bit for bit (a swept fold, a bar over
+    the ground, a drape); an unservable request gets an error record."""
+    import io
+    import json
+    from cloth_sim_self_collision_amd import serve
+    bar = {"kind": "bar_over_ground", "a": [-1, 0, 0], "b": [1, 0, 0], "radius": 0.05,
+           "ground_z": -0.3}
+    reqs = [
+        {"id": "drape", "scene": {"scene": "drape", "n": 16}, "steps": 12, "return": ["x", "v"]},
+        {"id": 7, "scene": {"scene": "drop", "n": 14, "height": 0.3, "seed": 1, "obstacle": bar},
+         "steps": 9, "arrival": 0.02},
+        {"id": "bad", "scene": {"scene": "drape", "n": 16}, "steps": 4, "precision": "bf16"},
+        {"id": "fold", "scene": {"scene": "fold", "n": 20, "size": 0.3}, "steps": 6},
+    ]
+    out = io.StringIO()
+    serve.serve_jsonl([json.dumps(r) for r in reqs], out, hbm_budget_bytes=4 << 30,
+                      realtime=False)
+    recs = {json.dumps(r["id"]): r for r in map(json.loads, out.getvalue().splitlines())}
+    assert len(recs) == 4 and recs['"bad"']["status"] == "error"
+    for r in reqs:
+        if r["id"] == "bad":
+            continue
+        rec = recs[json.dumps(r["id"])]
+        assert rec["status"] == "ok" and len(rec["step_ms"]) == r["steps"] == rec["steps"]
+        cl = scene.from_config(r["scene"])
+        with sim_mod.DeviceCloth(cl) as d:
+            d.step(r["steps"])
+            x, v = d.get_state()
+        assert np.array_equal(serve.decode_array(rec["x"]).reshape(-1), x), r["id"]
+        if "v" in r.get("return", []):
+            assert np.array_equal(serve.decode_array(rec["v"]).reshape(-1), v), r["id"]
