@@ -42,6 +42,12 @@ TRAFFIC_BYTES: dict = {
 
 
 def workload(name: str) -> scene.Cloth:
+    """fold256 (default), foldN, drapeN, or a scene configuration file (.json)."""
+    if name.endswith(".json"):
+        cl = scene.load_config(name)
+        if isinstance(cl, list):
+            raise SystemExit("the bench steps one cloth; the batch workload is --batch-count")
+        return cl
     if name == "fold256":
         return scene.fold_sweep(256)
     if name.startswith("fold"):

@@ -212,9 +212,9 @@ def run_plain(requests: List[Request]) -> dict:
             "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99))}
 
 
-def poisson_stream(count: int, rate_per_s: float, *, seed: int = 0, tri_range=(1.0e3, 2.0e5),
-                   aspect_range=(1.0, 4.0), step_range=(50, 300), pcg_maxit: int = 1000) -> List[Request]:
-    """Requests of the serving workload: Poisson arrivals at `rate_per_s`;
+def poisson_requests(count: int, rate_per_s: float, *, seed: int = 0, tri_range=(1.0e3, 2.0e5),
+                     aspect_range=(1.0, 4.0), step_range=(50, 300), pcg_maxit: int = 1000) -> list:
+    """The serving workload as wire requests: Poisson arrivals at `rate_per_s`;
     rectangular grids whose triangle count is log-uniform in `tri_range` and
     aspect ratio uniform in `aspect_range`; steps uniform in `step_range`;
     each dropped from a height in [0.3, 0.8] m onto its own sphere or box,
@@ -232,13 +232,18 @@ def poisson_stream(count: int, rate_per_s: float, *, seed: int = 0, tri_range=(1
         nx = max(3, int(round(asp * (ny - 1))) + 1)
         steps = step_range[0] + int(rng.uniform() * (step_range[1] - step_range[0] + 1))
         h = 0.3 + 0.5 * rng.uniform()
-        ob = scene.sphere((0.0, 0.0, 0.0), 0.3) if rng.uniform() < 0.5 else \
-            scene.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25))
+        ob = ({"kind": "sphere", "center": [0.0, 0.0, 0.0], "radius": 0.3} if rng.uniform() < 0.5
+              else {"kind": "box", "center": [0.0, 0.0, 0.0], "half": [0.25, 0.25, 0.25]})
         js = int(rng.next_u64() & 0x7FFFFFFF)
-        cl = scene.drop_on_obstacle(nx, h, ob, ny=ny, seed=js, pcg_maxit=pcg_maxit,
-                                    name=f"req{k}_{nx}x{ny}")
-        out.append(Request(rid=k, cloth=cl, steps=steps, arrival=t))
+        out.append({"id": k, "steps": steps, "arrival": t,
+                    "scene": {"scene": "drop", "n": nx, "ny": ny, "height": h, "obstacle": ob,
+                              "seed": js, "pcg_maxit": pcg_maxit, "name": f"req{k}_{nx}x{ny}"}})
     return out
+
+
+def poisson_stream(count: int, rate_per_s: float, **kw) -> List[Request]:
+    """poisson_requests parsed into Requests."""
+    return [parse_request(r, k) for k, r in enumerate(poisson_requests(count, rate_per_s, **kw))]
 
 
 # ---------------------------------------------------------------------------
@@ -349,7 +354,18 @@ def main(argv=None) -> int:
     ap.add_argument("--no-realtime", action="store_true",
                     help="admit every request at once (ignore arrival times)")
     ap.add_argument("--setup-cache", default=None, help="directory of cached per-mesh setups")
+    ap.add_argument("--make-poisson", type=int, default=0, metavar="COUNT",
+                    help="write COUNT requests of the Poisson serving workload and exit")
+    ap.add_argument("--rate", type=float, default=4.0, help="--make-poisson arrival rate, 1/s")
+    ap.add_argument("--seed", type=int, default=0, help="--make-poisson seed")
     args = ap.parse_args(argv)
+    if args.make_poisson > 0:
+        fout = sys.stdout if args.results == "-" else open(args.results, "w")
+        for r in poisson_requests(args.make_poisson, args.rate, seed=args.seed):
+            fout.write(json.dumps(r) + "\n")
+        if fout is not sys.stdout:
+            fout.close()
+        return 0
     if args.setup_cache:
         from . import _lib
         _lib.set_setup_cache(args.setup_cache)
