@@ -223,6 +223,9 @@ int build(Cloth* c, const cs_mesh* m) {
   int *qfp, *qbp;
   A.reserve(&qfp, S.qf_ptr.size());
   A.reserve(&qbp, S.qb_ptr.size());
+  int *qfc, *qbc;
+  A.reserve(&qfc, S.qf_comp.size());
+  A.reserve(&qbc, S.qb_comp.size());
   double* host_pd;
   A.reserve(&host_pd, 4 * (size_t)np);
   if (int st = A.commit()) return st;
@@ -264,8 +267,12 @@ int build(Cloth* c, const cs_mesh* m) {
   d.ising = c->ising;
   d.qf_ptr = qfp;
   d.qb_ptr = qbp;
+  d.qf_comp = qfc;
+  d.qb_comp = qbc;
   d.nqf = S.nqf;
   d.nqb = S.nqb;
+  d.ncomp = S.ncomp;
+  d.qwidth = S.qwidth;
   hipStream_t s = c->stream;
   CS_HIP(hipMemsetAsync(d.sum.ticket, 0, sizeof(unsigned int), s));
   CS_HIP(hipMemsetAsync(d.sum2.ticket, 0, sizeof(unsigned int), s));
@@ -300,6 +307,8 @@ int build(Cloth* c, const cs_mesh* m) {
   if (int st = upload(rord, S.refit_order, s)) return st;
   if (S.kl > 0) {
     if (int st = upload(qfp, S.qf_ptr, s)) return st;
+    if (int st = upload(qfc, S.qf_comp, s)) return st;
+    if (int st = upload(qbc, S.qb_comp, s)) return st;
     if (int st = upload(qbp, S.qb_ptr, s)) return st;
     if (int st = upload(frc, S.f_rci, s)) return st;
     if (int st = upload(fsl, S.fslot, s)) return st;
@@ -935,6 +944,8 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
     v.max_nq = std::max(v.max_nq, d.nq);
     v.max_nnz = std::max(v.max_nnz, d.nnz);
     v.max_wmax = std::max(v.max_wmax, d.wmax);
+    v.max_qwidth = std::max(v.max_qwidth, d.qwidth);
+    v.max_ncomp = std::max(v.max_ncomp, d.ncomp);
     v.piped = v.piped && quad_ok(d) && (v.kpack == 0 || v.kpack == d.kpack);
     v.kpack = d.kpack;
   }

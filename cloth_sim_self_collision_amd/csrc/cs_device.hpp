@@ -83,7 +83,11 @@ struct DevCloth {
   const int* lb_rows;
   const int* qf_ptr;  // sub-levels of at most kQuadRows slots
   const int* qb_ptr;
+  const int* qf_comp;  // each component's first sub-level (ncomp + 1 entries)
+  const int* qb_comp;
   int nqf, nqb;
+  int ncomp;   // connected components (one sweep workgroup each)
+  int qwidth;  // rows of the widest sub-level
   const int* crs_src_ptr;
   const int* crs_src;
   const int* dia_src_ptr;

@@ -2,6 +2,7 @@
 #include "cs_setup.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -283,13 +284,55 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
     s.nlb = nb;
     levels_to_lists(lf, nf, s.lf_ptr, s.lf_rows, &s.max_lf);
     levels_to_lists(lb, nb, s.lb_ptr, s.lb_rows, &s.max_lb);
-    auto split = [](const std::vector<int>& ptr, int nl, std::vector<int>& q) {
-      q.assign(1, 0);
-      for (int l = 0; l < nl; l++)
-        for (int b = ptr[l]; b < ptr[l + 1]; b += kQuadRows) q.push_back(std::min(b + kQuadRows, ptr[l + 1]));
+    // components (union-find over the pattern), numbered by smallest row
+    std::vector<int> up(np);
+    for (int i = 0; i < np; i++) up[i] = i;
+    auto find = [&up](int i) {
+      while (up[i] != i) {
+        up[i] = up[up[i]];
+        i = up[i];
+      }
+      return i;
     };
-    split(s.lf_ptr, s.nlf, s.qf_ptr);
-    split(s.lb_ptr, s.nlb, s.qb_ptr);
+    for (int i = 0; i < np; i++)
+      for (int k = s.rowptr[i]; k < s.rowptr[i + 1]; k++) {
+        const int a = find(i), b = find(s.colind[k]);
+        if (a != b) up[std::max(a, b)] = std::min(a, b);
+      }
+    s.comp.assign(np, -1);
+    std::vector<int> id_of_root(np, -1);
+    s.ncomp = 0;
+    for (int i = 0; i < np; i++) {
+      const int r = find(i);
+      if (id_of_root[r] < 0) id_of_root[r] = s.ncomp++;
+      s.comp[i] = id_of_root[r];
+    }
+    // quad sweep slot orders: the level lists regrouped by component (stable,
+    // so each component keeps level order and row order within a level)
+    auto by_comp = [&](const std::vector<int>& rows, const std::vector<int>& lvl,
+                       std::vector<int>& out, std::vector<int>& q, std::vector<int>& qc) {
+      out.assign(np, 0);
+      std::vector<int> start(s.ncomp + 1, 0);
+      for (int i = 0; i < np; i++) start[s.comp[i] + 1]++;
+      for (int c = 0; c < s.ncomp; c++) start[c + 1] += start[c];
+      std::vector<int> fill(start.begin(), start.end() - 1);
+      for (int k = 0; k < np; k++) out[fill[s.comp[rows[k]]]++] = rows[k];
+      q.assign(1, 0);
+      qc.assign(1, 0);
+      for (int c = 0; c < s.ncomp; c++) {
+        for (int b = start[c]; b < start[c + 1];) {
+          int e = b + 1;  // the rest of this level in this component, at most kQuadRows rows
+          while (e < start[c + 1] && e - b < kQuadRows && lvl[out[e]] == lvl[out[b]]) e++;
+          q.push_back(e);
+          s.qwidth = std::max(s.qwidth, e - b);
+          b = e;
+        }
+        qc.push_back((int)q.size() - 1);
+      }
+    };
+    s.qwidth = 0;
+    by_comp(s.lf_rows, lf, s.sf_rows, s.qf_ptr, s.qf_comp);
+    by_comp(s.lb_rows, lb, s.sb_rows, s.qb_ptr, s.qb_comp);
     s.nqf = (int)s.qf_ptr.size() - 1;
     s.nqb = (int)s.qb_ptr.size() - 1;
   }
@@ -306,8 +349,8 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
       s.fslot.assign(np, 0);
       std::vector<int> bslot(np);
       for (int sl = 0; sl < np; sl++) {
-        s.fslot[s.lf_rows[sl]] = sl;
-        bslot[s.lb_rows[sl]] = sl;
+        s.fslot[s.sf_rows[sl]] = sl;
+        bslot[s.sb_rows[sl]] = sl;
       }
       const int kq = std::max(std::max(s.kl, s.ku), 1);  // both sweeps padded alike
       const size_t mw = (size_t)meta_groups(kq) * 8;    // metadata words per slot
@@ -324,7 +367,7 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
         m[midx(kq + 1, sl)] = (unsigned short)(v >> 16);
       };
       for (int sl = 0; sl < np; sl++) {
-        const int i = s.lf_rows[sl];
+        const int i = s.sf_rows[sl];
         const int fc = s.diaind[i] - s.rowptr[i];
         s.f_rci[4 * sl] = i;
         s.f_rci[4 * sl + 1] = fc;
@@ -332,7 +375,7 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
         for (int k = 0; k < kq; k++)
           s.f_meta[midx(k, sl)] = k < fc ? ring_byte(s.fslot[s.colind[s.rowptr[i] + k]]) : zero_byte;
         put32(s.f_meta, sl, (unsigned)bslot[i]);
-        const int j = s.lb_rows[sl];
+        const int j = s.sb_rows[sl];
         const int bc = s.rowptr[j + 1] - s.diaind[j];
         s.b_rc[2 * sl] = j;
         s.b_rc[2 * sl + 1] = bc;
@@ -341,9 +384,11 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
         put32(s.b_meta, sl, (unsigned)j);
       }
       // live interval of a row in a sweep: [level written, last level read];
-      // rows whose sweep slots share a ring position must have disjoint
-      // intervals.  Slots run in level order, so the ring is a FIFO whose
-      // window only has to cover the rows live across a few levels.
+      // rows of one component whose sweep slots share a ring position must
+      // have disjoint intervals (components are swept by separate workgroups,
+      // each with its own ring).  Slots run in level order within a
+      // component, so the ring is a FIFO whose window only has to cover the
+      // rows live across a few levels.
       auto check = [&](const std::vector<int>& lvl, bool lower, const std::vector<int>& slot_of) {
         std::vector<int> last(np);
         for (int i = 0; i < np; i++) last[i] = lvl[i];
@@ -352,13 +397,14 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
           const int e = lower ? s.diaind[i] : s.rowptr[i + 1];
           for (int k = b; k < e; k++) last[s.colind[k]] = std::max(last[s.colind[k]], lvl[i]);
         }
-        std::vector<std::vector<std::pair<int, int>>> slot(kRingRows);
-        for (int i = 0; i < np; i++) slot[slot_of[i] & (kRingRows - 1)].push_back({lvl[i], last[i]});
-        for (auto& v : slot) {
-          std::sort(v.begin(), v.end());
-          for (size_t q = 1; q < v.size(); q++)
-            if (v[q].first <= v[q - 1].second) return false;
-        }
+        // (component, ring entry, first level, last level) per row
+        std::vector<std::array<int, 4>> iv(np);
+        for (int i = 0; i < np; i++)
+          iv[i] = {s.comp[i], slot_of[i] & (kRingRows - 1), lvl[i], last[i]};
+        std::sort(iv.begin(), iv.end());
+        for (int q = 1; q < np; q++)
+          if (iv[q][0] == iv[q - 1][0] && iv[q][1] == iv[q - 1][1] && iv[q][2] <= iv[q - 1][3])
+            return false;
         return true;
       };
       std::vector<int> lf(np), lb(np);

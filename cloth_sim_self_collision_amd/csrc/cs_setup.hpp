@@ -23,11 +23,20 @@ struct MeshSetup {
   // strictly lower part) and backward substitution (strictly upper part)
   std::vector<int> lf_ptr, lf_rows, lb_ptr, lb_rows;
   int nlf = 0, nlb = 0, max_lf = 0, max_lb = 0;
-  // the same slot orders cut into sub-levels of at most kQuadRows rows (the
-  // rows of a level are independent, so running a level in pieces keeps every
-  // dependency satisfied)
-  std::vector<int> qf_ptr, qb_ptr;
+  // connected components of the pattern (rows joined by a block), numbered
+  // in order of their smallest row; the matrix is block diagonal over them,
+  // so each component's sweeps are independent of the others'
+  int ncomp = 0;
+  std::vector<int> comp;  // component of each row
+  // quad sweep slot orders: rows by (component, level, row) for the forward
+  // and the backward sweep, cut into sub-levels of at most kQuadRows rows of
+  // one level of one component (the rows of a level are independent, so
+  // running a level in pieces keeps every dependency satisfied);
+  // qf_comp[c] .. qf_comp[c + 1] are component c's sub-levels (same backward)
+  std::vector<int> sf_rows, sb_rows;
+  std::vector<int> qf_ptr, qb_ptr, qf_comp, qb_comp;
   int nqf = 0, nqb = 0;
+  int qwidth = 0;  // rows of the widest sub-level
   // packed sweeps: rows in level order ("slots"), at most kl lower / ku upper
   // blocks per row.  f_rci[4*slot] = {row, block count, the row's backward
   // slot, 0}, b_rc[2*slot] = {row, block count}, fslot[row] = forward slot.
@@ -42,7 +51,8 @@ struct MeshSetup {
   std::vector<int> f_rci, b_rc, fslot;
   std::vector<unsigned short> f_meta, b_meta;
   // the LDS ring of kRingRows entries (entry = slot mod kRingRows) can hold the
-  // sweep dependencies: no two rows sharing an entry are live at the same level
+  // sweep dependencies: no two rows of one component sharing an entry are
+  // live at the same level (each component is swept by its own workgroup)
   bool ring_ok = false;
   // gather lists in element order (assembly without atomics, in the
   // reference's summation order)

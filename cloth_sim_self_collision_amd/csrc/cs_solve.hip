@@ -485,16 +485,16 @@ __device__ __forceinline__ T ld32(const T* base, unsigned i) {
 // on one path only (even a wave-uniform skip) make the compiler wait for every
 // outstanding load where the paths merge, defeating the prefetch.
 template <bool kFwd, int K>
-__device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int* qptr, int nq,
+__device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int* qptr, int q1,
                                           int l, QuadRow& q) {
   const int lane = threadIdx.x & 3, qi = threadIdx.x >> 2;
-  const int lc = l < nq ? l : 0;
+  const int lc = l < q1 ? l : 0;  // past the last sub-level: slot 0, marked empty
   const int first = qptr[lc], end = qptr[lc + 1];
   const int s0 = first + qi;
-  const bool valid = l < nq && s0 < end;
+  const bool valid = l < q1 && s0 < end;
   // the quads of a wave are consecutive: the wave has a row iff its first does
   q.live = __builtin_amdgcn_readfirstlane(
-               (int)(l < nq && first + (int)(threadIdx.x >> 6) * 16 < end)) != 0;
+               (int)(l < q1 && first + (int)(threadIdx.x >> 6) * 16 < end)) != 0;
   const unsigned sl = valid ? (unsigned)s0 : 0u;
   q.s = (int)sl;
   q.has = valid;
@@ -573,74 +573,84 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingLen], double* o
 // one sweep over the sub-levels; three register sets rotate so that the
 // loads of sub-levels l + 1 and l + 2 are in flight while l is computed
 template <bool kFwd, int K>
-__device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, int nq,
+// over the sub-levels q0 .. q1 - 1 (one component's)
+__device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, int q0, int q1,
                                            const double* rhs, double* out,
                                            double (*ring)[kRingLen]) {
   const QuadSrc<kFwd, K> src = quad_src<kFwd, K>(c, rhs);
   QuadRow qa, qb, qc;
-  quad_load<kFwd, K>(src, qptr, nq, 0, qa);
-  quad_load<kFwd, K>(src, qptr, nq, 1, qb);
-  quad_load<kFwd, K>(src, qptr, nq, 2, qc);
+  quad_load<kFwd, K>(src, qptr, q1, q0, qa);
+  quad_load<kFwd, K>(src, qptr, q1, q0 + 1, qb);
+  quad_load<kFwd, K>(src, qptr, q1, q0 + 2, qc);
   // all three thirds unconditional (a count that is not a multiple of three
   // ends with empty sub-levels): a skipped third would change which loads are
   // the newest on one path, and the compiler would then wait for all loads at
   // the top of every iteration
   // a set is reloaded right after its compute, before the barrier: waves
   // that finish computing early issue their loads while others still compute
-  for (int l = 0; l < nq; l += 3) {
+  for (int l = q0; l < q1; l += 3) {
     quad_compute<kFwd, K>(ring, out, qa);
-    quad_load<kFwd, K>(src, qptr, nq, l + 3, qa);
+    quad_load<kFwd, K>(src, qptr, q1, l + 3, qa);
     lds_barrier();
     quad_compute<kFwd, K>(ring, out, qb);
-    quad_load<kFwd, K>(src, qptr, nq, l + 4, qb);
+    quad_load<kFwd, K>(src, qptr, q1, l + 4, qb);
     lds_barrier();
     quad_compute<kFwd, K>(ring, out, qc);
-    quad_load<kFwd, K>(src, qptr, nq, l + 5, qc);
+    quad_load<kFwd, K>(src, qptr, q1, l + 5, qc);
     lds_barrier();
   }
 }
 
 // M^-1 (L+D)(I+D^-1 U) solve of the factor: forward sweep from the
 // right-hand side in forward-slot order (fwd_rhs_index) into c.sb, backward
-// sweep into dst
-// (row order); prod = src_row .* dst when prod is given
+// sweep into dst (row order); prod = src_row .* dst when prod is given.
+// One workgroup per connected component (blockIdx.x): the matrix is block
+// diagonal over them, so their sweeps are independent.
 // K: the packed blocks per row (DevCloth::kpack)
 template <int K, class Sel>
 __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const double* src_row,
                                                                  double* dst, double* prod) {
   const DevCloth c = sel.cloth();
-  if (c.flags[kDone]) return;
+  const int comp = blockIdx.x;
+  if (c.flags[kDone] || comp >= c.ncomp) return;  // batch grids fit the most components
   if (!src_row) {
     src_row = c.r;
     dst = c.z;
     prod = c.prod;
   }
   __shared__ double ring[3][kRingLen];  // + the dummy and the zero entry
-  __shared__ int qfp[kQuadLevels + 1];
-  __shared__ int qbp[kQuadLevels + 1];
-  for (int l = threadIdx.x; l <= c.nqf; l += blockDim.x) qfp[l] = c.qf_ptr[l];
-  for (int l = threadIdx.x; l <= c.nqb; l += blockDim.x) qbp[l] = c.qb_ptr[l];
   if (threadIdx.x < 3) ring[threadIdx.x][kRingZero] = 0.0;  // never written again
   __syncthreads();
-  quad_sweep<true, K>(c, qfp, c.nqf, nullptr, c.sb, ring);
-  __syncthreads();  // the backward sweep reads the complete forward result
-  quad_sweep<false, K>(c, qbp, c.nqb, c.sb, dst, ring);
+  // sub-level cuts are read at wave-uniform indices: scalar loads from global
+  // memory, no LDS copy and no limit on the number of sub-levels
+  quad_sweep<true, K>(c, c.qf_ptr, c.qf_comp[comp], c.qf_comp[comp + 1], nullptr, c.sb, ring);
+  __syncthreads();  // the backward sweep reads the component's complete forward result
+  quad_sweep<false, K>(c, c.qb_ptr, c.qb_comp[comp], c.qb_comp[comp + 1], c.sb, dst, ring);
   __syncthreads();
-  if (prod)
+  if (!prod) return;
+  if (c.ncomp == 1) {
     for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src_row[e] * dst[e];
+  } else {  // this component's rows: its forward slots
+    const int s0 = c.qf_ptr[c.qf_comp[comp]], s1 = c.qf_ptr[c.qf_comp[comp + 1]];
+    for (int k = threadIdx.x; k < 3 * (s1 - s0); k += blockDim.x) {
+      const int sl = s0 + k / 3;
+      const int e = 3 * c.f_rci[4 * sl] + (k - 3 * (k / 3));
+      prod[e] = src_row[e] * dst[e];
+    }
+  }
 }
 
 // threads of a quad sweep: four per row of the widest sub-level (at most
 // kQuadRows rows), in whole waves; waves past it would only load and wait
-inline int quad_threads(int wmax) {
-  const int rows = std::min(std::max(wmax, 1), kQuadRows);
+inline int quad_threads(int qwidth) {
+  const int rows = std::min(std::max(qwidth, 1), kQuadRows);
   return 4 * ((rows + 15) / 16 * 16);
 }
 
 template <class Sel>
-static void launch_quad(int kq, int wmax, dim3 grid, const Sel& sel, const double* src,
+static void launch_quad(int kq, int qwidth, dim3 grid, const Sel& sel, const double* src,
                         double* dst, double* prod, hipStream_t s) {
-  const dim3 tb(quad_threads(wmax));
+  const dim3 tb(quad_threads(qwidth));
   switch (kq) {
     case 1: hipLaunchKernelGGL((k_ilu_apply_quad<1, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
     case 2: hipLaunchKernelGGL((k_ilu_apply_quad<2, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
@@ -663,7 +673,7 @@ static void launch_sweep(const DevCloth& c, const double* src, double* dst, doub
   if (quad_ok(c) && c.sweep_variant != 1) {
     if (!rhs_ready)
       hipLaunchKernelGGL(k_gather_fslots<One>, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, one, src);
-    launch_quad(c.kpack, c.wmax, dim3(1), one, src, dst, prod, s);
+    launch_quad(c.kpack, c.qwidth, dim3(c.ncomp), one, src, dst, prod, s);
   } else {
     hipLaunchKernelGGL(k_ilu_apply<One>, dim3(1), dim3(1024), 0, s, one, src, dst, prod);
   }
@@ -1075,7 +1085,7 @@ static void launch_sweep_batch(const BatchView& b, hipStream_t s) {
   const Many m{b.tab, b.ptab};
   if (b.piped) {
     // every cloth packed: r is already in slot order (k_pcg_init / k_update_rx)
-    launch_quad(b.kpack, b.max_wmax, dim3(1, b.n), m, nullptr, nullptr, nullptr, s);
+    launch_quad(b.kpack, b.max_qwidth, dim3(b.max_ncomp, b.n), m, nullptr, nullptr, nullptr, s);
   } else {
     hipLaunchKernelGGL(k_ilu_apply<Many>, dim3(1, b.n), dim3(1024), 0, s, m, nullptr, nullptr, nullptr);
   }

@@ -5,13 +5,12 @@
 
 namespace cs {
 
-constexpr int kQuadLevels = 4096;  // sub-level pointers the quad sweep keeps in LDS
 // the packed (quad) sweep applies: packed factor, LDS ring proven safe,
 // sub-level pointers fit
 // the quad sweep addresses its operands with 32-bit byte offsets (at most
 // 3 * np 16-byte pairs per array)
 inline bool quad_ok(const DevCloth& c) {
-  return c.kpack > 0 && c.ring_ok && c.nqf <= kQuadLevels && c.nqb <= kQuadLevels &&
+  return c.kpack > 0 && c.ring_ok &&
          48LL * c.np < (1LL << 32);
 }
 
@@ -42,6 +41,7 @@ struct BatchView {
   const Params* ptab;
   int n;
   int max_np, max_nt, max_nq, max_nnz, max_wmax;
+  int max_qwidth, max_ncomp;  // widest sweep sub-level, most components of a cloth
   bool piped;  // every cloth can use the packed (quad) sweep ...
   int kpack;   // ... with this many packed blocks per row
 };
