@@ -381,3 +381,34 @@ def test_wire_served_results_equal_alone(sim_mod):
         assert np.array_equal(serve.decode_array(rec["x"]).reshape(-1), x), r["id"]
         if "v" in r.get("return", []):
             assert np.array_equal(serve.decode_array(rec["v"]).reshape(-1), v), r["id"]
+
+
+@pytest.mark.parametrize("name,maxit", [("fold256", 30), ("stack4x512", 3)])
+def test_full_size_configs(sim_mod, oracle_mod, name, maxit):
+    """The benchmark's single cloth (256^2, 65,536 vertices) and four stacked
+    512^2 sheets (1,048,576 vertices, four components, each swept by its own
+    workgroup) at full size, PCG capped so that the oracle finishes in
+    seconds: the internal step (assembly, ILU(0), PCG) bit for bit, and for
+    the 256^2 cloth a whole StepTime with its contact pass."""
+    import os
+    cfg = os.path.join(os.path.dirname(os.path.dirname(__file__)), "cloth_sim_self_collision_amd",
+                       "configs", name + ".json")
+    cl = scene.load_config(cfg)
+    cl.pcg_maxit = maxit
+    o = oracle_mod.Oracle(cl, "port")
+    x0, v0 = cl.xyz.copy(), cl.uvw.copy()
+    scene.apply_sweep(cl, x0, cl.dt)
+    ox, ov, oconv, oit = o.step_internal(x0, v0)
+    with sim_mod.DeviceCloth(cl) as d:
+        d.set_state(x0, v0)
+        conv, it = d.step_internal_dynamics_ilu()
+        gx, gv = d.get_state()
+        assert (it, conv) == (oit, oconv)
+        assert np.array_equal(gx, ox) and np.array_equal(gv, ov)
+    if name == "fold256":
+        x, v, imp, it, conv = o.step(x0, v0)
+        with sim_mod.DeviceCloth(cl) as d:
+            st = d.step(1)
+            gx, gv = d.get_state()
+        assert st["pcg_iters"] == it and st["impulse_applied"] == imp
+        assert np.array_equal(gx, x) and np.array_equal(gv, v)
