@@ -30,8 +30,8 @@
 // the binade walker below.  A wrong guess costs time, never accuracy.
 //
 // Binade walker (wave_walk).  Per step the wave forms the integer prefix for
-// the binade of the current s over the rest of the tile, 256 addends (four
-// per lane) at a time, and finds the first addend where the reasoning above
+// the binade of the current s over the rest of the tile, kWalkChunk addends
+// (kWalkPer per lane) at a time, and finds the first addend where the reasoning above
 // does not apply (tie, out-of-range prefix, huge or non-finite addend, s
 // zero/subnormal/non-finite).  Everything before it is exact integer work.
 // Where binade changes cluster (the running sum swinging through zero under
@@ -51,7 +51,9 @@ namespace cs {
 constexpr int kSumBlock = 256;                  // threads per block
 constexpr int kSumPer = 4;                      // consecutive elements per thread
 constexpr int kSumTile = kSumBlock * kSumPer;   // elements per tile
-constexpr int kSumBuf = kSumTile + 256;         // a tile plus the overreach of the last chunk
+constexpr int kWalkPer = 4;                     // addends per lane in a walker chunk (8 measured slower)
+constexpr int kWalkChunk = 64 * kWalkPer;       // addends per walker chunk
+constexpr int kSumBuf = kSumTile + kWalkChunk;  // a tile plus the overreach of the last chunk
 constexpr int kSumWaves = kSumBlock / 64;
 constexpr int kChainWin = 1024;   // tile summaries staged in LDS at a time
 constexpr int kBurstMin = 32;     // elements a burst always covers
@@ -296,18 +298,22 @@ __device__ inline double wave_walk(const double* buf, int len, double s,
       const double sg = s < 0.0 ? -1.0 : 1.0;
       double S = ldexp(s, scale);
       double Sm = 0.0;
-      for (int c0 = pos & ~3; c0 < len; c0 += 4 * 64) {
-        const int i0 = c0 + 4 * lane;
-        const double2 xa = *reinterpret_cast<const double2*>(buf + i0);
-        const double2 xb = *reinterpret_cast<const double2*>(buf + i0 + 2);
-        const double x[4] = {xa.x, xa.y, xb.x, xb.y};
-        double inc[4];
+      for (int c0 = pos & ~(kWalkPer - 1); c0 < len; c0 += kWalkChunk) {
+        const int i0 = c0 + kWalkPer * lane;
+        double x[kWalkPer];
+#pragma unroll
+        for (int j = 0; j < kWalkPer; j += 2) {
+          const double2 xy = *reinterpret_cast<const double2*>(buf + i0 + j);
+          x[j] = xy.x;
+          x[j + 1] = xy.y;
+        }
+        double inc[kWalkPer];
         double run = 0.0;
         unsigned usem = 0, failm = 0;  // bit j: addend j in [pos, len) / rejected
         // branch-free per addend; |r| >= 2^52 always leaves the binade, so it
         // is rejected with the ties
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < kWalkPer; j++) {
           const int i = i0 + j;
           const bool use = i >= pos && i < len;
           const double q = ldexp(x[j], scale);
@@ -322,18 +328,17 @@ __device__ inline double wave_walk(const double* buf, int len, double s,
         // holding a rejected addend, which is all that is used
         const double off = S + wave_excl_scan_f64(run);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < kWalkPer; j++) {
           const double a = sg * (off + inc[j]);
           failm |= (((usem >> j) & 1u) && (a < lo52 || a > hi53)) ? (1u << j) : 0u;
         }
-        const int jf = failm ? __builtin_ctz(failm) : 4;
-        const unsigned long long bal = __ballot(jf < 4);
+        const int jf = failm ? __builtin_ctz(failm) : kWalkPer;
+        const unsigned long long bal = __ballot(jf < kWalkPer);
         if (bal) {
           const int f = __ffsll((long long)bal) - 1;
           double before = off;
-          before = jf == 1 ? off + inc[0] : before;
-          before = jf == 2 ? off + inc[1] : before;
-          before = jf == 3 ? off + inc[2] : before;
+#pragma unroll
+          for (int j = 1; j < kWalkPer; j++) before = jf == j ? off + inc[j - 1] : before;
           Sm = readlane_f64(before, f);
           m = __builtin_amdgcn_readlane(i0 + jf, f);
           break;
