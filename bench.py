@@ -36,8 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # scaled by what the counter reads for k_stream_copy of known size), keyed by
 # (workload, kernel class); the passes and the scaling are in profiles/
 TRAFFIC_BYTES: dict = {
-    # profiles/fold256_hbm_traffic_pmc.md: (36622.1 * 2.000 + 4608.0 * 1.000) KiB
-    ("fold256", "ilu_sweep"): 79.72e6,
+    # profiles/fold256_hbm_traffic_pmc.md: (34064.5 * 2.000 + 4608.0 * 1.000) KiB
+    ("fold256", "ilu_sweep"): 74.48e6,
 }
 
 
