@@ -193,7 +193,7 @@ def main() -> int:
         print("no HIP device", file=sys.stderr)
         return 1
 
-    from cloth_sim_self_collision_amd.sim import bandwidth_probe, read_probe
+    from cloth_sim_self_collision_amd.sim import bandwidth_probe, issue_probe, read_probe
     n_copy = int(args.copy_mb * 2**20 / 8) // 2 * 2
     copy_gbs = bandwidth_probe(n_copy, 10) if n_copy > 0 else None
     # the one-workgroup sweep streams its factor through one CU: that CU's
@@ -204,6 +204,11 @@ def main() -> int:
         cloth.pcg_maxit = args.maxit
     d = DeviceCloth(cloth)
     d.set_option(1, args.sweep_variant)
+    # the sweep runs on one CU as a chain of dependent sub-levels, each paced by
+    # its waves' vector-load instructions: the issue floor prices them at the
+    # cost one load instruction has in a workgroup of the same shape
+    shape = d.sweep_shape()
+    issue_ns = issue_probe(shape["waves"]) if (shape["quad"] and n_copy > 0) else None
     # warmup; the first step is also compared against the CPU reference
     first_x = None
     for k in range(max(args.warmup, 1)):
@@ -250,6 +255,13 @@ def main() -> int:
                 "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None,
                 "one_cu_read_gbs": one_cu_gbs,
                 "frac_of_one_cu_read": (ach / one_cu_gbs) if (ach and one_cu_gbs) else None}
+        if dom == "ilu_sweep" and issue_ns:
+            loads = shape["waves"] * (shape["fwd_sublevels"] * shape["loads_fwd"]
+                                      + shape["bwd_sublevels"] * shape["loads_bwd"])
+            floor_us = loads * issue_ns * 1e-3
+            roof.update({"sweep_shape": shape, "ns_per_wave_load": issue_ns,
+                         "wave_loads_per_launch": loads, "issue_floor_us": floor_us,
+                         "frac_of_issue_floor": floor_us / (avg_s * 1e6)})
     d.close()
     batch = batch_measure(args.batch_count, args.batch_steps) if args.batch_count > 0 else None
     served = serve_measure(args.serve_count, args.serve_rate) if args.serve_count > 0 else None

@@ -74,6 +74,7 @@ SIGNATURES = {
     "cs_batch_step": (C.c_int, [_V, C.c_int, _V]),
     "cs_bandwidth_probe": (C.c_int, [C.c_longlong, C.c_int, _D]),
     "cs_read_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, _D]),
+    "cs_issue_probe": (C.c_int, [C.c_int, C.c_int, _D]),
     "cs_set_setup_cache": (C.c_int, [C.c_char_p]),
     "cs_setup_cache_stats": (C.c_int, [C.POINTER(C.c_longlong)] * 3),
     "cs_step_internal": (C.c_int, [_V, _D, _I]),
@@ -97,6 +98,7 @@ SIGNATURES = {
     "cs_matvec": (C.c_int, [_V, C.c_double, _D, C.c_double, _D]),
     "cs_pcg_solve": (C.c_int, [_V, _D, C.c_double, C.c_int, _D, _D, _D, _I]),
     "cs_get_bvh": (C.c_int, [_V, _I]),
+    "cs_sweep_shape": (C.c_int, [_V, _I]),
     "cs_bvh_refit_prx": (C.c_int, [_V, _D, C.c_double, _D]),
     "cs_bvh_refit_ccd": (C.c_int, [_V, _D, _D, C.c_double, _D]),
     "cs_query_proximity": (C.c_int, [_V, _D, C.c_double, _I, C.c_int, _I]),

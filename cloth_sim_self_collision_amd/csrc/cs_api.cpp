@@ -815,6 +815,20 @@ int cs_inspect_setup(const cs_mesh* m, int* sizes, int* rowptr, int* colind, int
   return kOk;
 }
 
+int cs_sweep_shape(const cs_cloth* p, int* out) {
+  if (!p || !out) return kBadArg;
+  const Cloth* c = H(p);
+  for (int i = 0; i < 7; i++) out[i] = 0;
+  out[5] = c->d.ncomp;
+  if (!sweep_shape(c->d, &out[2], &out[3], &out[4])) return kOk;
+  for (int k = 0; k < c->S.ncomp; k++) {
+    out[0] = std::max(out[0], c->S.qf_comp[k + 1] - c->S.qf_comp[k]);
+    out[1] = std::max(out[1], c->S.qb_comp[k + 1] - c->S.qb_comp[k]);
+  }
+  out[6] = 1;
+  return kOk;
+}
+
 int cs_get_bvh(const cs_cloth* p, int* nodes) {
   if (!p || !nodes) return kBadArg;
   const Cloth* c = H(p);

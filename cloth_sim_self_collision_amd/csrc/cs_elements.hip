@@ -230,6 +230,108 @@ extern "C" int cs_read_probe(long long n, int reps, int blocks, double* gbs) {
   return kOk;
 }
 
+namespace cs {
+
+// Vector-load issue cost of one workgroup shaped like the ILU sweep: `waves`
+// waves each issue kIssueLoads 16-byte loads per step into one of three
+// register sets in rotation (two steps of loads in flight), quads of lanes
+// reading a row's three components (lane 3 repeats lane 2), an LDS-only
+// barrier per step.  Step s reads block s mod kIssueBlocks of 8 KiB: 64 MiB
+// in all, a factor-sized working set that the first launch brings into the
+// device caches.
+constexpr int kIssueLoads = 12, kIssueBlocks = 8192, kIssueBlock = 512;  // double2 per block
+
+struct IssueSet {
+  double2 v[kIssueLoads];
+};
+
+__device__ __forceinline__ void issue_load(IssueSet& s, const double2* x, unsigned step,
+                                           unsigned off) {
+#pragma unroll
+  for (int k = 0; k < kIssueLoads; k++) {
+    const unsigned blk = (step * kIssueLoads + k) & (kIssueBlocks - 1);
+    s.v[k] = x[blk * kIssueBlock + off];
+  }
+}
+
+__device__ __forceinline__ double issue_use(const IssueSet& s) {
+  double a = 0.0;
+#pragma unroll
+  for (int k = 0; k < kIssueLoads; k++) a += s.v[k].x + s.v[k].y;
+  return a;
+}
+
+__device__ __forceinline__ void issue_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// steps: a multiple of three
+__global__ __launch_bounds__(512) void k_issue_probe(const double2* __restrict__ x, int steps,
+                                                     double* out) {
+  const int q = threadIdx.x >> 2, l = threadIdx.x & 3;
+  const unsigned off = (unsigned)(3 * q + (l < 3 ? l : 2));  // < 3 * 128 <= kIssueBlock
+  IssueSet a, b, c;
+  // the scheduling barriers keep the sets' loads in issue order (a, b, c
+  // before the loop as in it, so the loop needs only the oldest set) and each
+  // set's loads between its use and the step barrier, as in the sweep
+  issue_load(a, x, 0, off);
+  __builtin_amdgcn_sched_barrier(0);
+  issue_load(b, x, 1, off);
+  __builtin_amdgcn_sched_barrier(0);
+  issue_load(c, x, 2, off);
+  __builtin_amdgcn_sched_barrier(0);
+  double acc = 0.0;
+  for (int st = 0; st < steps; st += 3) {
+    acc += issue_use(a);
+    __builtin_amdgcn_sched_barrier(0);
+    issue_load(a, x, st + 3, off);
+    __builtin_amdgcn_sched_barrier(0);
+    issue_barrier();
+    acc += issue_use(b);
+    __builtin_amdgcn_sched_barrier(0);
+    issue_load(b, x, st + 4, off);
+    __builtin_amdgcn_sched_barrier(0);
+    issue_barrier();
+    acc += issue_use(c);
+    __builtin_amdgcn_sched_barrier(0);
+    issue_load(c, x, st + 5, off);
+    __builtin_amdgcn_sched_barrier(0);
+    issue_barrier();
+  }
+  acc += issue_use(a) + issue_use(b) + issue_use(c);
+  if (acc == -1.0) out[threadIdx.x] = acc;  // keeps the loads; never taken for zeros
+}
+
+}  // namespace cs
+
+extern "C" int cs_issue_probe(int waves, int steps, double* ns_per_load) {
+  using namespace cs;
+  if (waves < 1 || waves > 8 || steps < 3 || steps % 3 || !ns_per_load) return kBadArg;
+  Scratch s;
+  double *x, *out;
+  const size_t n = 2 * (size_t)kIssueBlocks * kIssueBlock;  // doubles
+  if (s.get(&x, n) || s.get(&out, 512)) return kHipError;
+  CS_HIP(hipMemset(x, 0, sizeof(double) * n));
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  const dim3 tb(64 * waves);
+  hipLaunchKernelGGL(k_issue_probe, dim3(1), tb, 0, 0, (const double2*)x, steps, out);
+  CS_HIP(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(k_issue_probe, dim3(1), tb, 0, 0, (const double2*)x, steps, out);
+  CS_HIP(hipEventRecord(e1, 0));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CS_HIP(hipGetLastError());
+  *ns_per_load = (double)ms * 1e6 / ((double)steps * waves * kIssueLoads);
+  return kOk;
+}
+
 extern "C" int cs_bandwidth_probe(long long n, int reps, double* gbs) {
   using namespace cs;
   if (n <= 0 || (n & 1) || reps <= 0 || !gbs) return kBadArg;

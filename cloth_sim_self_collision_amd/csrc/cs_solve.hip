@@ -663,6 +663,17 @@ static void launch_quad(int kq, int qwidth, dim3 grid, const Sel& sel, const dou
   }
 }
 
+bool sweep_shape(const DevCloth& c, int* waves, int* loads_fwd, int* loads_bwd) {
+  if (!quad_ok(c) || c.sweep_variant == 1) return false;
+  const int K = std::min(std::max(c.kpack, 1), 8);  // the kernel instance launch_quad picks
+  *waves = quad_threads(c.qwidth) / 64;
+  // per lane: the metadata groups and coefficient pairs, then two D^-1 pairs
+  // (forward, the right-hand side in the second) or the right-hand side
+  *loads_fwd = meta_groups(K) + pack_pairs(K) + 2;
+  *loads_bwd = meta_groups(K) + pack_pairs(K) + 1;
+  return true;
+}
+
 // one forward+backward substitution launch (variants give identical results):
 // 0 automatic (the quad sweep when the mesh allows it), 1 row-by-row CSR
 // sweep.  `rhs_ready`: the forward sweep's right-hand side already holds src

@@ -14,6 +14,9 @@ inline bool quad_ok(const DevCloth& c) {
          48LL * c.np < (1LL << 32);
 }
 
+// the quad sweep's workgroup waves and vector loads per wave per forward /
+// backward sub-level; false when the row-by-row sweep runs instead
+bool sweep_shape(const DevCloth& c, int* waves, int* loads_fwd, int* loads_bwd);
 int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s);
 // element kernels + gather assembly of A and b (after boundary conditions)
 int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s, Prof* prof = nullptr);

@@ -182,6 +182,16 @@ class DeviceCloth:
         return dict(zip(("sums", "tiles", "fast_tiles", "walker_steps", "burst_elements",
                          "chain_ns", "walker_ns", "burst_ns"), out))
 
+    def sweep_shape(self) -> dict:
+        """Shape of the ILU sweeps (cs_sweep_shape): the longest component's
+        forward/backward sub-levels, waves per workgroup, vector loads per wave
+        per forward/backward sub-level, components, and whether the packed
+        quad sweep runs."""
+        out = (C.c_int * 7)()
+        check(_lib.load().cs_sweep_shape(self.h, out))
+        return dict(zip(("fwd_sublevels", "bwd_sublevels", "waves", "loads_fwd", "loads_bwd",
+                         "components", "quad"), out))
+
     def profile_read(self) -> dict:
         out = {}
         for k, name in enumerate(self.PROFILE_KINDS):
@@ -365,6 +375,14 @@ def read_probe(n_doubles: int = 1 << 23, reps: int = 5, blocks: int = 1) -> floa
     n = max(unit, int(n_doubles) // unit * unit)
     g = C.c_double()
     check(_lib.load().cs_read_probe(n, int(reps), int(blocks), C.byref(g)))
+    return g.value
+
+
+def issue_probe(waves: int = 8, steps: int = 3000) -> float:
+    """ns per vector-load instruction of one wave in a sweep-shaped workgroup
+    of `waves` waves (cs_issue_probe)."""
+    g = C.c_double()
+    check(_lib.load().cs_issue_probe(int(waves), int(steps), C.byref(g)))
     return g.value
 
 

@@ -262,6 +262,19 @@ int cs_bandwidth_probe(long long n, int reps, double* gbs);
  * (blocks = 1: one CU, the ceiling of the one-workgroup ILU sweep), reps
  * times; n a multiple of 16384 * blocks; GB/s of reads */
 int cs_read_probe(long long n, int reps, int blocks, double* gbs);
+/* vector-load issue cost of one workgroup of `waves` waves (1..8) shaped like
+ * the ILU sweep: each wave issues twelve 16-byte loads per step with two
+ * steps of loads in flight and an LDS-only barrier per step, quads of lanes
+ * reading three consecutive 16-byte values (lane 3 repeats lane 2) from a
+ * 64 MiB buffer; steps a multiple of 3.  *ns_per_load = time of one wave's
+ * load instruction (the launch time over steps * waves * 12) */
+int cs_issue_probe(int waves, int steps, double* ns_per_load);
+/* shape of a cloth's ILU sweeps, for pricing them against cs_issue_probe:
+ * out[7] = sub-levels of the longest component's forward and backward
+ * sweeps, waves per sweep workgroup, vector loads per wave per forward and
+ * per backward sub-level, components (one workgroup each), 1 if the packed
+ * quad sweep is in use (0: the row-by-row sweep; the other fields are then 0) */
+int cs_sweep_shape(const cs_cloth* cloth, int* out);
 
 #ifdef __cplusplus
 }

@@ -152,6 +152,29 @@ def test_setup_products(sim_mod, oracle_mod, name):
         assert np.array_equal(d.bvh(), o.bvh())
 
 
+def test_sweep_shape_and_issue_probe(sim_mod):
+    """The sweep shape the bench prices its issue floor with: one sub-level per
+    level while levels are at most 128 rows, four lanes per row of the widest
+    level, and the per-lane loads of the packed layout (metadata groups,
+    coefficient pairs, then D^-1 pairs or the right-hand side)."""
+    from cloth_sim_self_collision_amd import _lib
+    cl = scene.fold_sweep(40, size=0.4)
+    info = _lib.inspect_setup(cl)
+    k = max(info["max_lower"], info["max_upper"])
+    with sim_mod.DeviceCloth(cl) as d:
+        sh = d.sweep_shape()
+        assert sh["quad"] == 1 and sh["components"] == 1
+        assert sh["fwd_sublevels"] == info["levels_forward"]
+        assert sh["bwd_sublevels"] == info["levels_backward"]
+        assert 1 <= sh["waves"] <= 8
+        pairs, groups = (3 * k + 1) // 2, (k + 2 + 7) // 8
+        assert (sh["loads_fwd"], sh["loads_bwd"]) == (groups + pairs + 2, groups + pairs + 1)
+        d.set_option(1, 1)  # the row-by-row sweep: no quad shape
+        assert d.sweep_shape()["quad"] == 0
+    ns = sim_mod.issue_probe(8, 300)
+    assert 0.5 < ns < 100.0
+
+
 @pytest.mark.parametrize("name,steps,keep", [
     ("drape32", 70, (0, 25, 50, 69)),
     ("refdemo1", 30, (0, 29)),
