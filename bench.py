@@ -256,8 +256,7 @@ def main() -> int:
                 "one_cu_read_gbs": one_cu_gbs,
                 "frac_of_one_cu_read": (ach / one_cu_gbs) if (ach and one_cu_gbs) else None}
         if dom == "ilu_sweep" and issue_ns:
-            loads = shape["waves"] * (shape["fwd_sublevels"] * shape["loads_fwd"]
-                                      + shape["bwd_sublevels"] * shape["loads_bwd"])
+            loads = shape["wave_loads"]
             floor_us = loads * issue_ns * 1e-3
             roof.update({"sweep_shape": shape, "ns_per_wave_load": issue_ns,
                          "wave_loads_per_launch": loads, "issue_floor_us": floor_us,

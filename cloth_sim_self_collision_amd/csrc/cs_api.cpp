@@ -2,6 +2,7 @@
 // and the host orchestration of StepTime (self_contact_sparse/main.cpp:213-263).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -815,17 +816,42 @@ int cs_inspect_setup(const cs_mesh* m, int* sizes, int* rowptr, int* colind, int
   return kOk;
 }
 
+// vector-load sets one wave of a quad sweep workgroup issues over the
+// sub-levels q0 .. q1 - 1: three before the loop, then three per group of
+// three sub-levels, except for the groups an idle wave passes without loading
+// (it loads only while the next group has a row of its own)
+static long long sweep_wave_sets(const std::vector<int>& qptr, int q0, int q1, int wave) {
+  auto idle = [&](int l) { return l >= q1 || qptr[l] + 16 * wave >= qptr[l + 1]; };
+  auto idle3 = [&](int l) { return idle(l) && idle(l + 1) && idle(l + 2); };
+  long long sets = 3;
+  for (int l = q0; l < q1; l += 3) {
+    if (idle3(l))
+      while (l + 3 < q1 && idle3(l + 3)) l += 3;
+    sets += 3;
+  }
+  return sets;
+}
+
 int cs_sweep_shape(const cs_cloth* p, int* out) {
   if (!p || !out) return kBadArg;
   const Cloth* c = H(p);
-  for (int i = 0; i < 7; i++) out[i] = 0;
+  for (int i = 0; i < 8; i++) out[i] = 0;
   out[5] = c->d.ncomp;
   if (!sweep_shape(c->d, &out[2], &out[3], &out[4])) return kOk;
+  long long most = 0;
   for (int k = 0; k < c->S.ncomp; k++) {
-    out[0] = std::max(out[0], c->S.qf_comp[k + 1] - c->S.qf_comp[k]);
-    out[1] = std::max(out[1], c->S.qb_comp[k + 1] - c->S.qb_comp[k]);
+    const int f0 = c->S.qf_comp[k], f1 = c->S.qf_comp[k + 1];
+    const int b0 = c->S.qb_comp[k], b1 = c->S.qb_comp[k + 1];
+    out[0] = std::max(out[0], f1 - f0);
+    out[1] = std::max(out[1], b1 - b0);
+    long long loads = 0;
+    for (int w = 0; w < out[2]; w++)
+      loads += sweep_wave_sets(c->S.qf_ptr, f0, f1, w) * out[3] +
+               sweep_wave_sets(c->S.qb_ptr, b0, b1, w) * out[4];
+    most = std::max(most, loads);
   }
   out[6] = 1;
+  out[7] = (int)std::min<long long>(most, INT_MAX);
   return kOk;
 }
 

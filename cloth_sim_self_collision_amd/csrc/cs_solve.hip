@@ -570,6 +570,17 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingLen], double* o
   if (own) out[3 * q.out + lane] = z;  // forward: backward-slot order; backward: row order
 }
 
+// this wave has no row in sub-levels s .. s + 2 (those past q1 have none);
+// wave-uniform, from scalar loads of the sub-level cuts
+__device__ __forceinline__ bool wave_idle3(const int* qptr, int q1, int s) {
+  const int w16 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * 16;
+  bool idle = true;
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    if (s + j < q1) idle = idle && qptr[s + j] + w16 >= qptr[s + j + 1];
+  return idle;
+}
+
 // one sweep over the sub-levels; three register sets rotate so that the
 // loads of sub-levels l + 1 and l + 2 are in flight while l is computed
 template <bool kFwd, int K>
@@ -589,6 +600,21 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, i
   // a set is reloaded right after its compute, before the barrier: waves
   // that finish computing early issue their loads while others still compute
   for (int l = q0; l < q1; l += 3) {
+    if (!qa.live && !qb.live && !qc.live) {
+      // This wave has no row in sub-levels l .. l + 2 (narrow levels leave
+      // the high waves idle).  Its loads would cost as much issue time as a
+      // working wave's, so while the next group is idle too it only takes
+      // the group's three barriers and moves on; the group before the next
+      // working one then runs below (nothing to compute) and loads that
+      // one.  No set is touched here, so both paths reach the code below
+      // with the same loads in flight.
+      while (l + 3 < q1 && wave_idle3(qptr, q1, l + 3)) {
+        lds_barrier();
+        lds_barrier();
+        lds_barrier();
+        l += 3;
+      }
+    }
     quad_compute<kFwd, K>(ring, out, qa);
     quad_load<kFwd, K>(src, qptr, q1, l + 3, qa);
     lds_barrier();

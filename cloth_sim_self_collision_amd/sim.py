@@ -185,12 +185,12 @@ class DeviceCloth:
     def sweep_shape(self) -> dict:
         """Shape of the ILU sweeps (cs_sweep_shape): the longest component's
         forward/backward sub-levels, waves per workgroup, vector loads per wave
-        per forward/backward sub-level, components, and whether the packed
-        quad sweep runs."""
-        out = (C.c_int * 7)()
+        per forward/backward sub-level, components, whether the packed quad
+        sweep runs, and the wave-load instructions one launch issues."""
+        out = (C.c_int * 8)()
         check(_lib.load().cs_sweep_shape(self.h, out))
         return dict(zip(("fwd_sublevels", "bwd_sublevels", "waves", "loads_fwd", "loads_bwd",
-                         "components", "quad"), out))
+                         "components", "quad", "wave_loads"), out))
 
     def profile_read(self) -> dict:
         out = {}

@@ -270,10 +270,13 @@ int cs_read_probe(long long n, int reps, int blocks, double* gbs);
  * load instruction (the launch time over steps * waves * 12) */
 int cs_issue_probe(int waves, int steps, double* ns_per_load);
 /* shape of a cloth's ILU sweeps, for pricing them against cs_issue_probe:
- * out[7] = sub-levels of the longest component's forward and backward
+ * out[8] = sub-levels of the longest component's forward and backward
  * sweeps, waves per sweep workgroup, vector loads per wave per forward and
  * per backward sub-level, components (one workgroup each), 1 if the packed
- * quad sweep is in use (0: the row-by-row sweep; the other fields are then 0) */
+ * quad sweep is in use (0: the row-by-row sweep; the fields but the
+ * components are then 0), and the wave-load instructions one launch issues in
+ * its busiest workgroup (forward and backward; waves without a row in three
+ * consecutive sub-levels skip their loads) */
 int cs_sweep_shape(const cs_cloth* cloth, int* out);
 
 #ifdef __cplusplus

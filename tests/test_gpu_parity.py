@@ -169,6 +169,38 @@ def test_sweep_shape_and_issue_probe(sim_mod):
         assert 1 <= sh["waves"] <= 8
         pairs, groups = (3 * k + 1) // 2, (k + 2 + 7) // 8
         assert (sh["loads_fwd"], sh["loads_bwd"]) == (groups + pairs + 2, groups + pairs + 1)
+        # wave-load instructions: level widths from the pattern, then each
+        # wave's load sets (three up front, three per group of three levels
+        # unless the group and the one before it have no row of the wave's)
+        rp, ci = info["rowptr"], info["colind"]
+        n = len(rp) - 1
+        lf, lb = np.zeros(n, int), np.zeros(n, int)
+        for i in range(n):
+            lo = ci[rp[i]:rp[i + 1]]
+            lo = lo[lo < i]
+            lf[i] = lf[lo].max() + 1 if len(lo) else 0
+        for i in range(n - 1, -1, -1):
+            hi = ci[rp[i]:rp[i + 1]]
+            hi = hi[hi > i]
+            lb[i] = lb[hi].max() + 1 if len(hi) else 0
+
+        def sets(widths, w):
+            q1 = len(widths)
+            idle = [not (s < q1 and widths[s] > 16 * w) for s in range(q1 + 3)]
+            total, g = 3, 0
+            while g < q1:
+                if all(idle[g:g + 3]):
+                    while g + 3 < q1 and all(idle[g + 3:g + 6]):
+                        g += 3
+                total += 3
+                g += 3
+            return total
+
+        wf, wb = np.bincount(lf), np.bincount(lb)
+        assert max(wf.max(), wb.max()) <= 128  # one sub-level per level
+        want = sum(sets(wf, w) * sh["loads_fwd"] + sets(wb, w) * sh["loads_bwd"]
+                   for w in range(sh["waves"]))
+        assert sh["wave_loads"] == want
         d.set_option(1, 1)  # the row-by-row sweep: no quad shape
         assert d.sweep_shape()["quad"] == 0
     ns = sim_mod.issue_probe(8, 300)
