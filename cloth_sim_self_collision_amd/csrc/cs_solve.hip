@@ -480,16 +480,34 @@ __device__ __forceinline__ T ld32(const T* base, unsigned i) {
                                      i * (unsigned)sizeof(T));
 }
 
+// Sub-level cuts of one sweep, read at wave-uniform indices.  CutsLds reads
+// a copy of one component's cuts in LDS (indices from its first sub-level
+// q0); CutsGlobal reads them from global memory as scalar loads (any number
+// of sub-levels).  A scalar load that misses the caches costs about a
+// microsecond, and each new cache line of cuts was paid for by the waves
+// waiting on it before they could issue their loads.
+struct CutsLds {
+  const int* p;
+  int q0;
+  __device__ int at(int l) const { return __builtin_amdgcn_readfirstlane(p[l - q0]); }
+  __device__ int any() const { return q0; }  // an index that may be read
+};
+struct CutsGlobal {
+  const int* p;
+  __device__ int at(int l) const { return p[l]; }
+  __device__ int any() const { return 0; }
+};
+
 // Branch-free on purpose: a quad without a row at this sub-level loads slot 0
 // and is only marked (i = -1), lane 3 loads lane 2's right-hand side.  Loads
 // on one path only (even a wave-uniform skip) make the compiler wait for every
 // outstanding load where the paths merge, defeating the prefetch.
-template <bool kFwd, int K>
-__device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const int* qptr, int q1,
+template <bool kFwd, int K, class Cuts>
+__device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const Cuts& cuts, int q1,
                                           int l, QuadRow& q) {
   const int lane = threadIdx.x & 3, qi = threadIdx.x >> 2;
-  const int lc = l < q1 ? l : 0;  // past the last sub-level: slot 0, marked empty
-  const int first = qptr[lc], end = qptr[lc + 1];
+  const int lc = l < q1 ? l : cuts.any();  // past the last sub-level: marked empty
+  const int first = cuts.at(lc), end = cuts.at(lc + 1);
   const int s0 = first + qi;
   const bool valid = l < q1 && s0 < end;
   // the quads of a wave are consecutive: the wave has a row iff its first does
@@ -570,29 +588,34 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingLen], double* o
   if (own) out[3 * q.out + lane] = z;  // forward: backward-slot order; backward: row order
 }
 
+// sub-level cuts of one component's sweep kept in LDS (a sweep with more
+// sub-levels reads them from global memory)
+constexpr int kCutsLds = 4096;
+
 // this wave has no row in sub-levels s .. s + 2 (those past q1 have none);
-// wave-uniform, from scalar loads of the sub-level cuts
-__device__ __forceinline__ bool wave_idle3(const int* qptr, int q1, int s) {
+// wave-uniform
+template <class Cuts>
+__device__ __forceinline__ bool wave_idle3(const Cuts& cuts, int q1, int s) {
   const int w16 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * 16;
   bool idle = true;
 #pragma unroll
   for (int j = 0; j < 3; j++)
-    if (s + j < q1) idle = idle && qptr[s + j] + w16 >= qptr[s + j + 1];
+    if (s + j < q1) idle = idle && cuts.at(s + j) + w16 >= cuts.at(s + j + 1);
   return idle;
 }
 
 // one sweep over the sub-levels; three register sets rotate so that the
 // loads of sub-levels l + 1 and l + 2 are in flight while l is computed
-template <bool kFwd, int K>
 // over the sub-levels q0 .. q1 - 1 (one component's)
-__device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, int q0, int q1,
+template <bool kFwd, int K, class Cuts>
+__device__ __forceinline__ void quad_sweep(const DevCloth& c, const Cuts& cuts, int q0, int q1,
                                            const double* rhs, double* out,
                                            double (*ring)[kRingLen]) {
   const QuadSrc<kFwd, K> src = quad_src<kFwd, K>(c, rhs);
   QuadRow qa, qb, qc;
-  quad_load<kFwd, K>(src, qptr, q1, q0, qa);
-  quad_load<kFwd, K>(src, qptr, q1, q0 + 1, qb);
-  quad_load<kFwd, K>(src, qptr, q1, q0 + 2, qc);
+  quad_load<kFwd, K>(src, cuts, q1, q0, qa);
+  quad_load<kFwd, K>(src, cuts, q1, q0 + 1, qb);
+  quad_load<kFwd, K>(src, cuts, q1, q0 + 2, qc);
   // all three thirds unconditional (a count that is not a multiple of three
   // ends with empty sub-levels): a skipped third would change which loads are
   // the newest on one path, and the compiler would then wait for all loads at
@@ -608,7 +631,7 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, i
       // working one then runs below (nothing to compute) and loads that
       // one.  No set is touched here, so both paths reach the code below
       // with the same loads in flight.
-      while (l + 3 < q1 && wave_idle3(qptr, q1, l + 3)) {
+      while (l + 3 < q1 && wave_idle3(cuts, q1, l + 3)) {
         lds_barrier();
         lds_barrier();
         lds_barrier();
@@ -616,13 +639,13 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const int* qptr, i
       }
     }
     quad_compute<kFwd, K>(ring, out, qa);
-    quad_load<kFwd, K>(src, qptr, q1, l + 3, qa);
+    quad_load<kFwd, K>(src, cuts, q1, l + 3, qa);
     lds_barrier();
     quad_compute<kFwd, K>(ring, out, qb);
-    quad_load<kFwd, K>(src, qptr, q1, l + 4, qb);
+    quad_load<kFwd, K>(src, cuts, q1, l + 4, qb);
     lds_barrier();
     quad_compute<kFwd, K>(ring, out, qc);
-    quad_load<kFwd, K>(src, qptr, q1, l + 5, qc);
+    quad_load<kFwd, K>(src, cuts, q1, l + 5, qc);
     lds_barrier();
   }
 }
@@ -645,13 +668,27 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
     prod = c.prod;
   }
   __shared__ double ring[3][kRingLen];  // + the dummy and the zero entry
+  __shared__ int cuts[kCutsLds + 1];
   if (threadIdx.x < 3) ring[threadIdx.x][kRingZero] = 0.0;  // never written again
+  const int f0 = c.qf_comp[comp], f1 = c.qf_comp[comp + 1];
+  const bool fl = f1 - f0 <= kCutsLds;
+  if (fl)
+    for (int i = threadIdx.x; i <= f1 - f0; i += blockDim.x) cuts[i] = c.qf_ptr[f0 + i];
   __syncthreads();
-  // sub-level cuts are read at wave-uniform indices: scalar loads from global
-  // memory, no LDS copy and no limit on the number of sub-levels
-  quad_sweep<true, K>(c, c.qf_ptr, c.qf_comp[comp], c.qf_comp[comp + 1], nullptr, c.sb, ring);
+  if (fl)
+    quad_sweep<true, K>(c, CutsLds{cuts, f0}, f0, f1, nullptr, c.sb, ring);
+  else
+    quad_sweep<true, K>(c, CutsGlobal{c.qf_ptr}, f0, f1, nullptr, c.sb, ring);
   __syncthreads();  // the backward sweep reads the component's complete forward result
-  quad_sweep<false, K>(c, c.qb_ptr, c.qb_comp[comp], c.qb_comp[comp + 1], c.sb, dst, ring);
+  const int b0 = c.qb_comp[comp], b1 = c.qb_comp[comp + 1];
+  const bool bl = b1 - b0 <= kCutsLds;
+  if (bl)
+    for (int i = threadIdx.x; i <= b1 - b0; i += blockDim.x) cuts[i] = c.qb_ptr[b0 + i];
+  __syncthreads();
+  if (bl)
+    quad_sweep<false, K>(c, CutsLds{cuts, b0}, b0, b1, c.sb, dst, ring);
+  else
+    quad_sweep<false, K>(c, CutsGlobal{c.qb_ptr}, b0, b1, c.sb, dst, ring);
   __syncthreads();
   if (!prod) return;
   if (c.ncomp == 1) {
