@@ -634,13 +634,14 @@ int cs_get_time(const cs_cloth* p, double* t) {
 }
 
 int cs_profile_enable(cs_cloth* p, int on) {
-  if (!p) return kBadArg;
+  if (!p || on < CS_PROFILE_OFF || on > CS_PROFILE_SWEEP) return kBadArg;
   Cloth* c = H(p);
   c->prof.stream = c->stream;
   c->prof.reset();
-  c->prof.on = on != 0;
+  c->prof.on = on != CS_PROFILE_OFF;
+  c->prof.mask = on == CS_PROFILE_SWEEP ? 1u << kProfSweep : ~0u;
   CS_HIP(hipMemsetAsync(c->sum_stats, 0, sizeof(unsigned long long) * kNumSumStat, c->stream));
-  c->d.sum.stats = on ? c->sum_stats : nullptr;
+  c->d.sum.stats = on == CS_PROFILE_ALL ? c->sum_stats : nullptr;
   c->d.sum2.stats = c->d.sum.stats;
   return sync(c);
 }

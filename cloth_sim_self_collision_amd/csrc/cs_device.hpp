@@ -195,6 +195,7 @@ enum ProfKind : int {
 
 struct Prof {
   bool on = false;
+  unsigned mask = ~0u;  // kernel classes timed while on
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   std::vector<int> kinds;
@@ -230,7 +231,7 @@ struct Prof {
     kinds.clear();
   }
   int begin(int kind, hipStream_t on_stream = nullptr) {
-    if (!on) return -1;
+    if (!on || !((mask >> kind) & 1u)) return -1;
     if (used + 2 > 4096) harvest();
     if (used + 2 > pool.size()) {
       for (int k = 0; k < 64; k++) {

@@ -173,8 +173,11 @@ class DeviceCloth:
     PROFILE_KINDS = ("ilu_sweep", "pcg_reduce", "spmv", "vector_update", "ilu_factor",
                      "assemble", "contact", "other")
 
-    def profile(self, on: bool = True) -> None:
-        check(_lib.load().cs_profile_enable(self.h, 1 if on else 0))
+    def profile(self, on: bool = True, sweep_only: bool = False) -> None:
+        """Per-kernel-class HIP-event timing (cs_profile_enable): every class
+        and the reduction counters, or with sweep_only the ILU sweeps alone."""
+        mode = (2 if sweep_only else 1) if on else 0
+        check(_lib.load().cs_profile_enable(self.h, mode))
 
     def reduction_stats(self) -> dict:
         out = (C.c_longlong * 8)()

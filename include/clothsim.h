@@ -175,7 +175,14 @@ int cs_set_option(cs_cloth* c, int option, int value);
 
 /* per-kernel-class timing with HIP events on the cloth's stream:
  * 0 ILU sweeps, 1 PCG reductions, 2 SpMV, 3 vector updates, 4 ILU factor,
- * 5 assembly, 6 contact pass, 7 other.  Enabling resets the totals. */
+ * 5 assembly, 6 contact pass, 7 other.  Enabling resets the totals.
+ * on = CS_PROFILE_OFF, CS_PROFILE_ALL (every class, plus the reduction
+ * counters of cs_reduction_stats) or CS_PROFILE_SWEEP (the ILU sweeps only:
+ * two events per sweep launch, no counters, for timing that barely
+ * disturbs the step) */
+#define CS_PROFILE_OFF 0
+#define CS_PROFILE_ALL 1
+#define CS_PROFILE_SWEEP 2
 int cs_profile_enable(cs_cloth* c, int on);
 int cs_profile_read(cs_cloth* c, int kind, double* total_ms, long long* launches);
 /* counters of the exact PCG reductions while profiling is on: out[8] =

@@ -207,6 +207,29 @@ def test_sweep_shape_and_issue_probe(sim_mod):
     assert 0.5 < ns < 100.0
 
 
+def test_profile_modes(sim_mod):
+    """Sweep-only profiling times the ILU sweeps and nothing else; full
+    profiling times every class and counts the reductions."""
+    cl = scene.fold_sweep(24, size=0.3)
+    cl.pcg_maxit = 20
+    with sim_mod.DeviceCloth(cl) as d:
+        d.profile(True, sweep_only=True)
+        d.step(1)
+        p = d.profile_read()
+        assert p["ilu_sweep"][1] > 0 and p["ilu_sweep"][0] > 0.0
+        assert all(n == 0 for k, (ms, n) in p.items() if k != "ilu_sweep")
+        assert d.reduction_stats()["sums"] == 0
+        d.profile(True)
+        d.step(1)
+        p = d.profile_read()
+        assert p["ilu_sweep"][1] > 0 and p["pcg_reduce"][1] > 0 and p["spmv"][1] > 0
+        assert d.reduction_stats()["sums"] > 0
+        d.profile(False)
+        with pytest.raises(Exception):
+            from cloth_sim_self_collision_amd import _lib
+            sim_mod.check(_lib.load().cs_profile_enable(d.h, 3))
+
+
 @pytest.mark.parametrize("name,steps,keep", [
     ("drape32", 70, (0, 25, 50, 69)),
     ("refdemo1", 30, (0, 29)),
