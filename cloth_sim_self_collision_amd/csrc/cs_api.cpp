@@ -141,6 +141,7 @@ int build(Cloth* c, const cs_mesh* m) {
   for (int l = 0; l < S.nlb; l++) d.wmax = std::max(d.wmax, S.lb_ptr[l + 1] - S.lb_ptr[l]);
   Arena& A = c->arena;
   int *tri, *quad, *bc, *rowptr, *colind, *diaind, *rowof, *lfp, *lfr, *lbp, *lbr;
+  int *fup, *fuk, *fuc;
   int *csp, *cs_, *dsp, *ds, *gsp, *gs, *rp, *rg, *node, *pre, *pree, *bypre, *leafof, *rord;
   double* xyz0;
   A.reserve(&tri, 3 * (size_t)nt);
@@ -151,6 +152,9 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&colind, nnz);
   A.reserve(&diaind, np);
   A.reserve(&rowof, nnz);
+  A.reserve(&fup, S.fu_ptr.size());
+  A.reserve(&fuk, S.fu_kj.size());
+  A.reserve(&fuc, S.fu_code.size());
   A.reserve(&lfp, S.lf_ptr.size());
   A.reserve(&lfr, S.lf_rows.size());
   A.reserve(&lbp, S.lb_ptr.size());
@@ -240,6 +244,14 @@ int build(Cloth* c, const cs_mesh* m) {
   d.rowof = rowof;
   d.lf_ptr = lfp;
   d.lf_rows = lfr;
+  d.fu_ptr = fup;
+  d.fu_kj = fuk;
+  d.fu_code = fuc;
+  {
+    int widest = 0;
+    for (int i = 0; i < np; i++) widest = std::max(widest, S.rowptr[i + 1] - S.rowptr[i]);
+    d.fu_blocks = ilu_stage_blocks(widest);
+  }
   d.lb_ptr = lbp;
   d.lb_rows = lbr;
   d.crs_src_ptr = csp;
@@ -288,6 +300,9 @@ int build(Cloth* c, const cs_mesh* m) {
   if (int st = upload(colind, S.colind, s)) return st;
   if (int st = upload(diaind, S.diaind, s)) return st;
   if (int st = upload(rowof, S.rowof, s)) return st;
+  if (int st = upload(fup, S.fu_ptr, s)) return st;
+  if (int st = upload(fuk, S.fu_kj, s)) return st;
+  if (int st = upload(fuc, S.fu_code, s)) return st;
   if (int st = upload(lfp, S.lf_ptr, s)) return st;
   if (int st = upload(lfr, S.lf_rows, s)) return st;
   if (int st = upload(lbp, S.lb_ptr, s)) return st;
@@ -987,6 +1002,9 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
     v.max_wmax = std::max(v.max_wmax, d.wmax);
     v.max_qwidth = std::max(v.max_qwidth, d.qwidth);
     v.max_ncomp = std::max(v.max_ncomp, d.ncomp);
+    v.max_fu_blocks = (v.max_fu_blocks < 0 || d.fu_blocks == 0)
+                          ? -1
+                          : std::max(v.max_fu_blocks, d.fu_blocks);
     v.piped = v.piped && quad_ok(d) && (v.kpack == 0 || v.kpack == d.kpack);
     v.kpack = d.kpack;
   }

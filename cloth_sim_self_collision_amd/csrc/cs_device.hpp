@@ -79,6 +79,13 @@ struct DevCloth {
   const int* rowof;
   const int* lf_ptr;
   const int* lf_rows;
+  // ILU(0) update lists in the factorisation's loop order (MeshSetup::fu_*)
+  // and the blocks of the widest row plus the diagonal (0: the staged
+  // factorisation is not used)
+  const int* fu_ptr;
+  const int* fu_kj;
+  const int* fu_code;
+  int fu_blocks;
   const int* lb_ptr;
   const int* lb_rows;
   const int* qf_ptr;  // sub-levels of at most kQuadRows slots

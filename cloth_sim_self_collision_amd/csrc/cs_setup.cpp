@@ -264,6 +264,29 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
       if (s.colind[k] > i && s.diaind[i] == s.rowptr[i + 1]) s.diaind[i] = k;
     }
   }
+  // ILU(0) update lists (the pattern is fixed, so the column searches of
+  // the factorisation are done once here)
+  s.fu_ptr.assign(np + 1, 0);
+  s.fu_kj.clear();
+  s.fu_code.clear();
+  for (int i = 0; i < np; i++) {
+    const int rb = s.rowptr[i], re = s.rowptr[i + 1];
+    for (int ik = rb; ik < s.diaind[i]; ik++) {
+      const int kr = s.colind[ik];
+      for (int kj = s.diaind[kr]; kj < s.rowptr[kr + 1]; kj++) {
+        const int j = s.colind[kj];
+        int tgt = kFuDiag;
+        if (j != i) {
+          const int* p = std::lower_bound(s.colind.data() + rb, s.colind.data() + re, j);
+          if (p == s.colind.data() + re || *p != j) continue;  // no fill-in
+          tgt = (int)(p - (s.colind.data() + rb));
+        }
+        s.fu_kj.push_back(kj);
+        s.fu_code.push_back((ik - rb) | (tgt << 8));
+      }
+    }
+    s.fu_ptr[i + 1] = (int)s.fu_kj.size();
+  }
   // level schedules
   {
     std::vector<int> lf(np, 0), lb(np, 0);

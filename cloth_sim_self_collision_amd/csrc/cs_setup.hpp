@@ -23,6 +23,13 @@ struct MeshSetup {
   // strictly lower part) and backward substitution (strictly upper part)
   std::vector<int> lf_ptr, lf_rows, lb_ptr, lb_rows;
   int nlf = 0, nlb = 0, max_lf = 0, max_lb = 0;
+  // ILU(0) updates of each row in the factorisation's loop order
+  // (ilu_sparse.cpp:507-569: for each lower block ik of row i in column
+  // order, for each upper block kj of row k = colind[ik] in column order
+  // whose column j is i or a column of row i): fu_ptr[i] .. fu_ptr[i + 1],
+  // fu_kj = kj, fu_code = (ik - rowptr[i]) | target << 8 with target the
+  // block's index in row i from rowptr[i], or kFuDiag for j == i
+  std::vector<int> fu_ptr, fu_kj, fu_code;
   // connected components of the pattern (rows joined by a block), numbered
   // in order of their smallest row; the matrix is block diagonal over them,
   // so each component's sweeps are independent of the others'
@@ -81,6 +88,7 @@ constexpr int kRingRows = 4096;
 constexpr int kRingDummy = kRingRows, kRingZero = kRingRows + 1, kRingLen = kRingRows + 2;
 constexpr int kQuadRows = 128;  // rows per sub-level of the quad sweep (4 lanes each)
 constexpr int kMaxBlocksPerRow = 8;
+constexpr int kFuDiag = 0xff;  // update target: the diagonal block
 // 16-byte groups of sweep metadata per slot for k blocks per row
 inline constexpr int meta_groups(int k) { return (k + 2 + 7) / 8; }
 

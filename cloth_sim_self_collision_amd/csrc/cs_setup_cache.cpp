@@ -24,7 +24,7 @@ namespace cs {
 namespace {
 
 constexpr char kMagic[8] = {'C', 'S', 'S', 'E', 'T', 'U', 'P', '1'};
-constexpr long long kVersion = 4;  // bump when MeshSetup's fields or their meaning change
+constexpr long long kVersion = 5;  // bump when MeshSetup's fields or their meaning change
 
 struct Fnv {
   unsigned long long h;
@@ -61,6 +61,7 @@ bool fields(IO& io, MeshSetup& s) {
          io.vec(s.rowptr) && io.vec(s.colind) && io.vec(s.diaind) && io.vec(s.rowof) &&
          io.vec(s.lf_ptr) && io.vec(s.lf_rows) && io.vec(s.lb_ptr) && io.vec(s.lb_rows) &&
          io.num(s.nlf) && io.num(s.nlb) && io.num(s.max_lf) && io.num(s.max_lb) &&
+         io.vec(s.fu_ptr) && io.vec(s.fu_kj) && io.vec(s.fu_code) &&
          io.num(s.ncomp) && io.vec(s.comp) && io.vec(s.sf_rows) && io.vec(s.sb_rows) &&
          io.vec(s.qf_ptr) && io.vec(s.qb_ptr) && io.vec(s.qf_comp) && io.vec(s.qb_comp) &&
          io.num(s.nqf) && io.num(s.nqb) && io.num(s.qwidth) &&

@@ -259,6 +259,115 @@ __global__ __launch_bounds__(1024) void k_ilu_factor(Sel sel, int copy_in) {
   }
 }
 
+// Block ILU(0) with each row staged in LDS.  The arithmetic is ilu_row's,
+// operation for operation, in the same order: the updates of row i come from
+// its precomputed list in the reference's loop order (so no column search),
+// the row's blocks live in LDS while they are updated, and the upper block
+// of the next update is loaded while the current one is applied (rows of
+// earlier levels are final).  One thread per row, one slot of nb blocks per
+// thread: the row's blocks from rowptr[i] at their index, the diagonal at
+// nb - 1.
+constexpr int kStageDoubles = 15360;  // 120 KiB of LDS for the row slots
+
+int ilu_stage_blocks(int widest_row) {
+  const int nb = widest_row + 1;
+  // at least one wave of rows, and every block index fits the 8-bit codes
+  return (widest_row < kFuDiag && kStageDoubles / (9 * nb) >= 64) ? nb : 0;
+}
+
+static int ilu_stage_threads(int nb) {
+  return std::min(1024, kStageDoubles / (9 * nb) / 64 * 64);
+}
+
+__device__ void ilu_row_staged(const DevCloth& c, int i, int* ising, double* sl, int nb) {
+  const int rb = c.rowptr[i], re = c.rowptr[i + 1], di = c.diaind[i];
+  double* F = c.fcrs;
+  const int deg = re - rb;
+  for (int q = 0; q < 9 * deg; q++) sl[q] = F[(size_t)rb * 9 + q];
+  double* d = sl + (nb - 1) * 9;
+  for (int q = 0; q < 9; q++) d[q] = c.fdia[(size_t)i * 9 + q];
+  const int u0 = c.fu_ptr[i], u1 = c.fu_ptr[i + 1];
+  double un[9];
+  int cn = 0;
+  if (u0 < u1) {
+    cn = c.fu_code[u0];
+    const double* u = F + (size_t)c.fu_kj[u0] * 9;
+    for (int q = 0; q < 9; q++) un[q] = u[q];
+  }
+  for (int t = u0; t < u1; t++) {
+    double u[9];
+    for (int q = 0; q < 9; q++) u[q] = un[q];
+    const int code = cn;
+    if (t + 1 < u1) {
+      cn = c.fu_code[t + 1];
+      const double* v = F + (size_t)c.fu_kj[t + 1] * 9;
+      for (int q = 0; q < 9; q++) un[q] = v[q];
+    }
+    const double* l = sl + (code & 0xff) * 9;
+    const int tg = code >> 8;
+    double* a = sl + (tg == kFuDiag ? nb - 1 : tg) * 9;
+    for (int r = 0; r < 3; r++) {
+      const double l0 = l[r * 3 + 0], l1 = l[r * 3 + 1], l2 = l[r * 3 + 2];
+      a[r * 3 + 0] -= l0 * u[0] + l1 * u[3] + l2 * u[6];
+      a[r * 3 + 1] -= l0 * u[1] + l1 * u[4] + l2 * u[7];
+      a[r * 3 + 2] -= l0 * u[2] + l1 * u[5] + l2 * u[8];
+    }
+  }
+  const double det = +d[0] * d[4] * d[8] + d[3] * d[7] * d[2] + d[6] * d[1] * d[5] -
+                     d[0] * d[7] * d[5] - d[6] * d[4] * d[2] - d[3] * d[1] * d[8];
+  double* fd = c.fdia + (size_t)i * 9;
+  if (fabs(det) > 1.0e-30) {
+    const double id = 1.0 / det;
+    double t[9];
+    for (int q = 0; q < 9; q++) t[q] = d[q];
+    d[0] = id * (t[4] * t[8] - t[5] * t[7]);
+    d[1] = id * (t[2] * t[7] - t[1] * t[8]);
+    d[2] = id * (t[1] * t[5] - t[2] * t[4]);
+    d[3] = id * (t[5] * t[6] - t[3] * t[8]);
+    d[4] = id * (t[0] * t[8] - t[2] * t[6]);
+    d[5] = id * (t[2] * t[3] - t[0] * t[5]);
+    d[6] = id * (t[3] * t[7] - t[4] * t[6]);
+    d[7] = id * (t[1] * t[6] - t[0] * t[7]);
+    d[8] = id * (t[0] * t[4] - t[1] * t[3]);
+  } else if (ising) {
+    ising[i] = 1;
+    atomicAdd(&c.flags[kNsing], 1);
+  }
+  for (int q = 0; q < 9; q++) fd[q] = d[q];
+  for (int q = 0; q < 9 * (di - rb); q++) F[(size_t)rb * 9 + q] = sl[q];  // lower blocks
+  for (int ij = di; ij < re; ij++) {
+    const double* t = sl + (ij - rb) * 9;
+    double* u = F + (size_t)ij * 9;
+    for (int r = 0; r < 3; r++) {
+      u[r * 3 + 0] = d[r * 3 + 0] * t[0] + d[r * 3 + 1] * t[3] + d[r * 3 + 2] * t[6];
+      u[r * 3 + 1] = d[r * 3 + 0] * t[1] + d[r * 3 + 1] * t[4] + d[r * 3 + 2] * t[7];
+      u[r * 3 + 2] = d[r * 3 + 0] * t[2] + d[r * 3 + 1] * t[5] + d[r * 3 + 2] * t[8];
+    }
+  }
+}
+
+// copy_in as k_ilu_factor; nb: blocks per row slot (at least every row's
+// blocks plus one), blockDim.x * nb * 9 <= kStageDoubles
+template <class Sel>
+__global__ __launch_bounds__(1024) void k_ilu_factor_staged(Sel sel, int copy_in, int nb) {
+  const DevCloth c = sel.cloth();
+  __shared__ double stage[kStageDoubles];
+  if (copy_in) {
+    for (long q = threadIdx.x; q < 9L * c.np; q += blockDim.x) c.fdia[q] = c.dia[q];
+    for (long q = threadIdx.x; q < 9L * c.nnz; q += blockDim.x) c.fcrs[q] = c.crs[q];
+    for (int q = threadIdx.x; q < c.np; q += blockDim.x) c.ising[q] = 0;
+    if (threadIdx.x == 0) c.flags[kNsing] = 0;
+    __syncthreads();
+  }
+  double* sl = stage + (size_t)threadIdx.x * nb * 9;
+  for (int l = 0; l < c.nlf; l++) {
+    const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
+    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x)
+      ilu_row_staged(c, c.lf_rows[t], c.ising, sl, nb);
+    __syncthreads();
+  }
+}
+
 // re-run of the factorisation when more than 10 pivots were singular: stop at
 // the 11th singular row in row order, leave later rows untouched
 template <class Sel>
@@ -1012,7 +1121,11 @@ int launch_ilu(const DevCloth& c, hipStream_t s, Prof* prof) {
                           hipMemcpyDeviceToDevice, s));
   CS_HIP(hipMemsetAsync(c.ising, 0, sizeof(int) * c.np, s));
   CS_HIP(hipMemsetAsync(c.flags + kNsing, 0, sizeof(int), s));
-  hipLaunchKernelGGL(k_ilu_factor<One>, dim3(1), dim3(1024), 0, s, One{c, {}}, 0);
+  if (c.fu_blocks > 0)
+    hipLaunchKernelGGL(k_ilu_factor_staged<One>, dim3(1), dim3(ilu_stage_threads(c.fu_blocks)), 0,
+                       s, One{c, {}}, 0, c.fu_blocks);
+  else
+    hipLaunchKernelGGL(k_ilu_factor<One>, dim3(1), dim3(1024), 0, s, One{c, {}}, 0);
   hipLaunchKernelGGL(k_ilu_redo<One>, dim3(1), dim3(1024), 0, s, One{c, {}});
   CS_HIP(hipGetLastError());
   return launch_pack_factor(c, s);
@@ -1186,7 +1299,11 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
   }
   {
     ProfScope ps(prof, kProfFactor);
-    hipLaunchKernelGGL(k_ilu_factor<Many>, dim3(1, B), dim3(1024), 0, s, m, 1);
+    if (b.max_fu_blocks > 0)
+      hipLaunchKernelGGL(k_ilu_factor_staged<Many>, dim3(1, B),
+                         dim3(ilu_stage_threads(b.max_fu_blocks)), 0, s, m, 1, b.max_fu_blocks);
+    else
+      hipLaunchKernelGGL(k_ilu_factor<Many>, dim3(1, B), dim3(1024), 0, s, m, 1);
     hipLaunchKernelGGL(k_ilu_redo<Many>, dim3(1, B), dim3(1024), 0, s, m);
     hipLaunchKernelGGL(k_pack_factor<Many>, dim3(gp, B), dim3(kTB), 0, s, m);
   }

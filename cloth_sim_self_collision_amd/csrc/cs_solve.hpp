@@ -17,6 +17,9 @@ inline bool quad_ok(const DevCloth& c) {
 // the quad sweep's workgroup waves and vector loads per wave per forward /
 // backward sub-level; false when the row-by-row sweep runs instead
 bool sweep_shape(const DevCloth& c, int* waves, int* loads_fwd, int* loads_bwd);
+// blocks of a staged ILU row (the widest row plus the diagonal) that the
+// staged factorisation can hold, 0 if a row is too wide for it
+int ilu_stage_blocks(int widest_row);
 int launch_bend_setup(const DevCloth& c, const Params& P, hipStream_t s);
 // element kernels + gather assembly of A and b (after boundary conditions)
 int launch_assemble(const DevCloth& c, const Params& P, hipStream_t s, Prof* prof = nullptr);
@@ -45,6 +48,7 @@ struct BatchView {
   int n;
   int max_np, max_nt, max_nq, max_nnz, max_wmax;
   int max_qwidth, max_ncomp;  // widest sweep sub-level, most components of a cloth
+  int max_fu_blocks;  // staged factorisation: blocks per row slot (0: some cloth cannot)
   bool piped;  // every cloth can use the packed (quad) sweep ...
   int kpack;   // ... with this many packed blocks per row
 };
