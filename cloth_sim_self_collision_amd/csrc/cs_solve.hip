@@ -268,6 +268,7 @@ __global__ __launch_bounds__(1024) void k_ilu_factor(Sel sel, int copy_in) {
 // thread: the row's blocks from rowptr[i] at their index, the diagonal at
 // nb - 1.
 constexpr int kStageDoubles = 15360;  // 120 KiB of LDS for the row slots
+constexpr int kFuGroup = 4;           // updates whose upper blocks load together
 
 int ilu_stage_blocks(int widest_row) {
   const int nb = widest_row + 1;
@@ -275,8 +276,10 @@ int ilu_stage_blocks(int widest_row) {
   return (widest_row < kFuDiag && kStageDoubles / (9 * nb) >= 64) ? nb : 0;
 }
 
+constexpr int kStageThreads = 256;  // at most: the pipelined loads need the registers
+
 static int ilu_stage_threads(int nb) {
-  return std::min(1024, kStageDoubles / (9 * nb) / 64 * 64);
+  return std::min(kStageThreads, kStageDoubles / (9 * nb) / 64 * 64);
 }
 
 __device__ void ilu_row_staged(const DevCloth& c, int i, int* ising, double* sl, int nb) {
@@ -287,30 +290,59 @@ __device__ void ilu_row_staged(const DevCloth& c, int i, int* ising, double* sl,
   double* d = sl + (nb - 1) * 9;
   for (int q = 0; q < 9; q++) d[q] = c.fdia[(size_t)i * 9 + q];
   const int u0 = c.fu_ptr[i], u1 = c.fu_ptr[i + 1];
-  double un[9];
-  int cn = 0;
   if (u0 < u1) {
-    cn = c.fu_code[u0];
-    const double* u = F + (size_t)c.fu_kj[u0] * 9;
-    for (int q = 0; q < 9; q++) un[q] = u[q];
-  }
-  for (int t = u0; t < u1; t++) {
-    double u[9];
-    for (int q = 0; q < 9; q++) u[q] = un[q];
-    const int code = cn;
-    if (t + 1 < u1) {
-      cn = c.fu_code[t + 1];
-      const double* v = F + (size_t)c.fu_kj[t + 1] * 9;
-      for (int q = 0; q < 9; q++) un[q] = v[q];
+    // groups of kFuGroup updates: while one group is applied, the upper
+    // blocks of the next are loading and the descriptors of the one after;
+    // loads past the list repeat its last entry (never applied), so every
+    // load is unconditional
+    const int last = u1 - 1;
+    int ka[kFuGroup], ca[kFuGroup], kb[kFuGroup], cb[kFuGroup];
+#pragma unroll
+    for (int j = 0; j < kFuGroup; j++) {
+      ka[j] = c.fu_kj[min(u0 + j, last)];
+      ca[j] = c.fu_code[min(u0 + j, last)];
+      kb[j] = c.fu_kj[min(u0 + kFuGroup + j, last)];
+      cb[j] = c.fu_code[min(u0 + kFuGroup + j, last)];
     }
-    const double* l = sl + (code & 0xff) * 9;
-    const int tg = code >> 8;
-    double* a = sl + (tg == kFuDiag ? nb - 1 : tg) * 9;
-    for (int r = 0; r < 3; r++) {
-      const double l0 = l[r * 3 + 0], l1 = l[r * 3 + 1], l2 = l[r * 3 + 2];
-      a[r * 3 + 0] -= l0 * u[0] + l1 * u[3] + l2 * u[6];
-      a[r * 3 + 1] -= l0 * u[1] + l1 * u[4] + l2 * u[7];
-      a[r * 3 + 2] -= l0 * u[2] + l1 * u[5] + l2 * u[8];
+    double ua[kFuGroup][9];
+#pragma unroll
+    for (int j = 0; j < kFuGroup; j++)
+#pragma unroll
+      for (int q = 0; q < 9; q++) ua[j][q] = F[(size_t)ka[j] * 9 + q];
+    for (int g = u0; g < u1; g += kFuGroup) {
+      double ub[kFuGroup][9];
+      int kc[kFuGroup], cc[kFuGroup];
+#pragma unroll
+      for (int j = 0; j < kFuGroup; j++)
+#pragma unroll
+        for (int q = 0; q < 9; q++) ub[j][q] = F[(size_t)kb[j] * 9 + q];
+#pragma unroll
+      for (int j = 0; j < kFuGroup; j++) {
+        kc[j] = c.fu_kj[min(g + 2 * kFuGroup + j, last)];
+        cc[j] = c.fu_code[min(g + 2 * kFuGroup + j, last)];
+      }
+#pragma unroll
+      for (int j = 0; j < kFuGroup; j++) {
+        if (g + j > last) break;
+        const double* u = ua[j];
+        const double* l = sl + (ca[j] & 0xff) * 9;
+        const int tg = ca[j] >> 8;
+        double* a = sl + (tg == kFuDiag ? nb - 1 : tg) * 9;
+        for (int r = 0; r < 3; r++) {
+          const double l0 = l[r * 3 + 0], l1 = l[r * 3 + 1], l2 = l[r * 3 + 2];
+          a[r * 3 + 0] -= l0 * u[0] + l1 * u[3] + l2 * u[6];
+          a[r * 3 + 1] -= l0 * u[1] + l1 * u[4] + l2 * u[7];
+          a[r * 3 + 2] -= l0 * u[2] + l1 * u[5] + l2 * u[8];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kFuGroup; j++) {
+        ca[j] = cb[j];
+        kb[j] = kc[j];
+        cb[j] = cc[j];
+#pragma unroll
+        for (int q = 0; q < 9; q++) ua[j][q] = ub[j][q];
+      }
     }
   }
   const double det = +d[0] * d[4] * d[8] + d[3] * d[7] * d[2] + d[6] * d[1] * d[5] -
@@ -349,7 +381,7 @@ __device__ void ilu_row_staged(const DevCloth& c, int i, int* ising, double* sl,
 // copy_in as k_ilu_factor; nb: blocks per row slot (at least every row's
 // blocks plus one), blockDim.x * nb * 9 <= kStageDoubles
 template <class Sel>
-__global__ __launch_bounds__(1024) void k_ilu_factor_staged(Sel sel, int copy_in, int nb) {
+__global__ __launch_bounds__(kStageThreads) void k_ilu_factor_staged(Sel sel, int copy_in, int nb) {
   const DevCloth c = sel.cloth();
   __shared__ double stage[kStageDoubles];
   if (copy_in) {
