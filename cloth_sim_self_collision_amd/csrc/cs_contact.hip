@@ -43,6 +43,7 @@ enum Counter : int {
   kDoneCnt = 2,   // contacts whose impulse has run
   kZoneVerts = 3, // vertices in impact zones
   kZoneOverflow = 4,
+  kCand = 5,      // overlapping leaf pairs found by the traversal
   kNumCounters = 16
 };
 }  // namespace
@@ -223,8 +224,12 @@ __device__ void narrow_pair(const DevCloth& c, const double* X, const double* U,
 }
 
 // one thread per leaf: all later leaves (pre-order) with overlapping boxes
-__global__ void k_traverse(DevCloth c, const double* X, const double* U, double dt, double delta,
-                           int ccd, Emit emit) {
+// The traversal records the overlapping leaf pairs and k_narrow tests them,
+// one pair per thread: inside the traversal, a thread's narrow tests (15 per
+// pair, cubic solves in the continuous case) waited for the busiest lane of
+// its wave.  The records are sorted by key afterwards, so their order does
+// not matter.
+__global__ void k_traverse(DevCloth c, int* counters, int cap, int* cand) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= c.nt) return;
   const int L = c.leaf_of[t];
@@ -242,9 +247,23 @@ __global__ void k_traverse(DevCloth c, const double* X, const double* U, double 
       r = end;
       continue;
     }
-    if (c.node[3 * n + 2] == -1 && r > rl) narrow_pair(c, X, U, dt, delta, ccd, L, n, emit);
+    if (c.node[3 * n + 2] == -1 && r > rl) {
+      const int slot = atomicAdd(&counters[kCand], 1);
+      if (slot < cap) {
+        cand[2 * slot] = L;
+        cand[2 * slot + 1] = n;
+      }
+    }
     r++;
   }
+}
+
+// n overlapping leaf pairs (n <= the buffer's capacity)
+__global__ void k_narrow(DevCloth c, const double* X, const double* U, double dt, double delta,
+                         int ccd, int n, const int* cand, Emit emit) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  narrow_pair(c, X, U, dt, delta, ccd, cand[2 * p], cand[2 * p + 1], emit);
 }
 
 // ---------------------------------------------------------------------------
@@ -763,6 +782,17 @@ int grow_records(ContactState& cs, int need, hipStream_t s) {
   return kOk;
 }
 
+int grow_cand(ContactState& cs, int need, hipStream_t s) {
+  if (need <= cs.cand_cap) return kOk;
+  CS_HIP(hipStreamSynchronize(s));
+  const int cap = std::max(need + need / 2, 1 << 16);
+  dfree(cs.cand);
+  cs.cand_cap = 0;
+  if (int st = dalloc(&cs.cand, 2L * cap)) return st;
+  cs.cand_cap = cap;
+  return kOk;
+}
+
 int read_counters(ContactState& cs, hipStream_t s) {
   CS_HIP(hipMemcpyAsync(cs.hcounters, cs.counters, sizeof(int) * kNumCounters,
                         hipMemcpyDeviceToHost, s));
@@ -782,12 +812,23 @@ int refit(ContactState& cs, const DevCloth& d, hipStream_t s, const double* X, c
 int query(ContactState& cs, const DevCloth& d, hipStream_t s, const double* X, const double* U,
           double delta, double dt, int ccd, int* g) {
   if (int st = refit(cs, d, s, X, U, delta, dt, ccd)) return st;
-  int raw = 0;
+  int ncand = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
     CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kNumCounters, s));
+    hipLaunchKernelGGL(k_traverse, dim3(grid_for(d.nt, 64)), dim3(64), 0, s, d, cs.counters,
+                       cs.cand_cap, cs.cand);
+    CS_HIP(hipGetLastError());
+    if (int st = read_counters(cs, s)) return st;
+    ncand = cs.hcounters[kCand];
+    if (ncand <= cs.cand_cap) break;
+    if (int st = grow_cand(cs, ncand, s)) return st;
+  }
+  int raw = 0;
+  for (int attempt = 0; attempt < 2 && ncand > 0; attempt++) {
+    CS_HIP(hipMemsetAsync(cs.counters + kRaw, 0, sizeof(int), s));
     Emit em{cs.counters, cs.cap, cs.rkey, cs.rinfo, cs.rpair};
-    hipLaunchKernelGGL(k_traverse, dim3(grid_for(d.nt, 64)), dim3(64), 0, s, d, X, U, dt, delta,
-                       ccd, em);
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(ncand)), dim3(kTB), 0, s, d, X, U, dt, delta, ccd,
+                       ncand, cs.cand, em);
     CS_HIP(hipGetLastError());
     if (int st = read_counters(cs, s)) return st;
     raw = cs.hcounters[kRaw];
@@ -894,10 +935,13 @@ int contact_init(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStr
   CS_HIP(hipHostMalloc((void**)&cs.hcounters, sizeof(int) * kNumCounters));
   CS_HIP(hipMemsetAsync(cs.refit_cnt, 0, sizeof(unsigned) * S.nnode, s));
   CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kNumCounters, s));
+  if (int st = grow_cand(cs, std::max(8 * S.nt, 1 << 16), s)) return st;
   return grow_records(cs, std::max(8 * S.nt, 1 << 16), s);
 }
 
 void contact_release(ContactState& cs) {
+  dfree(cs.cand);
+  cs.cand_cap = 0;
   dfree(cs.rkey);
   dfree(cs.rinfo);
   dfree(cs.rpair);

@@ -14,6 +14,10 @@ struct ContactState {
   int np = 0, nt = 0, nnode = 0;
   int nbits = 1;          // bits per vertex index in the sort keys
   int cap = 0;            // capacity of raw contact records
+  // leaf pairs whose boxes overlap (traversal output, narrow-phase input):
+  // 2 ints per pair (first and second leaf node)
+  int* cand = nullptr;
+  int cand_cap = 0;
   // raw records from the narrow phase (unordered)
   int* rkey = nullptr;    // 4 per record: canonical key
   int* rinfo = nullptr;   // is_fv | test index << 1
