@@ -230,6 +230,57 @@ def test_profile_modes(sim_mod):
             sim_mod.check(_lib.load().cs_profile_enable(d.h, 3))
 
 
+@pytest.mark.parametrize("nsing", [5, 14])
+def test_ilu_singular_pivots(sim_mod, oracle_mod, nsing):
+    """Exactly singular pivots: rows whose diagonal block and lower blocks are
+    zero get no update that could restore them.  Up to 10 of them the
+    factorisation goes on; past 10 the reference stops at the 11th singular
+    row (ilu_sparse.cpp:507-569), which the re-run kernel reproduces.  The
+    factor bit for bit against the oracle, the count of singular pivots as
+    constructed."""
+    cl = _scenes()["drape16"]
+    o = oracle_mod.Oracle(cl, "port")
+    with sim_mod.DeviceCloth(cl) as d:
+        dia, crs, b = d.assemble(cl.xyz, cl.uvw)
+        rp, ci, di = d.pattern()
+        rows = np.sort(np.random.default_rng(7).choice(np.arange(10, cl.np_ - 10), nsing,
+                                                       replace=False))
+        dia, crs = dia.copy(), crs.copy()
+        for i in rows:
+            dia[9 * i:9 * i + 9] = 0.0
+            for k in range(rp[i], di[i]):
+                crs[9 * k:9 * k + 9] = 0.0
+        d.set_system(dia, crs, b)
+        assert d.ilu_factor() == nsing
+        fd, fc = d.get_factor()
+        ofd, ofc = o.ilu(dia, crs)
+        assert np.array_equal(fd, ofd) and np.array_equal(fc, ofc)
+
+
+def test_ilu_wide_row_factor(sim_mod, oracle_mod):
+    """A fan of 40 triangles: the centre's row has 40 blocks, too wide for the
+    LDS-staged factorisation, so the row-searching kernel runs; its factor
+    bit for bit against the oracle."""
+    n = 40
+    ang = np.arange(n) * 2.0 * np.pi / n
+    xyz = np.concatenate([[0.0, 0.0, 0.0],
+                          np.stack([np.cos(ang), np.sin(ang), 0.01 * np.sin(3 * ang)], 1).ravel()])
+    tri = np.array([[0, 1 + k, 1 + (k + 1) % n] for k in range(n)]).ravel()
+    cl = scene.from_config({"scene": "mesh", "xyz": xyz.tolist(), "tri": tri.tolist(), "dt": 0.01})
+    o = oracle_mod.Oracle(cl, "port")
+    with sim_mod.DeviceCloth(cl) as d:
+        rp, _, _ = d.pattern()
+        assert int(np.diff(rp).max()) == n
+        x = cl.xyz + 0.01 * np.sin(np.arange(3 * cl.np_))
+        dia, crs, b = d.assemble(x, cl.uvw)
+        odia, ocrs, ob = o.assemble(x, cl.uvw)
+        assert np.array_equal(dia, odia) and np.array_equal(crs, ocrs)
+        assert d.ilu_factor() == 0
+        fd, fc = d.get_factor()
+        ofd, ofc = o.ilu(odia, ocrs)
+        assert np.array_equal(fd, ofd) and np.array_equal(fc, ofc)
+
+
 @pytest.mark.parametrize("name,steps,keep", [
     ("drape32", 70, (0, 25, 50, 69)),
     ("refdemo1", 30, (0, 29)),
