@@ -240,6 +240,8 @@ __device__ void ilu_row(const DevCloth& c, int i, int stop, int* ising) {
   }
 }
 
+// The factorisation for meshes with a row too wide for the LDS-staged kernel
+// (k_ilu_factor_staged, which computes the same values faster).
 // copy_in: also copy A into the factor storage and clear the pivot flags
 // first (batched launches; the single-cloth path uses device copies)
 template <class Sel>
