@@ -58,17 +58,6 @@ __device__ __forceinline__ Box load_box(const double* bb, int n) {
   b.hz = bb[6 * n + 5];
   return b;
 }
-__device__ __forceinline__ Box load_box_coherent(const double* bb, int n) {
-  const volatile double* p = bb + 6 * n;
-  Box b;
-  b.lx = p[0];
-  b.hx = p[1];
-  b.ly = p[2];
-  b.hy = p[3];
-  b.lz = p[4];
-  b.hz = p[5];
-  return b;
-}
 __device__ __forceinline__ void store_box(double* bb, int n, const Box& b) {
   bb[6 * n + 0] = b.lx;
   bb[6 * n + 1] = b.hx;
@@ -81,8 +70,11 @@ __device__ __forceinline__ void store_box(double* bb, int n, const Box& b) {
 // ---------------------------------------------------------------------------
 // refit (bvh_aabb.cpp:23-106)
 
-__global__ void k_refit(DevCloth c, const double* X, const double* U, double delta, double dt,
-                        int ccd, unsigned* cnt) {
+// Leaf boxes from their triangles; the internal nodes follow level by level
+// (k_refit_merge), deepest first.  A box merge is a componentwise min / max,
+// exact in any order, so the boxes are the reference's.
+__global__ void k_refit_leaves(DevCloth c, const double* X, const double* U, double delta,
+                               double dt, int ccd) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= c.nt) return;
   const int i0 = c.tri[3 * t], i1 = c.tri[3 * t + 1], i2 = c.tri[3 * t + 2];
@@ -101,21 +93,17 @@ __global__ void k_refit(DevCloth c, const double* X, const double* U, double del
     box_grow(b, at(X, i1), eps);
     box_grow(b, at(X, i2), eps);
   }
-  int n = c.leaf_of[t];
-  store_box(c.bb, n, b);
-  __threadfence();
-  while (true) {
-    const int par = c.node[3 * n];
-    if (par < 0) break;
-    const unsigned old = atomicAdd(&cnt[par], 1u);
-    if ((old & 1u) == 0u) break;  // first child to arrive: the sibling finishes the parent
-    __threadfence();
-    const Box a0 = load_box_coherent(c.bb, c.node[3 * par + 1]);
-    const Box a1 = load_box_coherent(c.bb, c.node[3 * par + 2]);
-    store_box(c.bb, par, box_merge(a0, a1));
-    __threadfence();
-    n = par;
-  }
+  store_box(c.bb, c.leaf_of[t], b);
+}
+
+// the internal nodes among refit_order[b .. e) (one depth): box = merge of
+// the children's boxes, written by earlier launches
+__global__ void k_refit_merge(DevCloth c, int b, int e) {
+  const int k = b + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= e) return;
+  const int n = c.refit_order[k];
+  if (c.node[3 * n + 2] == -1) return;  // a leaf: boxed from its triangle
+  store_box(c.bb, n, box_merge(load_box(c.bb, c.node[3 * n + 1]), load_box(c.bb, c.node[3 * n + 2])));
 }
 
 // ---------------------------------------------------------------------------
@@ -802,8 +790,12 @@ int read_counters(ContactState& cs, hipStream_t s) {
 
 int refit(ContactState& cs, const DevCloth& d, hipStream_t s, const double* X, const double* U,
           double delta, double dt, int ccd) {
-  hipLaunchKernelGGL(k_refit, dim3(grid_for(d.nt)), dim3(kTB), 0, s, d, X, U, delta, dt, ccd,
-                     cs.refit_cnt);
+  hipLaunchKernelGGL(k_refit_leaves, dim3(grid_for(d.nt)), dim3(kTB), 0, s, d, X, U, delta, dt,
+                     ccd);
+  for (size_t l = 0; l + 1 < cs.refit_lvl.size(); l++) {
+    const int b = cs.refit_lvl[l], e = cs.refit_lvl[l + 1];
+    if (e > b) hipLaunchKernelGGL(k_refit_merge, dim3(grid_for(e - b)), dim3(kTB), 0, s, d, b, e);
+  }
   CS_HIP(hipGetLastError());
   return kOk;
 }
@@ -924,7 +916,7 @@ int contact_init(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStr
   if (int st = dalloc(&cs.vstart, S.np + 1)) return st;
   if (int st = dalloc(&cs.vfill, S.np + 1)) return st;
   if (int st = dalloc(&cs.counters, kNumCounters)) return st;
-  if (int st = dalloc(&cs.refit_cnt, S.nnode)) return st;
+  cs.refit_lvl = S.refit_lvl_ptr;
   if (int st = dalloc(&cs.zpar, S.np)) return st;
   if (int st = dalloc(&cs.zroot, S.np)) return st;
   if (int st = dalloc(&cs.zmemb, S.np)) return st;
@@ -933,7 +925,6 @@ int contact_init(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStr
   if (int st = dalloc(&cs.qx, 3L * S.np)) return st;
   if (int st = dalloc(&cs.qv, 3L * S.np)) return st;
   CS_HIP(hipHostMalloc((void**)&cs.hcounters, sizeof(int) * kNumCounters));
-  CS_HIP(hipMemsetAsync(cs.refit_cnt, 0, sizeof(unsigned) * S.nnode, s));
   CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kNumCounters, s));
   if (int st = grow_cand(cs, std::max(8 * S.nt, 1 << 16), s)) return st;
   return grow_records(cs, std::max(8 * S.nt, 1 << 16), s);
@@ -959,7 +950,6 @@ void contact_release(ContactState& cs) {
   dfree(cs.vstart);
   dfree(cs.vfill);
   dfree(cs.counters);
-  dfree(cs.refit_cnt);
   dfree(cs.zpar);
   dfree(cs.zroot);
   dfree(cs.zmemb);

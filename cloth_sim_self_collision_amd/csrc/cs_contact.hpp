@@ -2,6 +2,8 @@
 // (GetIntermidiateVelocityContactResolved, self_collision_cloth.cpp:355-456).
 #pragma once
 
+#include <vector>
+
 #include <cstdint>
 
 #include "../../include/clothsim.h"
@@ -41,7 +43,9 @@ struct ContactState {
   // scalars
   int* counters = nullptr;   // device
   int* hcounters = nullptr;  // pinned mirror
-  unsigned* refit_cnt = nullptr;  // arrival counters per node (parity)
+  // refit levels: nodes of depth max_depth - l at refit_order[refit_lvl[l]
+  // .. refit_lvl[l + 1]) (MeshSetup::refit_lvl_ptr, host copy)
+  std::vector<int> refit_lvl;
   // rigid impact zones
   int* zpar = nullptr;       // union-find parent per vertex, -1 outside zones
   int* zroot = nullptr;
