@@ -44,6 +44,7 @@ enum Counter : int {
   kZoneVerts = 3, // vertices in impact zones
   kZoneOverflow = 4,
   kCand = 5,      // overlapping leaf pairs found by the traversal
+  kReady = 6,     // contacts with no earlier contact on their vertices
   kNumCounters = 16
 };
 }  // namespace
@@ -464,8 +465,8 @@ __global__ void k_inc_fill(const int* ce, int g, const int* vstart, int* vfill, 
 }
 
 // order each vertex's incidences by contact and link every incidence to the
-// previous contact on that vertex
-__global__ void k_inc_link(int np, const int* vstart, int* inc, int* pred) {
+// previous and the next contact on that vertex
+__global__ void k_inc_link(int np, const int* vstart, int* inc, int* pred, int* succ) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= np) return;
   const int b = vstart[v], e = vstart[v + 1];
@@ -478,24 +479,58 @@ __global__ void k_inc_link(int np, const int* vstart, int* inc, int* pred) {
     }
     inc[j + 1] = x;
   }
-  for (int i = b; i < e; i++) pred[inc[i]] = (i == b) ? -1 : (inc[i - 1] >> 2);
+  for (int i = b; i < e; i++) {
+    pred[inc[i]] = (i == b) ? -1 : (inc[i - 1] >> 2);
+    succ[inc[i]] = (i + 1 == e) ? -1 : (inc[i + 1] >> 2);
+  }
 }
 
-__global__ void k_impulse_round(const int* ce, int g, const int* pred, int* done_round, int round,
-                                const double* X, double* U, double delta, double k, double dt,
-                                double m, int ccd, int* counters) {
+// contacts with no earlier contact on their vertices start the schedule
+__global__ void k_impulse_ready(int g, const int* pred, int* npred, int* ready, int* counters) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g || done_round[i] != 0) return;
-  for (int a = 0; a < 4; a++) {
-    const int p = pred[4 * i + a];
-    if (p >= 0) {
-      const int r = done_round[p];
-      if (r == 0 || r == round) return;
+  if (i >= g) return;
+  int c = 0;
+  for (int a = 0; a < 4; a++) c += pred[4 * i + a] >= 0 ? 1 : 0;
+  npred[i] = c;
+  if (c == 0) ready[atomicAdd(&counters[kReady], 1)] = i;
+}
+
+// Runs the ready contacts and every contact that becomes ready, round by
+// round, in one workgroup: a contact is ready once every earlier contact on
+// its vertices has run.  The contacts of a round share no vertex and each
+// reads its vertices' velocities after every earlier contact on them has
+// written them, so the velocities are those of the key-order Gauss-Seidel
+// sweep.  Every contact enters a list once, so the loop ends after at most
+// g contacts; the count of those run goes to counters[kDoneCnt].
+__global__ __launch_bounds__(1024) void k_impulse_drain(const int* ce, const int* succ, int* npred,
+                                                        int* ready0, int* ready1, const double* X,
+                                                        double* U, double delta, double k,
+                                                        double dt, double m, int ccd,
+                                                        int* counters) {
+  __shared__ int n_next;
+  int n = counters[kReady];
+  int *cur = ready0, *nxt = ready1;
+  int done = 0;
+  while (n > 0) {
+    if (threadIdx.x == 0) n_next = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int i = cur[t];
+      impulse_one(ce + 5 * i, X, U, delta, k, dt, m, ccd);
+      for (int a = 0; a < 4; a++) {
+        const int sc = succ[4 * i + a];
+        if (sc >= 0 && atomicSub(&npred[sc], 1) == 1) nxt[atomicAdd(&n_next, 1)] = sc;
+      }
     }
+    done += n;
+    __syncthreads();  // the round's velocities and the next list are complete
+    n = n_next;
+    int* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+    __syncthreads();  // every thread has read n_next before it is reset
   }
-  impulse_one(ce + 5 * i, X, U, delta, k, dt, m, ccd);
-  done_round[i] = round;
-  atomicAdd(&counters[kDoneCnt], 1);
+  if (threadIdx.x == 0) counters[kDoneCnt] = done;
 }
 
 // ---------------------------------------------------------------------------
@@ -738,7 +773,10 @@ int grow_records(ContactState& cs, int need, hipStream_t s) {
   dfree(cs.gpos);
   dfree(cs.ce);
   dfree(cs.pred);
-  dfree(cs.done_round);
+  dfree(cs.succ);
+  dfree(cs.npred);
+  dfree(cs.ready[0]);
+  dfree(cs.ready[1]);
   dfree(cs.inc);
   if (cs.tmp) (void)hipFree(cs.tmp);
   cs.tmp = nullptr;
@@ -754,7 +792,10 @@ int grow_records(ContactState& cs, int need, hipStream_t s) {
   if (int st = dalloc(&cs.gpos, cap)) return st;
   if (int st = dalloc(&cs.ce, 5L * cap)) return st;
   if (int st = dalloc(&cs.pred, 4L * cap)) return st;
-  if (int st = dalloc(&cs.done_round, cap)) return st;
+  if (int st = dalloc(&cs.succ, 4L * cap)) return st;
+  if (int st = dalloc(&cs.npred, cap)) return st;
+  if (int st = dalloc(&cs.ready[0], cap)) return st;
+  if (int st = dalloc(&cs.ready[1], cap)) return st;
   if (int st = dalloc(&cs.inc, 4L * cap)) return st;
   // temporary storage for the largest sort / scan
   size_t a = 0, b = 0, c2 = 0;
@@ -871,7 +912,6 @@ int impulses(ContactState& cs, const DevCloth& d, const Params& P, hipStream_t s
   if (g == 0) return kOk;
   CS_HIP(hipMemsetAsync(cs.vcount, 0, sizeof(int) * (d.np + 1), s));
   CS_HIP(hipMemsetAsync(cs.vfill, 0, sizeof(int) * (d.np + 1), s));
-  CS_HIP(hipMemsetAsync(cs.done_round, 0, sizeof(int) * g, s));
   hipLaunchKernelGGL(k_inc_count, dim3(grid_for(g)), dim3(kTB), 0, s, cs.ce, g, cs.vcount);
   size_t tb = cs.tmp_bytes;
   CS_HIP(rocprim::exclusive_scan(cs.tmp, tb, cs.vcount, cs.vstart, 0, (size_t)d.np + 1,
@@ -879,22 +919,19 @@ int impulses(ContactState& cs, const DevCloth& d, const Params& P, hipStream_t s
   hipLaunchKernelGGL(k_inc_fill, dim3(grid_for(g)), dim3(kTB), 0, s, cs.ce, g, cs.vstart, cs.vfill,
                      cs.inc);
   hipLaunchKernelGGL(k_inc_link, dim3(grid_for(d.np)), dim3(kTB), 0, s, d.np, cs.vstart, cs.inc,
-                     cs.pred);
+                     cs.pred, cs.succ);
   CS_HIP(hipMemsetAsync(cs.counters + kDoneCnt, 0, sizeof(int), s));
+  CS_HIP(hipMemsetAsync(cs.counters + kReady, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_impulse_ready, dim3(grid_for(g)), dim3(kTB), 0, s, g, cs.pred, cs.npred,
+                     cs.ready[0], cs.counters);
+  hipLaunchKernelGGL(k_impulse_drain, dim3(1), dim3(1024), 0, s, cs.ce, cs.succ, cs.npred,
+                     cs.ready[0], cs.ready[1], X, U, P.clearance, P.k_contact, P.dt, P.mass, ccd,
+                     cs.counters);
   CS_HIP(hipGetLastError());
-  int round = 1;
-  while (true) {
-    for (int k = 0; k < 4; k++, round++)
-      hipLaunchKernelGGL(k_impulse_round, dim3(grid_for(g)), dim3(kTB), 0, s, cs.ce, g, cs.pred,
-                         cs.done_round, round, X, U, P.clearance, P.k_contact, P.dt, P.mass, ccd,
-                         cs.counters);
-    CS_HIP(hipGetLastError());
-    if (int st = read_counters(cs, s)) return st;
-    if (cs.hcounters[kDoneCnt] >= g) break;
-    if (round > g + 8) {
-      set_error("impulse schedule did not drain");
-      return kHipError;
-    }
+  if (int st = read_counters(cs, s)) return st;
+  if (cs.hcounters[kDoneCnt] != g) {
+    set_error("impulse schedule did not drain");
+    return kHipError;
   }
   return kOk;
 }
@@ -944,7 +981,10 @@ void contact_release(ContactState& cs) {
   dfree(cs.gpos);
   dfree(cs.ce);
   dfree(cs.pred);
-  dfree(cs.done_round);
+  dfree(cs.succ);
+  dfree(cs.npred);
+  dfree(cs.ready[0]);
+  dfree(cs.ready[1]);
   dfree(cs.inc);
   dfree(cs.vcount);
   dfree(cs.vstart);

@@ -35,7 +35,9 @@ struct ContactState {
   int* ce = nullptr;
   // impulse schedule
   int* pred = nullptr;       // 4 per contact: previous contact on the same vertex
-  int* done_round = nullptr;
+  int* succ = nullptr;       // 4 per contact: next contact on the same vertex
+  int* npred = nullptr;      // earlier contacts on its vertices not yet run
+  int* ready[2] = {nullptr, nullptr};  // contacts ready to run, this round and the next
   int* inc = nullptr;        // 4 per contact: vertex incidences grouped by vertex
   int* vcount = nullptr;     // np + 1
   int* vstart = nullptr;     // np + 1
