@@ -354,6 +354,14 @@ int sync(Cloth* c) {
   return kOk;
 }
 
+int check_tickets(const int* hflags) {
+  if (hflags[kTicketErr]) {
+    set_error("a PCG reduction ended with its ticket nonzero (inconsistent skip flag)");
+    return kHipError;
+  }
+  return kOk;
+}
+
 // StepTime_InternalDynamicsILU on the resident state
 int internal_step(Cloth* c, double* conv, int* iters) {
   DevCloth& d = c->d;
@@ -369,6 +377,7 @@ int internal_step(Cloth* c, double* conv, int* iters) {
   CS_HIP(hipMemcpyAsync(c->hflags, d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
   CS_HIP(hipMemcpyAsync(c->hscal, d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
   CS_HIP(hipStreamSynchronize(s));
+  if (int st = check_tickets(c->hflags)) return st;
   if (conv) *conv = c->hscal[kConv];
   if (iters) *iters = c->hflags[kDone] ? c->hflags[kIters] : c->P.pcg_maxit;
   return kOk;
@@ -796,6 +805,7 @@ int cs_pcg_solve(cs_cloth* p, const double* b, double tol, int maxit, double* x,
   CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
   CS_HIP(hipMemcpyAsync(c->hscal, c->d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
   if (int st = sync(c)) return st;
+  if (int st = check_tickets(c->hflags)) return st;
   if (conv) *conv = c->hscal[kConv];
   if (iters) *iters = c->hflags[kDone] ? c->hflags[kIters] : maxit;
   return kOk;
@@ -1071,6 +1081,8 @@ extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
       CS_HIP(hipMemcpyAsync(c->hscal, c->d.scal, sizeof(double) * kNumScal, hipMemcpyDeviceToHost, s));
     }
     CS_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; i++)
+      if (int st = check_tickets(b->cl[i]->hflags)) return st;
     for (size_t i = 0; i < n; i++) {
       Cloth* c = b->cl[i];
       c->have_x1 = true;

@@ -41,10 +41,10 @@ enum Counter : int {
   kRaw = 0,       // narrow-phase records emitted
   kGroups = 1,    // distinct contacts
   kDoneCnt = 2,   // contacts whose impulse has run
-  kZoneVerts = 3, // vertices in impact zones
-  kZoneOverflow = 4,
-  kCand = 5,      // overlapping leaf pairs found by the traversal
-  kReady = 6,     // contacts with no earlier contact on their vertices
+  kCand = 3,      // overlapping leaf pairs found by the traversal
+  kReady = 4,     // contacts with no earlier contact on their vertices
+  kQueryCounters = 8,  // a query clears counters [0, kQueryCounters)
+  kZoneVerts = 8, // vertices in impact zones after the last zone pass
   kNumCounters = 16
 };
 }  // namespace
@@ -582,40 +582,30 @@ __global__ __launch_bounds__(1024) void k_riz_pass(DevCloth c, const int* ce, in
                                                    const double* X, const double* U0, double* U,
                                                    int* counters) {
   // 1. zones touched by each contact through a vertex or its 1-ring; merge
-  //    (only the partition matters: zone order never changes a result)
+  //    (only the partition matters: zone order never changes a result).
+  //    Each touched zone is joined to the first one found as it is found, so
+  //    any number of zones merges; membership (zpar >= 0) is what the
+  //    reference tests, and a merge does not change it.
   if (threadIdx.x == 0) {
     for (int e = 0; e < g; e++) {
       const int* n4 = ce + 5 * e + 1;
-      int hit[128];
-      int nh = 0;
+      int root = -1;
       for (int a = 0; a < 4; a++) {
         const int v = n4[a];
         if (zpar[v] >= 0) {
           const int r = zfind(zpar, v);
-          bool dup = false;
-          for (int h = 0; h < nh; h++) dup |= hit[h] == r;
-          if (!dup) {
-            if (nh == 128) counters[kZoneOverflow] = 1; else hit[nh++] = r;
-          }
+          if (root < 0) root = r; else if (r != root) zpar[r] = root;
         }
         for (int q = c.ring_ptr[v]; q < c.ring_ptr[v + 1]; q++) {
           const int w = c.ring[q];
           if (zpar[w] < 0) continue;
           const int r = zfind(zpar, w);
-          bool dup = false;
-          for (int h = 0; h < nh; h++) dup |= hit[h] == r;
-          if (!dup) {
-            if (nh == 128) counters[kZoneOverflow] = 1; else hit[nh++] = r;
-          }
+          if (root < 0) root = r; else if (r != root) zpar[r] = root;
         }
       }
-      int root;
-      if (nh == 0) {
+      if (root < 0) {
         root = n4[0];
         zpar[root] = root;
-      } else {
-        root = hit[0];
-        for (int h = 1; h < nh; h++) zpar[hit[h]] = root;
       }
       for (int a = 0; a < 4; a++)
         if (zpar[n4[a]] < 0) zpar[n4[a]] = root;
@@ -847,7 +837,7 @@ int query(ContactState& cs, const DevCloth& d, hipStream_t s, const double* X, c
   if (int st = refit(cs, d, s, X, U, delta, dt, ccd)) return st;
   int ncand = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
-    CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kNumCounters, s));
+    CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kQueryCounters, s));
     hipLaunchKernelGGL(k_traverse, dim3(grid_for(d.nt, 64)), dim3(64), 0, s, d, cs.counters,
                        cs.cand_cap, cs.cand);
     CS_HIP(hipGetLastError());
@@ -1036,6 +1026,7 @@ int contact_resolve(ContactState& cs, const MeshSetup& S, const DevCloth& d, con
   // rigid impact zones, up to 100 passes
   CS_HIP(hipMemcpyAsync(cs.u0, U, sizeof(double) * 3 * d.np, hipMemcpyDeviceToDevice, s));
   CS_HIP(hipMemsetAsync(cs.zpar, 0xff, sizeof(int) * d.np, s));
+  CS_HIP(hipMemsetAsync(cs.counters + kZoneVerts, 0, sizeof(int), s));
   for (int it = 0; it < 100; it++) {
     if (int st = query(cs, d, s, X, U, P.clearance, P.dt, 1, &g)) return st;
     if (g == 0) break;

@@ -61,6 +61,13 @@ enum Flag : int {
   kIters = 1,   // iteration count reported like Solve_PCG
   kNsing = 2,   // ILU pivots with |det| <= 1e-30
   kFirstSing11 = 3,  // row of the 11th singular pivot, np if none
+  // kDone as the main stream saw it before the convergence test of the
+  // iteration started beside the sweep: the sweep and the (r, z) reduction
+  // read this copy, which nothing changes while they run.  (A reduction whose
+  // blocks disagreed on a flag changing under them would leave its ticket
+  // short and corrupt every later reduction on the same scratch.)
+  kSkip = 4,
+  kTicketErr = 5,  // a reduction ticket was found nonzero after the solve
   kNumFlag = 16
 };
 

@@ -235,6 +235,31 @@ void levels_to_lists(const std::vector<int>& lvl, int nlev, std::vector<int>& pt
   for (int i = 0; i < nlev; i++) *maxw = std::max(*maxw, ptr[i + 1] - ptr[i]);
 }
 
+// Live interval of a row in a sweep: [level written, last level read]; rows
+// of one component whose sweep slots share a ring position must have disjoint
+// intervals (components are swept by separate workgroups, each with its own
+// ring).  Slots run in level order within a component, so the ring is a FIFO
+// whose window only has to cover the rows live across a few levels.
+bool ring_fits(const MeshSetup& s, const std::vector<int>& lvl, bool lower,
+               const std::vector<int>& slot_of) {
+  const int np = s.np;
+  std::vector<int> last(np);
+  for (int i = 0; i < np; i++) last[i] = lvl[i];
+  for (int i = 0; i < np; i++) {
+    const int b = lower ? s.rowptr[i] : s.diaind[i];
+    const int e = lower ? s.diaind[i] : s.rowptr[i + 1];
+    for (int k = b; k < e; k++) last[s.colind[k]] = std::max(last[s.colind[k]], lvl[i]);
+  }
+  // (component, ring entry, first level, last level) per row
+  std::vector<std::array<int, 4>> iv(np);
+  for (int i = 0; i < np; i++) iv[i] = {s.comp[i], slot_of[i] & (kRingRows - 1), lvl[i], last[i]};
+  std::sort(iv.begin(), iv.end());
+  for (int q = 1; q < np; q++)
+    if (iv[q][0] == iv[q - 1][0] && iv[q][1] == iv[q - 1][1] && iv[q][2] <= iv[q - 1][3])
+      return false;
+  return true;
+}
+
 }  // namespace
 
 std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,
@@ -406,36 +431,12 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
           s.b_meta[midx(k, sl)] = k < bc ? ring_byte(bslot[s.colind[s.diaind[j] + k]]) : zero_byte;
         put32(s.b_meta, sl, (unsigned)j);
       }
-      // live interval of a row in a sweep: [level written, last level read];
-      // rows of one component whose sweep slots share a ring position must
-      // have disjoint intervals (components are swept by separate workgroups,
-      // each with its own ring).  Slots run in level order within a
-      // component, so the ring is a FIFO whose window only has to cover the
-      // rows live across a few levels.
-      auto check = [&](const std::vector<int>& lvl, bool lower, const std::vector<int>& slot_of) {
-        std::vector<int> last(np);
-        for (int i = 0; i < np; i++) last[i] = lvl[i];
-        for (int i = 0; i < np; i++) {
-          const int b = lower ? s.rowptr[i] : s.diaind[i];
-          const int e = lower ? s.diaind[i] : s.rowptr[i + 1];
-          for (int k = b; k < e; k++) last[s.colind[k]] = std::max(last[s.colind[k]], lvl[i]);
-        }
-        // (component, ring entry, first level, last level) per row
-        std::vector<std::array<int, 4>> iv(np);
-        for (int i = 0; i < np; i++)
-          iv[i] = {s.comp[i], slot_of[i] & (kRingRows - 1), lvl[i], last[i]};
-        std::sort(iv.begin(), iv.end());
-        for (int q = 1; q < np; q++)
-          if (iv[q][0] == iv[q - 1][0] && iv[q][1] == iv[q - 1][1] && iv[q][2] <= iv[q - 1][3])
-            return false;
-        return true;
-      };
       std::vector<int> lf(np), lb(np);
       for (int l = 0; l < s.nlf; l++)
         for (int q = s.lf_ptr[l]; q < s.lf_ptr[l + 1]; q++) lf[s.lf_rows[q]] = l;
       for (int l = 0; l < s.nlb; l++)
         for (int q = s.lb_ptr[l]; q < s.lb_ptr[l + 1]; q++) lb[s.lb_rows[q]] = l;
-      s.ring_ok = check(lf, true, s.fslot) && check(lb, false, bslot);
+      s.ring_ok = ring_fits(s, lf, true, s.fslot) && ring_fits(s, lb, false, bslot);
     } else {
       s.kl = s.ku = 0;
     }
@@ -549,6 +550,218 @@ std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, co
     std::vector<int> fill(s.refit_lvl_ptr.begin(), s.refit_lvl_ptr.end() - 1);
     for (int v = 0; v < n; v++) s.refit_order[fill[s.max_depth - s.depth[v]]++] = v;
   }
+  return "";
+}
+
+// ---------------------------------------------------------------------------
+// structural validation of a MeshSetup against its mesh
+
+namespace {
+
+bool is_perm(const std::vector<int>& v, int n) {
+  if ((int)v.size() != n) return false;
+  std::vector<char> seen(n, 0);
+  for (int x : v) {
+    if (x < 0 || x >= n || seen[x]) return false;
+    seen[x] = 1;
+  }
+  return true;
+}
+
+bool is_ptr(const std::vector<int>& p, int n, int total) {
+  if ((int)p.size() != n + 1 || p[0] != 0 || p[n] != total) return false;
+  for (int i = 0; i < n; i++)
+    if (p[i + 1] < p[i]) return false;
+  return true;
+}
+
+// level of each row from a level list; false unless it lists every row once
+bool level_of(const std::vector<int>& ptr, const std::vector<int>& rows, int nl, int np,
+              std::vector<int>& lvl) {
+  if (!is_ptr(ptr, nl, np) || !is_perm(rows, np)) return false;
+  lvl.assign(np, -1);
+  for (int l = 0; l < nl; l++)
+    for (int q = ptr[l]; q < ptr[l + 1]; q++) lvl[rows[q]] = l;
+  return true;
+}
+
+// slot order `rows` cut by `q` into sub-levels of one level and one
+// component, at most kQuadRows rows each; component c's sub-levels are
+// qc[c] .. qc[c + 1]
+bool subs_ok(const MeshSetup& s, const std::vector<int>& rows, const std::vector<int>& q,
+             const std::vector<int>& qc, const std::vector<int>& lvl, int nsub) {
+  if (!is_perm(rows, s.np) || !is_ptr(q, nsub, s.np) || !is_ptr(qc, s.ncomp, nsub)) return false;
+  for (int c = 0; c < s.ncomp; c++)
+    for (int l = qc[c]; l < qc[c + 1]; l++) {
+      if (q[l + 1] - q[l] < 1 || q[l + 1] - q[l] > kQuadRows || q[l + 1] - q[l] > s.qwidth)
+        return false;
+      for (int k = q[l]; k < q[l + 1]; k++)
+        if (s.comp[rows[k]] != c || lvl[rows[k]] != lvl[rows[q[l]]]) return false;
+      if (l > qc[c] && lvl[rows[q[l]]] < lvl[rows[q[l - 1]]]) return false;
+    }
+  return true;
+}
+
+bool src_ok(const std::vector<int>& ptr, const std::vector<int>& src, int n, long long tri_lim,
+            long long quad_lim) {
+  if (!is_ptr(ptr, n, (int)src.size())) return false;
+  for (int v : src)
+    if (v >= 0 ? (long long)v >= tri_lim : -(long long)v - 1 >= quad_lim) return false;
+  return true;
+}
+
+}  // namespace
+
+std::string validate_setup(const MeshSetup& s, int np, int nt, int nq, const int* tri,
+                           const int* quad) {
+  if (s.np != np || s.nt != nt || s.nq != nq) return "sizes differ from the mesh";
+  // pattern: the union of the bending stencils, sorted, no diagonal
+  if (!is_ptr(s.rowptr, np, s.nnz) || (int)s.colind.size() != s.nnz) return "bad row offsets";
+  if ((int)s.diaind.size() != np || (int)s.rowof.size() != s.nnz) return "bad pattern sizes";
+  for (int i = 0; i < np; i++) {
+    int first_upper = s.rowptr[i + 1];
+    for (int k = s.rowptr[i]; k < s.rowptr[i + 1]; k++) {
+      const int j = s.colind[k];
+      if (j < 0 || j >= np || j == i || s.rowof[k] != i) return "bad column index";
+      if (k > s.rowptr[i] && j <= s.colind[k - 1]) return "columns not ascending";
+      if (j > i && first_upper == s.rowptr[i + 1]) first_upper = k;
+    }
+    if (s.diaind[i] != first_upper) return "bad diagonal index";
+  }
+  {
+    // every stencil pair is an entry and every entry comes from a stencil
+    std::vector<char> covered(s.nnz, 0);
+    for (int e = 0; e < nq; e++)
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+          const int i = quad[4 * e + a], j = quad[4 * e + b];
+          if (i == j) continue;
+          if (i < 0 || i >= np || j < 0 || j >= np) return "bending stencil index out of range";
+          const int* lo = s.colind.data() + s.rowptr[i];
+          const int* hi = s.colind.data() + s.rowptr[i + 1];
+          const int* f = std::lower_bound(lo, hi, j);
+          if (f == hi || *f != j) return "pattern misses a stencil pair";
+          covered[f - s.colind.data()] = 1;
+        }
+    for (int k = 0; k < s.nnz; k++)
+      if (!covered[k]) return "pattern entry outside the bending stencils";
+  }
+  // ILU(0) update lists
+  if (!is_ptr(s.fu_ptr, np, (int)s.fu_kj.size()) || s.fu_code.size() != s.fu_kj.size())
+    return "bad ILU update lists";
+  for (int i = 0; i < np; i++)
+    for (int u = s.fu_ptr[i]; u < s.fu_ptr[i + 1]; u++) {
+      const int ik = s.fu_code[u] & 0xff, tgt = (s.fu_code[u] >> 8) & 0xff;
+      const int kj = s.fu_kj[u];
+      if (ik >= s.diaind[i] - s.rowptr[i] || kj < 0 || kj >= s.nnz) return "bad ILU update";
+      const int k = s.colind[s.rowptr[i] + ik];
+      if (kj < s.diaind[k] || kj >= s.rowptr[k + 1]) return "ILU update outside the upper row";
+      const int j = s.colind[kj];
+      if (tgt == kFuDiag ? j != i : (tgt >= s.rowptr[i + 1] - s.rowptr[i] ||
+                                      s.colind[s.rowptr[i] + tgt] != j))
+        return "ILU update target is not the column";
+    }
+  // level schedules: every dependency lies in a strictly earlier level, so
+  // the rows of one level are independent
+  std::vector<int> lf, lb;
+  if (!level_of(s.lf_ptr, s.lf_rows, s.nlf, np, lf) || !level_of(s.lb_ptr, s.lb_rows, s.nlb, np, lb))
+    return "bad level lists";
+  for (int i = 0; i < np; i++) {
+    for (int k = s.rowptr[i]; k < s.diaind[i]; k++)
+      if (lf[s.colind[k]] >= lf[i]) return "forward level schedule breaks a dependency";
+    for (int k = s.diaind[i]; k < s.rowptr[i + 1]; k++)
+      if (lb[s.colind[k]] >= lb[i]) return "backward level schedule breaks a dependency";
+  }
+  // components: rows joined by a block share one
+  if ((int)s.comp.size() != np || s.ncomp < 1) return "bad components";
+  for (int i = 0; i < np; i++) {
+    if (s.comp[i] < 0 || s.comp[i] >= s.ncomp) return "bad component id";
+    for (int k = s.rowptr[i]; k < s.rowptr[i + 1]; k++)
+      if (s.comp[s.colind[k]] != s.comp[i]) return "a block joins two components";
+  }
+  if (s.qwidth < 1 || s.qwidth > kQuadRows) return "bad sub-level width";
+  if ((int)s.qf_ptr.size() != s.nqf + 1 || (int)s.qb_ptr.size() != s.nqb + 1 ||
+      !subs_ok(s, s.sf_rows, s.qf_ptr, s.qf_comp, lf, s.nqf) ||
+      !subs_ok(s, s.sb_rows, s.qb_ptr, s.qb_comp, lb, s.nqb))
+    return "bad sweep sub-levels";
+  // packed sweeps: slots, counts and LDS ring offsets (the kernel indexes
+  // its ring with these offsets, so they must stay inside it)
+  if (s.kl < 0 || s.ku < 0 || s.kl > kMaxBlocksPerRow || s.ku > kMaxBlocksPerRow)
+    return "bad packed block counts";
+  if (s.kl > 0) {
+    int kl = 0, ku = 0;
+    for (int i = 0; i < np; i++) {
+      kl = std::max(kl, s.diaind[i] - s.rowptr[i]);
+      ku = std::max(ku, s.rowptr[i + 1] - s.diaind[i]);
+    }
+    if (kl != s.kl || ku != s.ku) return "packed block counts differ from the pattern";
+    const int kq = std::max(std::max(s.kl, s.ku), 1);
+    const size_t mw = (size_t)meta_groups(kq) * 8;
+    if (s.f_rci.size() != 4 * (size_t)np || s.b_rc.size() != 2 * (size_t)np ||
+        !is_perm(s.fslot, np) || s.f_meta.size() != mw * np || s.b_meta.size() != mw * np)
+      return "bad packed sweep sizes";
+    std::vector<int> bslot(np);
+    for (int sl = 0; sl < np; sl++) bslot[s.sb_rows[sl]] = sl;
+    auto midx = [np](int h, int sl) { return ((size_t)(h / 8) * np + sl) * 8 + h % 8; };
+    const unsigned short zero_byte = (unsigned short)(kRingZero * (int)sizeof(double));
+    for (int sl = 0; sl < np; sl++) {
+      const int i = s.sf_rows[sl], j = s.sb_rows[sl];
+      if (s.fslot[i] != sl || s.f_rci[4 * sl] != i || s.f_rci[4 * sl + 1] != s.diaind[i] - s.rowptr[i] ||
+          s.f_rci[4 * sl + 2] != bslot[i] || s.b_rc[2 * sl] != j ||
+          s.b_rc[2 * sl + 1] != s.rowptr[j + 1] - s.diaind[j])
+        return "bad packed slot record";
+      for (int k = 0; k < kq; k++) {
+        const unsigned short fo = s.f_meta[midx(k, sl)], bo = s.b_meta[midx(k, sl)];
+        const int fc = s.diaind[i] - s.rowptr[i], bc = s.rowptr[j + 1] - s.diaind[j];
+        const unsigned short fw =
+            k < fc ? (unsigned short)((s.fslot[s.colind[s.rowptr[i] + k]] & (kRingRows - 1)) * 8)
+                   : zero_byte;
+        const unsigned short bw =
+            k < bc ? (unsigned short)((bslot[s.colind[s.diaind[j] + k]] & (kRingRows - 1)) * 8)
+                   : zero_byte;
+        if (fo != fw || bo != bw) return "bad sweep ring offset";
+      }
+      const unsigned fout = s.f_meta[midx(kq, sl)] | ((unsigned)s.f_meta[midx(kq + 1, sl)] << 16);
+      const unsigned bout = s.b_meta[midx(kq, sl)] | ((unsigned)s.b_meta[midx(kq + 1, sl)] << 16);
+      if (fout != (unsigned)bslot[i] || bout != (unsigned)j) return "bad sweep output index";
+    }
+    if (s.ring_ok && !(ring_fits(s, lf, true, s.fslot) && ring_fits(s, lb, false, bslot)))
+      return "sweep dependencies do not fit the LDS ring";
+  }
+  // gather lists
+  if (!src_ok(s.crs_src_ptr, s.crs_src, s.nnz, 9LL * nt, 16LL * nq) ||
+      !src_ok(s.dia_src_ptr, s.dia_src, np, 9LL * nt, 16LL * nq) ||
+      !src_ok(s.grad_src_ptr, s.grad_src, np, 3LL * nt, 4LL * nq))
+    return "bad assembly gather lists";
+  if (!is_ptr(s.ring_ptr, np, (int)s.ring.size())) return "bad vertex rings";
+  for (int v : s.ring)
+    if (v < 0 || v >= np) return "bad vertex ring entry";
+  // BVH: a binary tree over the triangles rooted at node 0
+  const int nn = s.nnode;
+  if (nn != 2 * nt - 1 || (int)s.node.size() != 3 * nn || s.root != 0) return "bad BVH size";
+  if ((int)s.pre.size() != nn || (int)s.pre_end.size() != nn || !is_perm(s.by_pre, nn) ||
+      (int)s.depth.size() != nn || (int)s.leaf_of.size() != nt || !is_perm(s.refit_order, nn) ||
+      !is_ptr(s.refit_lvl_ptr, s.max_depth + 1, nn))
+    return "bad BVH arrays";
+  std::vector<int> leaf_seen(nt, 0);
+  for (int v = 0; v < nn; v++) {
+    const int p = s.node[3 * v], c0 = s.node[3 * v + 1], c1 = s.node[3 * v + 2];
+    if (v == 0 ? p != -1 : (p < 0 || p >= nn)) return "bad BVH parent";
+    if (s.by_pre[s.pre[v]] != v || s.pre_end[v] <= s.pre[v] || s.pre_end[v] > nn)
+      return "bad BVH pre-order";
+    if (s.depth[v] < 0 || s.depth[v] > s.max_depth) return "bad BVH depth";
+    if (c1 == -1) {
+      if (c0 < 0 || c0 >= nt || leaf_seen[c0]++ || s.leaf_of[c0] != v) return "bad BVH leaf";
+    } else {
+      if (c0 <= 0 || c0 >= nn || c1 <= 0 || c1 >= nn || s.node[3 * c0] != v || s.node[3 * c1] != v ||
+          s.depth[c0] != s.depth[v] + 1 || s.depth[c1] != s.depth[v] + 1)
+        return "bad BVH child";
+    }
+  }
+  for (int l = 0; l <= s.max_depth; l++)
+    for (int k = s.refit_lvl_ptr[l]; k < s.refit_lvl_ptr[l + 1]; k++)
+      if (s.depth[s.refit_order[k]] != s.max_depth - l) return "bad BVH refit order";
+  (void)tri;
   return "";
 }
 

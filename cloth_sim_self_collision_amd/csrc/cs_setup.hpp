@@ -96,6 +96,16 @@ inline constexpr int meta_groups(int k) { return (k + 2 + 7) / 8; }
 std::string build_setup(MeshSetup& s, int np, int nt, int nq, const int* tri, const int* quad,
                         const double* xyz_init);
 
+// Structural check of a setup against its mesh: the pattern is the stencils'
+// union, every index is in range, the level schedules respect every
+// dependency (rows of a level are independent), sub-levels and sweep slots
+// are consistent, the LDS ring offsets stay inside the ring (and ring_ok
+// holds when claimed), gather lists and the BVH are well formed.  Empty
+// string when it holds.  Every setup loaded from the cache passes it before
+// use; fresh builds are checked when CLOTHSIM_VALIDATE_SETUP is set.
+std::string validate_setup(const MeshSetup& s, int np, int nt, int nq, const int* tri,
+                           const int* quad);
+
 // build_setup through the on-disk cache of setup products (cs_setup_cache.cpp;
 // no cache directory set: plain build_setup)
 std::string build_setup_cached(MeshSetup& s, int np, int nt, int nq, const int* tri,

@@ -8,10 +8,14 @@
 // File: "<dir>/<key>.cssetup", key = two independent 64-bit FNV-1a hashes of
 // the inputs and the layout constants.  Contents: a header (magic, version,
 // sizes, both hashes), then every MeshSetup field in a fixed order (scalars
-// as int64, vectors as an int64 count followed by the elements).  A file that
-// is missing, truncated or for another mesh is ignored and rebuilt; writes go
+// as int64, vectors as an int64 count followed by the elements), then a
+// 64-bit FNV-1a hash of those field bytes.  A file that is missing,
+// truncated, for another mesh, whose payload hash differs, or whose contents
+// fail validate_setup (indices out of range, broken schedules) is ignored
+// and rebuilt: nothing read from disk reaches a kernel unchecked.  Writes go
 // to a temporary name first and are renamed into place.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -24,7 +28,7 @@ namespace cs {
 namespace {
 
 constexpr char kMagic[8] = {'C', 'S', 'S', 'E', 'T', 'U', 'P', '1'};
-constexpr long long kVersion = 5;  // bump when MeshSetup's fields or their meaning change
+constexpr long long kVersion = 6;  // bump when MeshSetup's fields or their meaning change
 
 struct Fnv {
   unsigned long long h;
@@ -77,7 +81,11 @@ bool fields(IO& io, MeshSetup& s) {
 
 struct Writer {
   FILE* f;
-  bool raw(const void* p, size_t n) { return n == 0 || fwrite(p, 1, n, f) == n; }
+  Fnv payload{0xcbf29ce484222325ull};
+  bool raw(const void* p, size_t n) {
+    payload.bytes(p, n);
+    return n == 0 || fwrite(p, 1, n, f) == n;
+  }
   bool num(int v) {
     const long long x = v;
     return raw(&x, sizeof(x));
@@ -93,10 +101,13 @@ struct Writer {
 struct Reader {
   FILE* f;
   long long left;  // bytes left in the file
+  Fnv payload{0xcbf29ce484222325ull};
   bool raw(void* p, size_t n) {
     if ((long long)n > left) return false;
     left -= (long long)n;
-    return n == 0 || fread(p, 1, n, f) == n;
+    if (n != 0 && fread(p, 1, n, f) != n) return false;
+    payload.bytes(p, n);
+    return true;
   }
   bool num(int& v) {
     long long x = 0;
@@ -129,6 +140,20 @@ std::mutex g_mu;
 std::string g_dir;
 long long g_hits = 0, g_misses = 0, g_writes = 0;
 
+// build_setup, then validate_setup when CLOTHSIM_VALIDATE_SETUP is set to a
+// value other than "0" (a debug check of what the kernels rely on)
+std::string build_setup_checked(MeshSetup& s, int np, int nt, int nq, const int* tri,
+                                const int* quad, const double* xyz) {
+  std::string err = build_setup(s, np, nt, nq, tri, quad, xyz);
+  if (!err.empty()) return err;
+  const char* v = std::getenv("CLOTHSIM_VALIDATE_SETUP");
+  if (v && *v && std::strcmp(v, "0") != 0) {
+    err = validate_setup(s, np, nt, nq, tri, quad);
+    if (!err.empty()) return "setup failed validation: " + err;
+  }
+  return std::string();
+}
+
 }  // namespace
 
 void set_setup_cache_dir(const char* dir) {
@@ -151,7 +176,7 @@ std::string build_setup_cached(MeshSetup& s, int np, int nt, int nq, const int* 
     dir = g_dir;
   }
   if (dir.empty() || np <= 0 || nt <= 0 || !tri || !xyz || (nq > 0 && !quad))
-    return build_setup(s, np, nt, nq, tri, quad, xyz);
+    return build_setup_checked(s, np, nt, nq, tri, quad, xyz);
   Fnv a(1469598103934665603ull), b(0x9e3779b97f4a7c15ull);
   hash_inputs(a, np, nt, nq, tri, quad, xyz);
   hash_inputs(b, np, nt, nq, tri, quad, xyz);
@@ -165,12 +190,18 @@ std::string build_setup_cached(MeshSetup& s, int np, int nt, int nq, const int* 
     Reader r{f, size};
     Header h;
     MeshSetup t;
-    const bool ok = r.raw(&h, sizeof(h)) && std::memcmp(h.magic, kMagic, 8) == 0 &&
-                    h.version == kVersion && h.np == np && h.nt == nt && h.nq == nq &&
-                    h.h1 == a.h && h.h2 == b.h && fields(r, t) && r.left == 0 &&
-                    t.np == np && t.nt == nt && t.nq == nq &&
-                    (int)t.rowptr.size() == np + 1 && (int)t.diaind.size() == np;
+    bool ok = r.raw(&h, sizeof(h)) && std::memcmp(h.magic, kMagic, 8) == 0 &&
+              h.version == kVersion && h.np == np && h.nt == nt && h.nq == nq &&
+              h.h1 == a.h && h.h2 == b.h;
+    if (ok) {
+      r.payload = Fnv(0xcbf29ce484222325ull);
+      ok = fields(r, t);
+      const unsigned long long want = r.payload.h;
+      unsigned long long got = 0;
+      ok = ok && r.raw(&got, sizeof(got)) && r.left == 0 && got == want;
+    }
     fclose(f);
+    ok = ok && validate_setup(t, np, nt, nq, tri, quad).empty();
     if (ok) {
       s = std::move(t);
       std::lock_guard<std::mutex> lk(g_mu);
@@ -182,7 +213,7 @@ std::string build_setup_cached(MeshSetup& s, int np, int nt, int nq, const int* 
     std::lock_guard<std::mutex> lk(g_mu);
     g_misses++;
   }
-  std::string err = build_setup(s, np, nt, nq, tri, quad, xyz);
+  std::string err = build_setup_checked(s, np, nt, nq, tri, quad, xyz);
   if (!err.empty()) return err;
   // best effort: a cache that cannot be written only costs the rebuild
   char tmp_name[96];
@@ -198,7 +229,11 @@ std::string build_setup_cached(MeshSetup& s, int np, int nt, int nq, const int* 
     h.nq = nq;
     h.h1 = a.h;
     h.h2 = b.h;
-    const bool ok = w.raw(&h, sizeof(h)) && fields(w, s);
+    bool ok = w.raw(&h, sizeof(h));
+    w.payload = Fnv(0xcbf29ce484222325ull);
+    ok = ok && fields(w, s);
+    const unsigned long long sum = w.payload.h;
+    ok = ok && w.raw(&sum, sizeof(sum));
     const bool closed = fclose(f) == 0;
     if (ok && closed && std::rename(tmp.c_str(), path.c_str()) == 0) {
       std::lock_guard<std::mutex> lk(g_mu);

@@ -442,7 +442,7 @@ template <class Sel>
 __global__ __launch_bounds__(1024) void k_ilu_apply(Sel sel, const double* src, double* dst,
                                                     double* prod) {
   const DevCloth c = sel.cloth();
-  if (c.flags[kDone]) return;
+  if (c.flags[kSkip]) return;
   if (!src) {
     src = c.r;
     dst = c.z;
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
                                                                  double* dst, double* prod) {
   const DevCloth c = sel.cloth();
   const int comp = blockIdx.x;
-  if (c.flags[kDone] || comp >= c.ncomp) return;  // batch grids fit the most components
+  if (c.flags[kSkip] || comp >= c.ncomp) return;  // batch grids fit the most components
   if (!src_row) {
     src_row = c.r;
     dst = c.z;
@@ -982,11 +982,15 @@ __global__ void k_spmv_dir(Sel sel, int it) {
 }
 
 // r -= alpha A p; x += alpha p; prod = r .* r  (matrix_square_sparse.cpp:327-336)
+// Also snapshots kDone into kSkip for the kernels that run beside the next
+// convergence test (no kernel changes kDone while this one runs).
 template <class Sel>
 __global__ void k_update_rx(Sel sel, int it) {
   const DevCloth c = sel.cloth();
   const double* __restrict__ p = pcg_dir(c, it);
-  if (c.flags[kDone]) return;
+  const int done = c.flags[kDone];
+  if (blockIdx.x == 0 && threadIdx.x == 0) c.flags[kSkip] = done;
+  if (done) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
   const double alpha = c.scal[kAlpha];
@@ -1030,7 +1034,9 @@ enum DotMode : int { kDotRr0 = 0, kDotRz0 = 1, kDotPap = 2, kDotRr = 3, kDotRz =
 
 // exact left-to-right sum of prod, then the PCG bookkeeping of `mode`
 // (ilu_sparse.cpp:646-700); every mode but the first is skipped once the
-// solve has converged
+// solve has converged.  The (r, z) reductions run beside the convergence test
+// that may set kDone, so they read its snapshot kSkip (k_update_rx), which is
+// constant while they run: every block of a launch takes the same branch.
 template <class Sel>
 struct DotSrc {
   Sel sel;
@@ -1041,7 +1047,10 @@ struct DotSrc {
     // the convergence test reduces its own products with its own scratch: it
     // runs beside the sweep and the (r, z) reduction
     if (mode == kDotRr) return SumJob{c.prod2, 3 * c.np, c.sum2, c.flags + kDone};
-    return SumJob{c.prod, 3 * c.np, c.sum, mode == kDotRr0 ? nullptr : c.flags + kDone};
+    const int* skip = mode == kDotRr0 ? nullptr
+                      : (mode == kDotRz0 || mode == kDotRz) ? c.flags + kSkip
+                                                            : c.flags + kDone;
+    return SumJob{c.prod, 3 * c.np, c.sum, skip};
   }
   __device__ void finish(double s) const {
     const DevCloth c = sel.cloth();
@@ -1052,6 +1061,7 @@ struct DotSrc {
           c.scal[kConv] = 0.0;
           c.flags[kIters] = 0;
           c.flags[kDone] = 1;
+          c.flags[kSkip] = 1;  // nothing runs beside this one
         } else {
           c.scal[kInvRr0] = 1.0 / s;
         }
@@ -1094,6 +1104,16 @@ __global__ void k_square(Sel sel) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
   c.prod[e] = c.r[e] * c.r[e];
+}
+
+// after a solve every reduction has run to its end, which leaves its ticket
+// at zero; a nonzero ticket would mean blocks of one launch disagreed on
+// skipping it, and is reported (kTicketErr) instead of corrupting the next solve
+template <class Sel>
+__global__ void k_pcg_check(Sel sel) {
+  const DevCloth c = sel.cloth();
+  if (threadIdx.x == 0)
+    c.flags[kTicketErr] = (*c.sum.ticket != 0u || *c.sum2.ticket != 0u) ? 1 : 0;
 }
 
 // x += dx, v = dx / dt (solve_internal_sparse.h:323-325)
@@ -1262,6 +1282,8 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
     launch_dot(c, kDotFinal, maxit, s);
     CS_HIP(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_pcg_check<One>, dim3(1), dim3(64), 0, s, One{c, {}});
+  CS_HIP(hipGetLastError());
   return kOk;
 }
 
@@ -1410,6 +1432,7 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
     ProfScope ps(prof, kProfReduce);
     launch_dot_batch(b, kDotFinal, maxit, s);
   }
+  hipLaunchKernelGGL(k_pcg_check<Many>, dim3(1, B), dim3(64), 0, s, m);
   hipLaunchKernelGGL(k_finish_internal<Many>, dim3(gv, B), dim3(kTB), 0, s, m);
   CS_HIP(hipGetLastError());
   return kOk;
