@@ -1037,20 +1037,27 @@ enum DotMode : int { kDotRr0 = 0, kDotRz0 = 1, kDotPap = 2, kDotRr = 3, kDotRz =
 // solve has converged.  The (r, z) reductions run beside the convergence test
 // that may set kDone, so they read its snapshot kSkip (k_update_rx), which is
 // constant while they run: every block of a launch takes the same branch.
+//
+// Which products, scratch and skip flag a mode uses is decided on the host
+// (dot_src) and passed as two plain values, so job() is two two-way selects.
+// (A three-way choice of the skip pointer written as a conditional on `mode`
+// was compiled for gfx950 into a branch structure that left the addend and
+// skip-flag pointers undefined for the (p, Ap) mode, and the kernel faulted
+// on the load of the flag.)
 template <class Sel>
 struct DotSrc {
   Sel sel;
   int mode;
   int iter;
+  int aux;   // 1: the convergence test's products and scratch (prod2, sum2)
+  int skip;  // flag that cancels the reduction, -1: none
   __device__ SumJob job() const {
     const DevCloth c = sel.cloth();
+    const bool a = aux != 0;
     // the convergence test reduces its own products with its own scratch: it
     // runs beside the sweep and the (r, z) reduction
-    if (mode == kDotRr) return SumJob{c.prod2, 3 * c.np, c.sum2, c.flags + kDone};
-    const int* skip = mode == kDotRr0 ? nullptr
-                      : (mode == kDotRz0 || mode == kDotRz) ? c.flags + kSkip
-                                                            : c.flags + kDone;
-    return SumJob{c.prod, 3 * c.np, c.sum, skip};
+    return SumJob{a ? c.prod2 : c.prod, 3 * c.np, a ? c.sum2 : c.sum,
+                  skip < 0 ? nullptr : c.flags + skip};
   }
   __device__ void finish(double s) const {
     const DevCloth c = sel.cloth();
@@ -1094,8 +1101,16 @@ struct DotSrc {
   }
 };
 
+// the reduction of PCG mode `mode` (see DotSrc)
+template <class Sel>
+DotSrc<Sel> dot_src(const Sel& sel, int mode, int iter) {
+  const int aux = mode == kDotRr ? 1 : 0;
+  const int skip = mode == kDotRr0 ? -1 : (mode == kDotRz0 || mode == kDotRz) ? kSkip : kDone;
+  return DotSrc<Sel>{sel, mode, iter, aux, skip};
+}
+
 static void launch_dot(const DevCloth& c, int mode, int iter, hipStream_t s) {
-  launch_ordered_sum(DotSrc<One>{One{c, {}}, mode, iter}, 3 * c.np, 1, s);
+  launch_ordered_sum(dot_src(One{c, {}}, mode, iter), 3 * c.np, 1, s);
 }
 
 template <class Sel>
@@ -1335,7 +1350,7 @@ static void launch_sweep_batch(const BatchView& b, hipStream_t s) {
 }
 
 static void launch_dot_batch(const BatchView& b, int mode, int iter, hipStream_t s) {
-  launch_ordered_sum(DotSrc<Many>{Many{b.tab, b.ptab}, mode, iter}, 3 * b.max_np, b.n, s);
+  launch_ordered_sum(dot_src(Many{b.tab, b.ptab}, mode, iter), 3 * b.max_np, b.n, s);
 }
 
 int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int* dgather,
