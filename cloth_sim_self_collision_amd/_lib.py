@@ -42,14 +42,15 @@ class Params(C.Structure):
 class StepStats(C.Structure):
     _fields_ = [("pcg_conv", C.c_double), ("pcg_iters", C.c_int), ("ilu_singular", C.c_int),
                 ("impulse_applied", C.c_int), ("n_prox", C.c_int), ("n_ccd_pass", C.c_int),
-                ("n_ccd", C.c_int * 5), ("n_riz_pass", C.c_int), ("n_riz_vertices", C.c_int)]
+                ("n_ccd", C.c_int * 5), ("n_riz_pass", C.c_int), ("n_riz_vertices", C.c_int),
+                ("status", C.c_int)]
 
     def as_dict(self) -> dict:
         return {"pcg_conv": self.pcg_conv, "pcg_iters": self.pcg_iters,
                 "ilu_singular": self.ilu_singular, "impulse_applied": bool(self.impulse_applied),
                 "n_prox": self.n_prox, "n_ccd_pass": self.n_ccd_pass,
                 "n_ccd": list(self.n_ccd), "n_riz_pass": self.n_riz_pass,
-                "n_riz_vertices": self.n_riz_vertices}
+                "n_riz_vertices": self.n_riz_vertices, "status": self.status}
 
 
 # name -> (restype, argtypes); every exported symbol of include/clothsim.h
@@ -72,6 +73,8 @@ SIGNATURES = {
     "cs_batch_destroy": (C.c_int, [_V]),
     "cs_batch_size": (C.c_int, [_V, C.POINTER(C.c_int)]),
     "cs_batch_step": (C.c_int, [_V, C.c_int, _V]),
+    "cs_batch_profile_enable": (C.c_int, [_V, C.c_int]),
+    "cs_batch_profile_read": (C.c_int, [_V, C.c_int, _D, C.POINTER(C.c_longlong)]),
     "cs_bandwidth_probe": (C.c_int, [C.c_longlong, C.c_int, _D]),
     "cs_read_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, _D]),
     "cs_issue_probe": (C.c_int, [C.c_int, C.c_int, _D]),

@@ -2,9 +2,12 @@
 // and the host orchestration of StepTime (self_contact_sparse/main.cpp:213-263).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/clothsim.h"
@@ -380,6 +383,11 @@ int internal_step(Cloth* c, double* conv, int* iters) {
   if (int st = check_tickets(c->hflags)) return st;
   if (conv) *conv = c->hscal[kConv];
   if (iters) *iters = c->hflags[kDone] ? c->hflags[kIters] : c->P.pcg_maxit;
+  if (c->hflags[kFlagNonFinite]) {
+    set_error("the internal solve produced an infinite or NaN displacement (the ILU(0)-PCG "
+              "iterate overflowed)");
+    return kNonFinite;
+  }
   return kOk;
 }
 
@@ -595,7 +603,17 @@ int cs_step(cs_cloth* p, int n, cs_step_stats* stats) {
       const double off[3] = {c->sweep_delta[0] * sv, c->sweep_delta[1] * sv, c->sweep_delta[2] * sv};
       if (int st = launch_apply_sweep(c->d, c->sweep_verts, c->n_sweep, off, c->stream)) return st;
     }
-    if (int st = internal_step(c, &conv, &iters)) return st;
+    if (int st = internal_step(c, &conv, &iters)) {
+      if (st == kNonFinite) {
+        c->time += c->P.dt;
+        st_local.pcg_conv = conv;
+        st_local.pcg_iters = iters;
+        st_local.ilu_singular = c->hflags[kNsing];
+        st_local.status = kNonFinite;
+        if (stats) *stats = st_local;
+      }
+      return st;
+    }
     int applied = 0;
     Prof* prof = c->prof.on ? &c->prof : nullptr;
     if (int st = contact_resolve(c->contact, c->S, c->d, c->P, c->stream, &applied, &st_local, prof))
@@ -639,6 +657,14 @@ int cs_set_option(cs_cloth* p, int option, int value) {
   Cloth* c = H(p);
   if (option == CS_OPT_SWEEP && value >= 0 && value <= 1) {
     c->d.sweep_variant = value;
+    return kOk;
+  }
+  if (option == CS_OPT_PCG && value >= 0 && value <= 2) {
+    if (value == 2 && !pcg_wg_ok(c->d)) {
+      set_error("the one-workgroup PCG solve needs the packed sweep and 3 np <= 65536");
+      return kBadArg;
+    }
+    c->d.pcg_variant = value;
     return kOk;
   }
   set_error("unknown option or value");
@@ -934,7 +960,40 @@ struct Batch {
   int poll = 8;
   std::vector<DevCloth> htab;
   std::vector<Params> hptab;
+  hipEvent_t internal_done = nullptr;    // the batched internal step of this step
+  std::vector<hipEvent_t> cloth_done;    // each cloth's contact pass and update
+  Prof prof;                             // per-kernel-class timing of the batched launches
 };
+
+// host threads that run the contact passes of a batch's cloths
+constexpr int kContactThreads = 8;
+
+// fn(i) for i in [0, n) on up to `threads` host threads; returns the status
+// of the first failing index (its message becomes this thread's error)
+template <class F>
+int parallel_for(int n, int threads, F&& fn) {
+  if (n <= 0) return kOk;
+  const int t = std::max(1, std::min(threads, n));
+  std::vector<int> status(n, kOk);
+  std::vector<std::string> msg(n);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int i = next++; i < n; i = next++) {
+      status[i] = fn(i);
+      if (status[i] != kOk) msg[i] = last_error();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int k = 1; k < t; k++) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  for (int i = 0; i < n; i++)
+    if (status[i] != kOk) {
+      set_error(msg[i]);
+      return status[i];
+    }
+  return kOk;
+}
 
 namespace {
 
@@ -952,6 +1011,9 @@ void batch_free(Batch* b) {
   if (b->ptab) (void)hipFree(b->ptab);
   if (b->dgather) (void)hipFree(b->dgather);
   if (b->hgather) (void)hipHostFree(b->hgather);
+  if (b->internal_done) (void)hipEventDestroy(b->internal_done);
+  for (hipEvent_t e : b->cloth_done) (void)hipEventDestroy(e);
+  b->prof.release();
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -1018,6 +1080,8 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
     v.piped = v.piped && quad_ok(d) && (v.kpack == 0 || v.kpack == d.kpack);
     v.kpack = d.kpack;
   }
+  v.wg = v.piped;
+  for (Cloth* c : b->cl) v.wg = v.wg && pcg_wg_ok(c->d) && c->d.pcg_variant == 2;
   b->htab.resize(n);
   b->hptab.resize(n);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1027,11 +1091,19 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
       hipMalloc((void**)&b->dtab, sizeof(DevCloth) * n) != hipSuccess ||
       hipMalloc((void**)&b->ptab, sizeof(Params) * n) != hipSuccess ||
       hipMalloc((void**)&b->dgather, sizeof(int) * n) != hipSuccess ||
-      hipHostMalloc((void**)&b->hgather, sizeof(int) * n) != hipSuccess) {
+      hipHostMalloc((void**)&b->hgather, sizeof(int) * n) != hipSuccess ||
+      hipEventCreateWithFlags(&b->internal_done, hipEventDisableTiming) != hipSuccess) {
     batch_free(b);
     set_error("cs_batch_create: device allocation failed");
     return kOutOfMemory;
   }
+  b->cloth_done.assign(n, nullptr);
+  for (int i = 0; i < n; i++)
+    if (hipEventCreateWithFlags(&b->cloth_done[i], hipEventDisableTiming) != hipSuccess) {
+      batch_free(b);
+      set_error("cs_batch_create: event creation failed");
+      return kHipError;
+    }
   v.tab = b->dtab;
   v.ptab = b->ptab;
   *out = reinterpret_cast<cs_batch*>(b);
@@ -1074,7 +1146,7 @@ extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
         if (int st = launch_apply_sweep(c->d, c->sweep_verts, c->n_sweep, off, s)) return st;
       }
     if (int st = batch_internal_step(b->view, b->tol, b->maxit, b->poll, b->dgather, b->hgather, s,
-                                     nullptr, &b->aux))
+                                     b->prof.on ? &b->prof : nullptr, &b->aux))
       return st;
     for (Cloth* c : b->cl) {
       CS_HIP(hipMemcpyAsync(c->hflags, c->d.flags, sizeof(int) * kNumFlag, hipMemcpyDeviceToHost, s));
@@ -1083,7 +1155,11 @@ extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
     CS_HIP(hipStreamSynchronize(s));
     for (size_t i = 0; i < n; i++)
       if (int st = check_tickets(b->cl[i]->hflags)) return st;
-    for (size_t i = 0; i < n; i++) {
+    // contact passes: every cloth on its own stream, from up to
+    // kContactThreads host threads (each pass waits on the host for its
+    // contact counts, so passes of different cloths overlap those waits)
+    CS_HIP(hipEventRecord(b->internal_done, s));
+    auto one = [&](int i) -> int {
       Cloth* c = b->cl[i];
       c->have_x1 = true;
       cs_step_stats st_local;
@@ -1091,17 +1167,50 @@ extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
       st_local.pcg_conv = c->hscal[kConv];
       st_local.pcg_iters = c->hflags[kDone] ? c->hflags[kIters] : c->P.pcg_maxit;
       st_local.ilu_singular = c->hflags[kNsing];
+      c->time += c->P.dt;
+      if (c->hflags[kFlagNonFinite]) {
+        st_local.status = kNonFinite;
+        if (stats) stats[i] = st_local;
+        return kOk;
+      }
+      CS_HIP(hipStreamWaitEvent(c->stream, b->internal_done, 0));
       int applied = 0;
-      if (int st = contact_resolve(c->contact, c->S, c->d, c->P, s, &applied, &st_local, nullptr))
+      if (int st = contact_resolve(c->contact, c->S, c->d, c->P, c->stream, &applied, &st_local,
+                                   nullptr))
         return st;
       if (applied)
-        if (int st = post_contact_update(c->d, c->P, s)) return st;
+        if (int st = post_contact_update(c->d, c->P, c->stream)) return st;
       st_local.impulse_applied = applied;
+      CS_HIP(hipEventRecord(b->cloth_done[i], c->stream));
       if (stats) stats[i] = st_local;
-      c->time += c->P.dt;
-    }
+      return kOk;
+    };
+    if (int st = parallel_for((int)n, kContactThreads, one)) return st;
+    for (size_t i = 0; i < n; i++)
+      if (!b->cl[i]->hflags[kFlagNonFinite]) CS_HIP(hipStreamWaitEvent(s, b->cloth_done[i], 0));
   }
   CS_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+extern "C" int cs_batch_profile_enable(cs_batch* p, int on) {
+  using namespace cs;
+  if (!p || on < CS_PROFILE_OFF || on > CS_PROFILE_SWEEP) return kBadArg;
+  Batch* b = HB(p);
+  b->prof.stream = b->stream;
+  b->prof.reset();
+  b->prof.on = on != CS_PROFILE_OFF;
+  b->prof.mask = on == CS_PROFILE_SWEEP ? 1u << kProfSweep : ~0u;
+  return kOk;
+}
+
+extern "C" int cs_batch_profile_read(cs_batch* p, int kind, double* total_ms, long long* launches) {
+  using namespace cs;
+  if (!p || kind < 0 || kind >= kNumProf) return kBadArg;
+  Batch* b = HB(p);
+  b->prof.harvest();
+  if (total_ms) *total_ms = b->prof.total_ms[kind];
+  if (launches) *launches = b->prof.count[kind];
   return kOk;
 }
 

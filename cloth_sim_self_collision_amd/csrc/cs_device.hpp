@@ -68,6 +68,7 @@ enum Flag : int {
   // short and corrupt every later reduction on the same scratch.)
   kSkip = 4,
   kTicketErr = 5,  // a reduction ticket was found nonzero after the solve
+  kFlagNonFinite = 6,  // the step's displacement has an infinite or NaN entry
   kNumFlag = 16
 };
 
@@ -150,6 +151,7 @@ struct DevCloth {
   int kpack;          // packed blocks per row of both sweeps: max(kl, ku), 0 = not packed
   int wmax;           // widest level of either sweep
   int sweep_variant;  // 0 automatic (quad sweep), 1 row-by-row
+  int pcg_variant;    // 0 automatic, 1 one launch per PCG operation, 2 one-workgroup solve
   const int* f_rci;  // {row, block count, backward slot, 0} per forward slot
   const int* b_rc;   // {row, block count} per backward slot
   // sweep metadata per slot: kpack 16-bit ring byte offsets of the column
@@ -204,7 +206,8 @@ enum ProfKind : int {
   kProfAssemble = 5,  // element kernels + gather assembly
   kProfContact = 6,   // refit, traversal, selection, impulses, zones
   kProfOther = 7,
-  kNumProf = 8
+  kProfPcgWg = 8,     // the whole PCG solve in one workgroup (k_pcg_wg)
+  kNumProf = 9
 };
 
 struct Prof {

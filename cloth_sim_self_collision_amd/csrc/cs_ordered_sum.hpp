@@ -384,6 +384,60 @@ __device__ inline double wave_walk(const double* buf, int len, double s,
   return s;
 }
 
+// The chain over tiles b0 .. wend - 1 on one wave (every lane holds s):
+// tiles whose summary applies are added 64 per step, a failing tile is
+// walked from an LDS copy in buf.  code/tot/pmin/pmax hold the tiles'
+// summaries at [b - b0]; `fallback` counts the walked tiles.
+__device__ __forceinline__ double chain_tiles(double s, int b0, int wend, const int* code,
+                                              const long long* tot, const long long* pmin,
+                                              const long long* pmax,
+                                              const double* __restrict__ t, int n, double* buf,
+                                              unsigned long long* stats, int& fallback) {
+  const int lane = threadIdx.x & 63;
+  const long long lo52 = (1LL << 52) + 1, hi53 = (1LL << 53) - 1;
+  int bb = b0;
+  while (bb < wend && !isnan(s)) {
+    // tiles applied from their summaries, 64 per step
+    while (bb < wend && sum_normal(s)) {
+      const int e = ilogb(s);
+      const long long S = (long long)ldexp(s, 52 - e);
+      const int k = bb + lane;
+      const bool in = k < wend;
+      const int kk = min(k, wend - 1) - b0;
+      const long long tt = in ? tot[kk] : 0;
+      const long long incl = wave_scan_dpp64(tt);
+      const long long base = S + incl - tt;  // S at the start of tile k
+      const long long lo = base + pmin[kk], hi = base + pmax[kk];
+      const bool inside = s > 0.0 ? (lo >= lo52 && hi <= hi53) : (lo >= -hi53 && hi <= -lo52);
+      const bool ok = in && code[kk] == sign_exp(s) && inside;
+      const unsigned long long bad = __ballot(!ok);
+      const int f = bad ? __ffsll((long long)bad) - 1 : 64;
+      if (f > 0) s = ldexp((double)(S + readlane64(incl, f - 1)), e - 52);
+      bb += f;
+      if (f < 64) break;
+    }
+    if (bb >= wend || isnan(s)) break;
+    // tile bb by the walker, from an LDS copy; len is 0 only for the one
+    // tile of an empty sum, which adds nothing (and must not load)
+    const int begin = bb * kSumTile, len = min(n, begin + kSumTile) - begin;
+    if (len > 0) {
+      // every load issued before the first store (past len: copies of the
+      // last addend, never added)
+#pragma unroll
+      for (int r = 0; r < kSumTile / 64; r++) {
+        const int i = r * 64 + lane;
+        buf[i] = t[begin + min(i, len - 1)];
+      }
+      wave_lds_sync();
+      s = wave_walk(buf, len, s, stats);
+      wave_lds_sync();  // the next walk rewrites buf
+    }
+    fallback++;
+    bb++;
+  }
+  return s;
+}
+
 // One reduction: the addends, their count, the scratch and an optional flag
 // that cancels it.  A source policy Src provides
 //   __device__ SumJob job() const        -- the reduction of grid row blockIdx.y
@@ -434,7 +488,6 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
   if (b >= nt) return;
   __shared__ SumShared sh;
   const double two62 = 4611686018427387904.0;  // 2^62
-  const long long lo52 = (1LL << 52) + 1, hi53 = (1LL << 53) - 1;
   {
     // approximate running sum at the start of this tile -> binade guess
     double p = 0.0;
@@ -517,46 +570,8 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
       }
     }
     wave_lds_sync();
-    int bb = b0;
-    while (bb < wend && !isnan(s)) {
-      // tiles applied from their summaries, 64 per step
-      while (bb < wend && sum_normal(s)) {
-        const int e = ilogb(s);
-        const long long S = (long long)ldexp(s, 52 - e);
-        const int k = bb + lane;
-        const bool in = k < wend;
-        const int kk = min(k, wend - 1) - b0;
-        const long long tot = in ? sh.tot[kk] : 0;
-        const long long incl = wave_scan_dpp64(tot);
-        const long long base = S + incl - tot;  // S at the start of tile k
-        const long long lo = base + sh.pmin[kk], hi = base + sh.pmax[kk];
-        const bool inside = s > 0.0 ? (lo >= lo52 && hi <= hi53) : (lo >= -hi53 && hi <= -lo52);
-        const bool ok = in && sh.code[kk] == sign_exp(s) && inside;
-        const unsigned long long bad = __ballot(!ok);
-        const int f = bad ? __ffsll((long long)bad) - 1 : 64;
-        if (f > 0) s = ldexp((double)(S + readlane64(incl, f - 1)), e - 52);
-        bb += f;
-        if (f < 64) break;
-      }
-      if (bb >= wend || isnan(s)) break;
-      // tile bb by the walker, from an LDS copy; len is 0 only for the one
-      // tile of an empty sum, which adds nothing (and must not load)
-      const int begin = bb * kSumTile, len = min(n, begin + kSumTile) - begin;
-      if (len > 0) {
-        // every load issued before the first store (past len: copies of the
-        // last addend, never added)
-#pragma unroll
-        for (int r = 0; r < kSumTile / 64; r++) {
-          const int i = r * 64 + lane;
-          sh.buf[i] = t[begin + min(i, len - 1)];
-        }
-        wave_lds_sync();
-        s = wave_walk(sh.buf, len, s, w.stats);
-        wave_lds_sync();  // the next walk rewrites buf
-      }
-      fallback++;
-      bb++;
-    }
+    s = chain_tiles(s, b0, wend, sh.code, sh.tot, sh.pmin, sh.pmax, t, n, sh.buf, w.stats,
+                    fallback);
   }
   if (lane == 0) {
     src.finish(s);
@@ -583,6 +598,115 @@ inline void launch_ordered_sum(const Src& src, int max_n, int jobs, hipStream_t 
   const dim3 grid(sum_tiles(max_n), jobs);
   hipLaunchKernelGGL(k_sum_tiles<Src>, grid, dim3(kSumBlock), 0, stream, src);
   hipLaunchKernelGGL(k_sum_exact<Src>, grid, dim3(kSumBlock), 0, stream, src);
+}
+
+// ---------------------------------------------------------------------------
+// The same exact sum computed by one workgroup, for kernels that run a whole
+// solve in one workgroup: the tile summaries are formed one tile per wave
+// (sixteen consecutive addends per lane instead of four per thread of a
+// block: the same inclusive integer prefixes, so the same summaries), then
+// the first wave chains them.  Every thread of the workgroup must call it.
+
+constexpr int kWgMaxTiles = 64;  // sums of at most 64 tiles (65,536 addends)
+
+struct WgSum {
+  double buf[kSumBuf];  // a walked tile
+  double tsum[kWgMaxTiles];
+  long long tot[kWgMaxTiles];
+  long long pmin[kWgMaxTiles];
+  long long pmax[kWgMaxTiles];
+  int code[kWgMaxTiles];
+  double result;
+};
+
+// two's-complement addition (the integer views wrap like the block version's)
+__device__ __forceinline__ long long add_wrap(long long a, long long b) {
+  return (long long)((unsigned long long)a + (unsigned long long)b);
+}
+
+__device__ __forceinline__ double wg_ordered_sum(const double* __restrict__ t, int n, WgSum& w) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int nt = sum_tiles(n);
+  // approximate tile sums (any order; they only steer the binade guesses)
+  for (int b = wid; b < nt; b += nw) {
+    double a = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < kSumTile / 64; j++) {
+      const int idx = b * kSumTile + j * 64 + lane;
+      if (idx < n) a += t[idx];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (lane == 0) w.tsum[b] = a;
+  }
+  __syncthreads();
+  // binade guess and integer summary of each tile, one wave per tile
+  constexpr int kPer = kSumTile / 64;
+  const double two62 = 4611686018427387904.0;  // 2^62
+  for (int b = wid; b < nt; b += nw) {
+    double p = 0.0;
+    for (int j = lane; j < b; j += 64) p += w.tsum[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+    int code = -1, scale = 0;
+    if (fabs(p) >= DBL_MIN && fabs(p) <= DBL_MAX) {
+      code = sign_exp(p);
+      scale = 52 - ilogb(p);
+    }
+    long long pref[kPer];
+    long long run = 0;
+    bool ok = code >= 0;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int idx = b * kSumTile + lane * kPer + j;
+      const double x = idx < n ? t[idx] : 0.0;
+      if (ok) {
+        const double q = ldexp(x, scale);
+        const double r = rint(q);
+        if (!(fabs(q) < two62) || fabs(q - r) == 0.5)
+          ok = false;
+        else
+          run = add_wrap(run, (long long)r);
+      }
+      pref[j] = run;
+    }
+    const bool all_ok = __ballot(!ok) == 0ull;
+    long long incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl = add_wrap(incl, y);
+    }
+    const long long off = add_wrap(incl, -run);
+    long long mn = LLONG_MAX, mx = LLONG_MIN;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const long long v = add_wrap(off, pref[j]);
+      mn = min(mn, v);
+      mx = max(mx, v);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = min(mn, (long long)__shfl_xor(mn, o, 64));
+      mx = max(mx, (long long)__shfl_xor(mx, o, 64));
+    }
+    const long long total = __shfl(incl, 63, 64);
+    if (lane == 0) {
+      w.code[b] = all_ok ? code : -1;
+      w.tot[b] = total;
+      w.pmin[b] = mn;
+      w.pmax[b] = mx;
+    }
+  }
+  __syncthreads();
+  if (wid == 0) {
+    int fallback = 0;
+    const double s = chain_tiles(0.0, 0, nt, w.code, w.tot, w.pmin, w.pmax, t, n, w.buf, nullptr,
+                                 fallback);
+    if (lane == 0) w.result = s;
+  }
+  __syncthreads();
+  return w.result;
 }
 
 }  // namespace cs

@@ -799,20 +799,13 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const Cuts& cuts, 
 // One workgroup per connected component (blockIdx.x): the matrix is block
 // diagonal over them, so their sweeps are independent.
 // K: the packed blocks per row (DevCloth::kpack)
-template <int K, class Sel>
-__global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const double* src_row,
-                                                                 double* dst, double* prod) {
-  const DevCloth c = sel.cloth();
-  const int comp = blockIdx.x;
-  if (c.flags[kSkip] || comp >= c.ncomp) return;  // batch grids fit the most components
-  if (!src_row) {
-    src_row = c.r;
-    dst = c.z;
-    prod = c.prod;
-  }
-  __shared__ double ring[3][kRingLen];  // + the dummy and the zero entry
-  __shared__ int cuts[kCutsLds + 1];
-  if (threadIdx.x < 3) ring[threadIdx.x][kRingZero] = 0.0;  // never written again
+// Forward then backward sweep of component `comp` by the whole workgroup:
+// the forward result into c.sb (backward-slot order), the backward result
+// into dst (row order).  ring: the LDS ring, its zero entry already set;
+// cuts: LDS for the component's sub-level cuts (kCutsLds + 1 entries).
+template <int K>
+__device__ __forceinline__ void quad_component(const DevCloth& c, int comp, double* dst,
+                                               double (*ring)[kRingLen], int* cuts) {
   const int f0 = c.qf_comp[comp], f1 = c.qf_comp[comp + 1];
   const bool fl = f1 - f0 <= kCutsLds;
   if (fl)
@@ -833,6 +826,23 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
   else
     quad_sweep<false, K>(c, CutsGlobal{c.qb_ptr}, b0, b1, c.sb, dst, ring);
   __syncthreads();
+}
+
+template <int K, class Sel>
+__global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const double* src_row,
+                                                                 double* dst, double* prod) {
+  const DevCloth c = sel.cloth();
+  const int comp = blockIdx.x;
+  if (c.flags[kSkip] || comp >= c.ncomp) return;  // batch grids fit the most components
+  if (!src_row) {
+    src_row = c.r;
+    dst = c.z;
+    prod = c.prod;
+  }
+  __shared__ double ring[3][kRingLen];  // + the dummy and the zero entry
+  __shared__ int cuts[kCutsLds + 1];
+  if (threadIdx.x < 3) ring[threadIdx.x][kRingZero] = 0.0;  // never written again
+  quad_component<K>(c, comp, dst, ring, cuts);
   if (!prod) return;
   if (c.ncomp == 1) {
     for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src_row[e] * dst[e];
@@ -927,16 +937,11 @@ __global__ void k_matvec(int np, const int* rowptr, const int* colind, const dou
 //           that holds z (ilu_sparse.cpp:670-672), scaling it by beta = 0
 //   prod  = p_new .* ap
 // iteration `it`: p_new = c.p[it & 1], p_old = c.p[(it + 1) & 1]
-template <class Sel>
-__global__ void k_spmv_dir(Sel sel, int it) {
-  const DevCloth c = sel.cloth();
-  const int first = it == 0;
-  const double* __restrict__ pold = pcg_dir(c, it + 1);
-  double* __restrict__ pnew = pcg_dir(c, it);
-  if (c.flags[kDone]) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.np) return;
-  const double beta = c.scal[kBeta];
+// one row of k_spmv_dir: p_new[i] (= z[i] or z[i] + beta p_old[i]), Ap[i]
+// and p_new[i] .* Ap[i]
+__device__ __forceinline__ void spmv_dir_row(const DevCloth& c, int i, bool first, double beta,
+                                             const double* __restrict__ pold,
+                                             double* __restrict__ pnew) {
   const double* z = c.z;
   double y0 = z[3 * i] * 0.0, y1 = z[3 * i + 1] * 0.0, y2 = z[3 * i + 2] * 0.0;
   for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; k++) {
@@ -981,6 +986,32 @@ __global__ void k_spmv_dir(Sel sel, int it) {
   c.prod[3 * i + 2] = x2 * y2;
 }
 
+template <class Sel>
+__global__ void k_spmv_dir(Sel sel, int it) {
+  const DevCloth c = sel.cloth();
+  const int first = it == 0;
+  const double* __restrict__ pold = pcg_dir(c, it + 1);
+  double* __restrict__ pnew = pcg_dir(c, it);
+  if (c.flags[kDone]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.np) return;
+  spmv_dir_row(c, i, first, c.scal[kBeta], pold, pnew);
+}
+
+// one entry of k_update_rx: r -= alpha Ap, dx += alpha p, r .* r, and r as
+// the next forward sweep's right-hand side
+__device__ __forceinline__ void update_rx_elem(const DevCloth& c, int e, double alpha,
+                                               const double* __restrict__ p) {
+  const double r = c.r[e] + -alpha * c.ap[e];
+  c.r[e] = r;
+  c.dx[e] = c.dx[e] + alpha * p[e];
+  c.prod2[e] = r * r;
+  if (c.fslot) {
+    const int i = e / 3;
+    c.f_dinv[fwd_rhs_index(c.np, c.fslot[i], e - 3 * i)] = r;  // the next sweep's input
+  }
+}
+
 // r -= alpha A p; x += alpha p; prod = r .* r  (matrix_square_sparse.cpp:327-336)
 // Also snapshots kDone into kSkip for the kernels that run beside the next
 // convergence test (no kernel changes kDone while this one runs).
@@ -993,15 +1024,7 @@ __global__ void k_update_rx(Sel sel, int it) {
   if (done) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * c.np) return;
-  const double alpha = c.scal[kAlpha];
-  const double r = c.r[e] + -alpha * c.ap[e];
-  c.r[e] = r;
-  c.dx[e] = c.dx[e] + alpha * p[e];
-  c.prod2[e] = r * r;
-  if (c.fslot) {
-    const int i = e / 3;
-    c.f_dinv[fwd_rhs_index(c.np, c.fslot[i], e - 3 * i)] = r;  // the next sweep's input
-  }
+  update_rx_elem(c, e, c.scal[kAlpha], p);
 }
 
 template <class Sel>
@@ -1009,8 +1032,12 @@ __global__ void k_pcg_init(Sel sel, double tol2) {
   const DevCloth c = sel.cloth();
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e == 0) {
-    for (int q = 0; q < kNumFlag; q++) c.flags[q] = 0;
-    c.flags[kFirstSing11] = c.np;
+    // the solve's flags (the factorisation's singular-pivot count stays)
+    c.flags[kDone] = 0;
+    c.flags[kIters] = 0;
+    c.flags[kSkip] = 0;
+    c.flags[kTicketErr] = 0;
+    c.flags[kFlagNonFinite] = 0;
     c.scal[kTol2] = tol2;
     c.scal[kBeta] = 0.0;
   }
@@ -1131,6 +1158,121 @@ __global__ void k_pcg_check(Sel sel) {
     c.flags[kTicketErr] = (*c.sum.ticket != 0u || *c.sum2.ticket != 0u) ? 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Solve_PCG (ilu_sparse.cpp:628-709) of one cloth in one workgroup: the
+// search direction and A p, the update of x and r, the preconditioner sweeps
+// and the four exact reductions of an iteration run back to back with
+// workgroup barriers between them, and the convergence test decides inside
+// the kernel.  Each operation is the one of the multi-launch solve (the same
+// row and entry functions, the same sweeps, sums equal to the left-to-right
+// loop), so x, r, the ratio and the count are identical.  One launch per
+// solve; in a batch every cloth iterates to its own convergence, none waits
+// for another.  The LDS holds the sweep ring and cuts and the sums' staging.
+
+constexpr int kWgThreads = 512;
+
+// z = M^-1 r (every component, forward then backward) and prod = r .* z
+template <int K>
+__device__ __forceinline__ void wg_precondition(const DevCloth& c, double (*ring)[kRingLen],
+                                                int* cuts) {
+  for (int comp = 0; comp < c.ncomp; comp++) quad_component<K>(c, comp, c.z, ring, cuts);
+  for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) c.prod[e] = c.r[e] * c.z[e];
+  __syncthreads();
+}
+
+template <int K, class Sel>
+__global__ __launch_bounds__(kWgThreads) void k_pcg_wg(Sel sel, double tol2, int maxit) {
+  const DevCloth c = sel.cloth();
+  __shared__ double ring[3][kRingLen];
+  __shared__ int cuts[kCutsLds + 1];
+  __shared__ WgSum ws;
+  const int tid = threadIdx.x, nth = blockDim.x, n3 = 3 * c.np;
+  if (tid < 3) ring[tid][kRingZero] = 0.0;  // never written again
+  // r = b, dx = 0, b .* b, r as the forward sweep's right-hand side (k_pcg_init)
+  for (int e = tid; e < n3; e += nth) {
+    const double b = c.b[e];
+    c.r[e] = b;
+    c.dx[e] = 0;
+    c.prod[e] = b * b;
+    const int i = e / 3;
+    c.f_dinv[fwd_rhs_index(c.np, c.fslot[i], e - 3 * i)] = b;
+  }
+  __syncthreads();
+  double conv = 0.0, inv_rr0 = 0.0, rz = 0.0, beta = 0.0;
+  int iters = 0;
+  bool done = false;
+  const double rr0 = wg_ordered_sum(c.prod, n3, ws);
+  if (rr0 < 1.0e-30) {
+    done = true;
+  } else {
+    inv_rr0 = 1.0 / rr0;
+    wg_precondition<K>(c, ring, cuts);
+    rz = wg_ordered_sum(c.prod, n3, ws);
+  }
+  for (int it = 0; it < maxit && !done; it++) {
+    const double* __restrict__ pold = pcg_dir(c, it + 1);
+    double* __restrict__ pnew = pcg_dir(c, it);
+    for (int i = tid; i < c.np; i += nth) spmv_dir_row(c, i, it == 0, beta, pold, pnew);
+    __syncthreads();
+    const double alpha = rz / wg_ordered_sum(c.prod, n3, ws);
+    for (int e = tid; e < n3; e += nth) update_rx_elem(c, e, alpha, pnew);
+    __syncthreads();
+    const double q = wg_ordered_sum(c.prod2, n3, ws) * inv_rr0;
+    if (q < tol2) {
+      conv = sqrt(q);
+      iters = it;
+      done = true;
+      break;
+    }
+    wg_precondition<K>(c, ring, cuts);
+    const double rzn = wg_ordered_sum(c.prod, n3, ws);
+    beta = rzn / rz;
+    rz = rzn;
+  }
+  if (!done) {
+    // cap reached: the achieved ratio (ilu_sparse.cpp:706-707)
+    for (int e = tid; e < n3; e += nth) c.prod[e] = c.r[e] * c.r[e];
+    __syncthreads();
+    conv = sqrt(wg_ordered_sum(c.prod, n3, ws) * inv_rr0);
+    iters = maxit;
+  }
+  if (tid == 0) {
+    c.scal[kConv] = conv;
+    c.scal[kInvRr0] = inv_rr0;
+    c.scal[kRz] = rz;
+    c.scal[kBeta] = beta;
+    c.scal[kTol2] = tol2;
+    c.flags[kIters] = iters;
+    c.flags[kDone] = done ? 1 : 0;
+    c.flags[kSkip] = 0;
+    c.flags[kTicketErr] = 0;
+    c.flags[kFlagNonFinite] = 0;
+  }
+}
+
+// the one-workgroup solve applies: the packed sweep with its slot-order
+// right-hand side, and reductions of at most kWgMaxTiles tiles
+bool pcg_wg_ok(const DevCloth& c) {
+  return quad_ok(c) && c.sweep_variant != 1 && c.fslot != nullptr &&
+         sum_tiles(3 * c.np) <= kWgMaxTiles;
+}
+
+template <class Sel>
+static void launch_pcg_wg(int kq, dim3 grid, const Sel& sel, double tol2, int maxit,
+                          hipStream_t s) {
+  const dim3 tb(kWgThreads);
+  switch (kq) {
+    case 1: hipLaunchKernelGGL((k_pcg_wg<1, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+    case 2: hipLaunchKernelGGL((k_pcg_wg<2, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+    case 3: hipLaunchKernelGGL((k_pcg_wg<3, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+    case 4: hipLaunchKernelGGL((k_pcg_wg<4, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+    case 5: hipLaunchKernelGGL((k_pcg_wg<5, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+    case 6: hipLaunchKernelGGL((k_pcg_wg<6, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+    case 7: hipLaunchKernelGGL((k_pcg_wg<7, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+    default: hipLaunchKernelGGL((k_pcg_wg<8, Sel>), grid, tb, 0, s, sel, tol2, maxit); break;
+  }
+}
+
 // x += dx, v = dx / dt (solve_internal_sparse.h:323-325)
 template <class Sel>
 __global__ void k_finish_internal(Sel sel) {
@@ -1141,6 +1283,10 @@ __global__ void k_finish_internal(Sel sel) {
   const double d = c.dx[e];
   c.x[e] = c.x[e] + d;
   c.v[e] = d / dt;
+  // the reference's ILU(0) can overflow on large meshes (its factors grow
+  // along the ordering); such a step is reported instead of handing inf/NaN
+  // positions to the contact pass
+  if (!isfinite(d)) c.flags[kFlagNonFinite] = 1;
 }
 
 // prescribed positions of swept vertices: x = rest + delta * s
@@ -1227,6 +1373,13 @@ int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStrea
 int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream_t s, int poll,
             Prof* prof, const PcgAux* aux) {
   const long n3 = 3L * c.np;
+  const bool wg = pcg_wg_ok(c) && c.pcg_variant == 2;
+  if (wg) {
+    ProfScope ps(prof, kProfPcgWg);
+    launch_pcg_wg(c.kpack, dim3(1), One{c, {}}, tol * tol, maxit, s);
+    CS_HIP(hipGetLastError());
+    return kOk;
+  }
   {
     ProfScope ps(prof, kProfOther);
     hipLaunchKernelGGL(k_pcg_init<One>, dim3(grid_for(n3)), dim3(kTB), 0, s, One{c, {}}, tol * tol);
@@ -1379,6 +1532,16 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
     hipLaunchKernelGGL(k_pack_factor<Many>, dim3(gp, B), dim3(kTB), 0, s, m);
   }
   CS_HIP(hipGetLastError());
+  if (b.wg) {
+    // one workgroup per cloth runs its whole solve: no cloth waits for another
+    {
+      ProfScope ps(prof, kProfPcgWg);
+      launch_pcg_wg(b.kpack, dim3(1, B), m, tol * tol, maxit, s);
+    }
+    hipLaunchKernelGGL(k_finish_internal<Many>, dim3(gv, B), dim3(kTB), 0, s, m);
+    CS_HIP(hipGetLastError());
+    return kOk;
+  }
   {
     ProfScope ps(prof, kProfOther);
     hipLaunchKernelGGL(k_pcg_init<Many>, dim3(gv, B), dim3(kTB), 0, s, m, tol * tol);

@@ -30,7 +30,10 @@ int launch_pack_factor(const DevCloth& c, hipStream_t s);
 int launch_matvec(const DevCloth& c, double alpha, const double* x, double beta, double* y,
                   hipStream_t s);
 int launch_ilu_apply(const DevCloth& c, const double* src, double* dst, hipStream_t s);
+// the one-workgroup PCG solve (k_pcg_wg) applies to this cloth
+bool pcg_wg_ok(const DevCloth& c);
 // Solve_PCG from dx = 0 with r = b; poll convergence every `poll` iterations
+// (the one-workgroup solve needs no poll)
 int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream_t s, int poll,
             Prof* prof = nullptr, const PcgAux* aux = nullptr);
 // x[v] = rest[v] + off for the swept vertices
@@ -51,6 +54,7 @@ struct BatchView {
   int max_fu_blocks;  // staged factorisation: blocks per row slot (0: some cloth cannot)
   bool piped;  // every cloth can use the packed (quad) sweep ...
   int kpack;   // ... with this many packed blocks per row
+  bool wg;     // every cloth takes the one-workgroup PCG solve (one workgroup per cloth)
 };
 // StepTime_InternalDynamicsILU for every cloth of the batch (start positions
 // saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); dgather/hgather hold n

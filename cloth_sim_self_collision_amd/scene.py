@@ -383,13 +383,16 @@ def drape_over_sphere(n: int = 32, *, seed: int = 0, jitter: float = 1.0e-3,
 
 def drop_on_obstacle(n: int, height: float, obstacle: Obstacle, *, ny: Optional[int] = None,
                      seed: int = 0, jitter: float = 1.0e-3, dt: float = 0.01,
-                     pcg_tol: float = 1.0e-6, pcg_maxit: int = 1000, name: str = "") -> Cloth:
-    """Cloth of n x ny vertices (ny = n: square), 1 m along x with square
-    cells, alternating diagonals, flat at z=height with U(+-jitter) z-jitter,
-    falling onto `obstacle`; clearance 0.01."""
+                     pcg_tol: float = 1.0e-6, pcg_maxit: int = 1000, name: str = "",
+                     side: Optional[float] = None) -> Cloth:
+    """Cloth of n x ny vertices (ny = n: square) with square cells,
+    alternating diagonals, flat at z=height with U(+-jitter) z-jitter, falling
+    onto `obstacle`; clearance 0.01.  Size: 1 m along x, or with `side` the
+    longer of the two sides that long."""
     ny = n if ny is None else ny
-    xyz, tri, quad, area = grid_cloth(n, ny, 1.0, (ny - 1) / (n - 1), z=height, alternating=True,
-                                      jitter=jitter, seed=seed)
+    h = (1.0 / (n - 1)) if side is None else side / (max(n, ny) - 1)
+    xyz, tri, quad, area = grid_cloth(n, ny, h * (n - 1), h * (ny - 1), z=height,
+                                      alternating=True, jitter=jitter, seed=seed)
     np_ = len(xyz) // 3
     return Cloth(xyz0=xyz.copy(), xyz=xyz.copy(), uvw=np.zeros_like(xyz), tri=tri, quad=quad,
                  bc=np.zeros(np_, dtype=np.int32), area=area, material=material_default(),
@@ -585,7 +588,7 @@ def from_config(cfg: dict):
     """A cloth (or, for "batch", a list of cloths) from a JSON scene object.
 
     "scene": "drape" (n, seed, jitter), "fold" (n, size, seed, jitter),
-    "drop" (n, ny, height, obstacle, seed, jitter, dt), "stack" (count, n,
+    "drop" (n, ny, height, obstacle, seed, jitter, dt, side), "stack" (count, n,
     gap, size, z0, seed, jitter, dt, clearance, obstacle), "batch" (count,
     seed, side_range, height_range), "mesh" (xyz, tri, fixed vertex ids,
     area, obstacle, dt, clearance) for any triangle mesh (bending stencils
@@ -604,7 +607,8 @@ def from_config(cfg: dict):
                                obstacle_from_config(cfg.get("obstacle",
                                                             {"kind": "sphere", "center": [0, 0, 0],
                                                              "radius": 0.3})),
-                               ny=cfg.get("ny"), dt=float(cfg.get("dt", 0.01)), **opt)
+                               ny=cfg.get("ny"), dt=float(cfg.get("dt", 0.01)),
+                               side=(float(cfg["side"]) if "side" in cfg else None), **opt)
     elif kind == "stack":
         extra = {k: float(cfg[k]) for k in ("gap", "size", "z0", "dt", "clearance") if k in cfg}
         ob = obstacle_from_config(cfg["obstacle"]) if "obstacle" in cfg else None

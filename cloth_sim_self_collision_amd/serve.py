@@ -2,20 +2,34 @@
 
 Requests (a cloth and a number of steps) arrive over time.  The server keeps
 every admitted simulation resident on the device (mesh, matrix, factor, BVH,
-state) and runs a tick loop: admit the requests that have arrived while the
-resident device footprint stays under the HBM budget, step all active cloths
-together with one ``cs_batch_step`` (shared kernel launches, one grid row per
-cloth), retire the finished ones and free their memory.  A request that
-arrives while others are on the GPU joins at the next tick; its steps overlap
-with theirs on the device.
+state) until it finishes.  Admitted requests join one of a few *lanes* by
+size (vertex count); each lane is a host thread that steps its resident
+cloths together, one ``cs_batch_step(1)`` per tick, and picks up newly
+admitted requests at the next tick.  Lanes run concurrently on their own
+streams, so small cloths tick at their own pace instead of waiting for the
+largest resident cloth, and the work of all lanes overlaps on the device.
+Within a lane every cloth whose size allows it runs its whole PCG solve in
+its own workgroup, so cloths of one lane do not wait for each other's solver
+iterations either.
+
+Admission is by resident device footprint: a request is admitted once it has
+arrived and its estimated footprint (BYTES_PER_VERTEX_ESTIMATE per vertex, or
+the measured ``cs_cloth_memory`` once created) fits the HBM budget beside the
+resident ones.  A request that can never fit, whose cloth cannot be created,
+whose step fails or whose solve turns non-finite gets an error result while
+the others continue.  Requests can be cancelled (``Server.cancel``) or evicted
+(``Server.evict``: its state and simulated time are saved, its device memory
+is released and it is admitted again later, continuing exactly where it
+stopped).  A creation that runs out of device memory while others are
+resident is retried after they retire.
 
 Each cloth's trajectory is bit-identical to stepping it alone (the batched
 kernels run each cloth's arithmetic unchanged), so "the same alone and in the
 mix" holds exactly, not only within a tolerance.
 
 Reported per request: the final state, the latency of each of its steps (the
-wall time of the tick that advanced it) and its queueing delay; per run:
-aggregate vertex-steps/s and p50/p99 step latency.
+wall time of its lane's tick that advanced it) and its queueing delay; per
+run: aggregate vertex-steps/s and p50/p99 step latency.
 
 Wire format (JSON lines, `python -m cloth_sim_self_collision_amd.serve`):
 
@@ -30,7 +44,8 @@ Wire format (JSON lines, `python -m cloth_sim_self_collision_amd.serve`):
             "n_prox": [...], "n_ccd": [[...] per step],
             "x": {"dtype": "<f8", "shape": [np, 3], "data": base64}, "v": ...}
            or {"id", "status": "error", "error": message} for a request
-           that cannot be served (bad record, unknown scene, precision).
+           that cannot be served (bad record, unknown scene, precision, too
+           large for the budget, a failed or non-finite step, cancelled).
 
 Results are written as requests finish, one line each, in completion order.
 """
@@ -40,11 +55,13 @@ import base64
 import dataclasses
 import json
 import sys
+import threading
 import time
 from typing import Callable, Iterable, List, Optional, TextIO
 
 import numpy as np
 
+from . import _lib
 from . import scene
 from .scene import Cloth, SplitMix64
 from .sim import DeviceBatch, DeviceCloth
@@ -52,6 +69,11 @@ from .sim import DeviceBatch, DeviceCloth
 # admission estimate before a cloth exists on the device: bytes per vertex of
 # the resident footprint (measured ~5.5 KB/vertex at 65k vertices, rounded up)
 BYTES_PER_VERTEX_ESTIMATE = 8192
+
+# lane boundaries by vertex count: [0, 2500), [2500, 21846), [21846, inf).
+# Cloths of the first two lanes run their PCG solve in one workgroup each
+# (3 np <= 65536); the last lane's cloths use one launch per PCG operation.
+DEFAULT_LANES = (2500, 21846)
 
 
 @dataclasses.dataclass
@@ -80,74 +102,91 @@ class Result:
 
 
 @dataclasses.dataclass
+class Failure:
+    rid: object
+    error: str
+
+
+@dataclasses.dataclass
 class _Active:
     req: Request
-    dev: DeviceCloth
+    dev: Optional[DeviceCloth] = None
     done: int = 0
     step_ms: List[float] = dataclasses.field(default_factory=list)
     pcg_iters: List[int] = dataclasses.field(default_factory=list)
     impulse_applied: List[bool] = dataclasses.field(default_factory=list)
     n_prox: List[int] = dataclasses.field(default_factory=list)
     n_ccd: List[list] = dataclasses.field(default_factory=list)
-    admitted_s: float = 0.0
+    admitted_s: float = -1.0
     memory: int = 0
+    saved: Optional[tuple] = None  # (x, v, t) of an evicted request
+    cancel: bool = False
+    evict: bool = False
 
 
-class Server:
-    """Continuous-batching tick loop over resident simulations."""
+class _Lane:
+    """One host thread stepping a set of resident cloths together."""
 
-    def __init__(self, hbm_budget_bytes: int, clock: Callable[[], float] = time.perf_counter,
-                 on_result: Optional[Callable[["Result"], None]] = None):
-        self.budget = int(hbm_budget_bytes)
-        self.clock = clock
-        self.on_result = on_result  # called as each request finishes
+    def __init__(self, server: "Server", index: int):
+        self.server = server
+        self.index = index
+        self.inbox: List[_Active] = []
+        self.active: List[_Active] = []
+        self.batch: Optional[DeviceBatch] = None
+        self.members: tuple = ()
+        self.ticks = 0
+        self.thread = threading.Thread(target=self._run, name=f"lane{index}", daemon=True)
 
-    def run(self, requests: List[Request], *, realtime: bool = True) -> dict:
-        """Serve `requests` (arrival times in seconds from now).  realtime:
-        a request is admitted only once its arrival time has passed; if not,
-        every request counts as arrived at once (throughput runs)."""
-        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        for r in pending:
-            if r.cloth.np_ * BYTES_PER_VERTEX_ESTIMATE > self.budget:
-                raise ValueError(f"request {r.rid} can never fit the HBM budget")
-        active: List[_Active] = []
-        results: List[Result] = []
-        t0 = self.clock()
-        resident = 0
-        batch: Optional[DeviceBatch] = None
-        members: tuple = ()
-        ticks = 0
+    def _close_batch(self):
+        if self.batch is not None:
+            self.batch.close()
+        self.batch = None
+        self.members = ()
+
+    def _run(self):
+        srv = self.server
         try:
-            while pending or active:
-                now = self.clock() - t0
-                # admission: arrived, in order, while the footprint fits
-                while pending and (not realtime or pending[0].arrival <= now):
-                    need = pending[0].cloth.np_ * BYTES_PER_VERTEX_ESTIMATE
-                    if active and resident + need > self.budget:
-                        break
-                    r = pending.pop(0)
-                    dev = DeviceCloth(r.cloth)
-                    a = _Active(req=r, dev=dev, admitted_s=self.clock() - t0)
-                    a.memory = max(dev.memory(), need)
-                    resident += a.memory
-                    active.append(a)
-                if not active:
-                    # idle until the next arrival
-                    if realtime and pending:
-                        time.sleep(max(0.0, min(0.05, pending[0].arrival - now)))
+            while True:
+                with srv.cv:
+                    while not self.inbox and not self.active and not srv.stopping:
+                        srv.cv.wait(0.05)
+                    if srv.stopping and not self.inbox and not self.active:
+                        return
+                    self.active.extend(self.inbox)
+                    self.inbox.clear()
+                    going = [a for a in self.active if a.cancel or a.evict]
+                if going:
+                    self._close_batch()  # before any member cloth is destroyed
+                    for a in going:
+                        self.active.remove(a)
+                        self._leave(a, "cancelled" if a.cancel else None)
+                if not self.active:
+                    self._close_batch()
                     continue
-                key = tuple(id(a.dev) for a in active)
-                if key != members:
-                    if batch is not None:
-                        batch.close()
-                    batch = DeviceBatch([a.dev for a in active])
-                    members = key
-                ts = self.clock()
-                stats = batch.step(1)
-                dt_ms = (self.clock() - ts) * 1e3
-                ticks += 1
-                still = []
-                for a, st in zip(active, stats):
+                key = tuple(id(a.dev) for a in self.active)
+                if key != self.members:
+                    self._close_batch()
+                    self.batch = DeviceBatch([a.dev for a in self.active])
+                    self.members = key
+                ts = srv.clock()
+                try:
+                    stats = self.batch.step(1)
+                except _lib.ClothSimError as e:
+                    # the whole tick failed: every member gets the error
+                    self._close_batch()
+                    srv.log(f"lane {self.index}: step failed: {e}")
+                    for a in list(self.active):
+                        self.active.remove(a)
+                        self._leave(a, f"step failed: {e}")
+                    continue
+                dt_ms = (srv.clock() - ts) * 1e3
+                self.ticks += 1
+                leaving = []
+                for a, st in zip(list(self.active), stats):
+                    if st["status"] != 0:
+                        leaving.append((a, "the solve produced an infinite or NaN displacement "
+                                           "(the ILU(0)-PCG iterate overflowed)", False))
+                        continue
                     a.done += 1
                     a.step_ms.append(dt_ms)
                     a.pcg_iters.append(st["pcg_iters"])
@@ -155,70 +194,269 @@ class Server:
                     a.n_prox.append(int(st["n_prox"]))
                     a.n_ccd.append([int(c) for c in st["n_ccd"][:st["n_ccd_pass"]]])
                     if a.done >= a.req.steps:
-                        x, v = a.dev.get_state()
-                        results.append(Result(
-                            rid=a.req.rid, xyz=x, uvw=v, step_ms=a.step_ms,
-                            queued_s=a.admitted_s - a.req.arrival if realtime else 0.0,
-                            admitted_s=a.admitted_s, finished_s=self.clock() - t0,
-                            vertices=a.req.cloth.np_, pcg_iters=a.pcg_iters,
-                            impulse_applied=a.impulse_applied, n_prox=a.n_prox,
-                            n_ccd=a.n_ccd))
-                        if self.on_result is not None:
-                            self.on_result(results[-1])
-                        resident -= a.memory
-                        a.dev.close()
-                    else:
-                        still.append(a)
-                if len(still) != len(active):
-                    active = still
-                    if batch is not None:
-                        batch.close()
-                    batch = None
-                    members = ()
+                        leaving.append((a, None, True))
+                if leaving:
+                    self._close_batch()  # before any member cloth is destroyed
+                    for a, err, fin in leaving:
+                        self.active.remove(a)
+                        self._leave(a, err, finished=fin)
+        except BaseException as e:  # noqa: BLE001 - reported, the server stops
+            with srv.cv:
+                srv.fatal = e
+                srv.cv.notify_all()
         finally:
-            if batch is not None:
-                batch.close()
-            for a in active:
-                a.dev.close()
-        wall = self.clock() - t0
-        lat = np.array([m for r in results for m in r.step_ms])
-        vsteps = sum(r.vertices * len(r.step_ms) for r in results)
-        return {"results": sorted(results, key=lambda r: r.rid), "wall_s": wall, "ticks": ticks,
+            self._close_batch()
+
+    def _leave(self, a: _Active, error: Optional[str], finished: bool = False):
+        """Retire a request from this lane: result, error, or eviction."""
+        srv = self.server
+        res = None
+        if finished:
+            x, v = a.dev.get_state()
+            res = Result(rid=a.req.rid, xyz=x, uvw=v, step_ms=a.step_ms,
+                         queued_s=max(0.0, a.admitted_s - a.req.arrival) if srv.realtime else 0.0,
+                         admitted_s=a.admitted_s, finished_s=srv.clock() - srv.t0,
+                         vertices=a.req.cloth.np_, pcg_iters=a.pcg_iters,
+                         impulse_applied=a.impulse_applied, n_prox=a.n_prox, n_ccd=a.n_ccd)
+        elif error is None:  # eviction: keep the state, release the device memory
+            x, v = a.dev.get_state()
+            a.saved = (x, v, a.dev.time)
+        a.dev.close()
+        a.dev = None
+        with srv.cv:
+            srv.resident -= a.memory
+            a.memory = 0
+            if res is not None:
+                srv._finish(a, res)
+            elif error is not None:
+                srv._fail(a.req.rid, error)
+                srv.live.discard(id(a))
+            else:
+                a.evict = False
+                srv.pending.append(a)
+                srv.pending.sort(key=lambda q: (q.req.arrival, str(q.req.rid)))
+            srv.cv.notify_all()
+
+
+class Server:
+    """Continuous batching over resident simulations in concurrent lanes.
+
+    hbm_budget_bytes: device memory the resident cloths may hold;
+    lanes: vertex-count boundaries of the lanes (one lane per interval; () is
+    a single lane that steps every resident cloth together)."""
+
+    def __init__(self, hbm_budget_bytes: int, clock: Callable[[], float] = time.perf_counter,
+                 on_result: Optional[Callable[["Result"], None]] = None,
+                 on_error: Optional[Callable[["Failure"], None]] = None,
+                 lanes: tuple = DEFAULT_LANES):
+        self.budget = int(hbm_budget_bytes)
+        self.clock = clock
+        self.on_result = on_result
+        self.on_error = on_error
+        self.bounds = tuple(lanes)
+        self.cv = threading.Condition()
+        self.pending: List[_Active] = []
+        self.results: List[Result] = []
+        self.failures: List[Failure] = []
+        self.live: set = set()
+        self.resident = 0
+        self.stopping = False
+        self.fatal: Optional[BaseException] = None
+        self.realtime = True
+        self.t0 = 0.0
+        self._by_rid: dict = {}
+
+    def log(self, msg: str) -> None:
+        print(f"[serve] {msg}", file=sys.stderr, flush=True)
+
+    def _lane_of(self, np_: int) -> int:
+        k = 0
+        while k < len(self.bounds) and np_ >= self.bounds[k]:
+            k += 1
+        return k
+
+    def _finish(self, a: _Active, res: Result):  # with cv held
+        self.results.append(res)
+        self.live.discard(id(a))
+        if self.on_result is not None:
+            self.on_result(res)
+
+    def _fail(self, rid, msg: str):  # with cv held
+        f = Failure(rid, msg)
+        self.failures.append(f)
+        if self.on_error is not None:
+            self.on_error(f)
+
+    def cancel(self, rid) -> None:
+        """Drop request rid: pending ones at once, resident ones at their
+        lane's next tick; it gets an error result "cancelled"."""
+        with self.cv:
+            for a in list(self.pending):
+                if a.req.rid == rid:
+                    self.pending.remove(a)
+                    self.live.discard(id(a))
+                    self._fail(rid, "cancelled")
+            a = self._by_rid.get(rid)
+            if a is not None and a.dev is not None:
+                a.cancel = True
+            self.cv.notify_all()
+
+    def evict(self, rid) -> None:
+        """Move resident request rid off the device at its lane's next tick
+        (state and time saved, memory released); it is admitted again like a
+        pending request and continues from the saved state."""
+        with self.cv:
+            a = self._by_rid.get(rid)
+            if a is not None and a.dev is not None:
+                a.evict = True
+            self.cv.notify_all()
+
+    def run(self, requests: List[Request], *, realtime: bool = True,
+            control: Optional[Callable[["Server", float], None]] = None) -> dict:
+        """Serve `requests` (arrival times in seconds from now).  realtime: a
+        request is admitted only once its arrival time has passed; if not,
+        every request counts as arrived at once (throughput runs).  control,
+        if given, is called from the admission loop with the elapsed time
+        (for cancel/evict)."""
+        self.realtime = realtime
+        self.pending = [_Active(req=r) for r in sorted(requests, key=lambda r: (r.arrival, str(r.rid)))]
+        self._by_rid = {a.req.rid: a for a in self.pending}
+        self.live = {id(a) for a in self.pending}
+        lanes = [_Lane(self, k) for k in range(len(self.bounds) + 1)]
+        self.t0 = self.clock()
+        for ln in lanes:
+            ln.thread.start()
+        try:
+            with self.cv:
+                for a in list(self.pending):
+                    need = a.req.cloth.np_ * BYTES_PER_VERTEX_ESTIMATE
+                    if need > self.budget:
+                        self.pending.remove(a)
+                        self.live.discard(id(a))
+                        self._fail(a.req.rid, f"needs about {need} bytes of device memory, more "
+                                              f"than the budget of {self.budget}")
+            while True:
+                now = self.clock() - self.t0
+                if control is not None:
+                    control(self, now)
+                with self.cv:
+                    if self.fatal is not None:
+                        raise self.fatal
+                    if not self.live:
+                        break
+                    cand = None
+                    for a in self.pending:
+                        if realtime and a.req.arrival > now:
+                            break
+                        need = max(a.memory, a.req.cloth.np_ * BYTES_PER_VERTEX_ESTIMATE)
+                        if self.resident + need <= self.budget or self.resident == 0:
+                            cand = a
+                        break  # admission in arrival order
+                    if cand is not None:
+                        self.pending.remove(cand)
+                        cand.memory = cand.req.cloth.np_ * BYTES_PER_VERTEX_ESTIMATE
+                        self.resident += cand.memory
+                if cand is None:
+                    with self.cv:
+                        nxt = [a.req.arrival - now for a in self.pending[:1] if realtime]
+                        self.cv.wait(max(0.0, min([0.02] + nxt)))
+                    continue
+                self._admit(cand, lanes)
+        finally:
+            with self.cv:
+                self.stopping = True
+                for ln in lanes:
+                    for a in ln.active + ln.inbox:
+                        a.cancel = True
+                self.cv.notify_all()
+            for ln in lanes:
+                ln.thread.join()
+        wall = self.clock() - self.t0
+        lat = np.array([m for r in self.results for m in r.step_ms])
+        vsteps = sum(r.vertices * len(r.step_ms) for r in self.results)
+        return {"results": sorted(self.results, key=lambda r: str(r.rid)),
+                "failures": list(self.failures), "wall_s": wall,
+                "ticks": sum(ln.ticks for ln in lanes), "lanes": [ln.ticks for ln in lanes],
                 "vertex_steps_per_s": vsteps / wall if wall > 0 else 0.0,
                 "p50_step_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
                 "p99_step_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0}
 
+    def _admit(self, a: _Active, lanes: List[_Lane]) -> None:
+        """Create the cloth of an admitted request (outside the lock) and hand
+        it to its lane; creation failures become errors or a later retry."""
+        try:
+            dev = DeviceCloth(a.req.cloth)
+            if a.saved is not None:
+                x, v, t = a.saved
+                dev.set_state(x, v)
+                dev.time = t
+                a.saved = None
+        except _lib.ClothSimError as e:
+            with self.cv:
+                self.resident -= a.memory
+                a.memory = 0
+                if e.status in (2, 3) and self.resident > 0:
+                    # device memory exhausted while others are resident: retry later
+                    self.pending.insert(0, a)
+                else:
+                    self.live.discard(id(a))
+                    self._fail(a.req.rid, f"cannot create the cloth: {e}")
+                self.cv.notify_all()
+            return
+        with self.cv:
+            mem = dev.memory()
+            self.resident += max(mem, a.memory) - a.memory
+            a.memory = max(mem, a.memory)
+            a.dev = dev
+            if a.admitted_s < 0:
+                a.admitted_s = self.clock() - self.t0
+            lanes[self._lane_of(a.req.cloth.np_)].inbox.append(a)
+            self.cv.notify_all()
+
 
 def run_plain(requests: List[Request]) -> dict:
     """The same requests one after another, each cloth stepped alone (the
-    plain GPU step), for comparison."""
+    plain GPU step), for comparison.  A request whose step fails is recorded
+    in "failures" and contributes no steps."""
     t0 = time.perf_counter()
-    results, lat = [], []
+    results, lat, failures = [], [], []
     vsteps = 0
-    for r in sorted(requests, key=lambda r: (r.arrival, r.rid)):
-        with DeviceCloth(r.cloth) as d:
-            ms = []
-            for _ in range(r.steps):
-                t1 = time.perf_counter()
-                d.step(1)
-                ms.append((time.perf_counter() - t1) * 1e3)
-            x, v = d.get_state()
+    for r in sorted(requests, key=lambda r: (r.arrival, str(r.rid))):
+        try:
+            with DeviceCloth(r.cloth) as d:
+                ms = []
+                for _ in range(r.steps):
+                    t1 = time.perf_counter()
+                    d.step(1)
+                    ms.append((time.perf_counter() - t1) * 1e3)
+                x, v = d.get_state()
+        except _lib.ClothSimError as e:
+            failures.append(Failure(r.rid, str(e)))
+            continue
         lat += ms
         vsteps += r.cloth.np_ * r.steps
         results.append((r.rid, x, v))
     wall = time.perf_counter() - t0
     lat = np.array(lat)
-    return {"results": results, "wall_s": wall, "vertex_steps_per_s": vsteps / wall,
-            "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99))}
+    return {"results": results, "failures": failures, "wall_s": wall,
+            "vertex_steps_per_s": vsteps / wall if wall > 0 else 0.0,
+            "p50_step_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
+            "p99_step_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0}
 
 
 def poisson_requests(count: int, rate_per_s: float, *, seed: int = 0, tri_range=(1.0e3, 2.0e5),
                      aspect_range=(1.0, 4.0), step_range=(50, 300), pcg_maxit: int = 1000) -> list:
     """The serving workload as wire requests: Poisson arrivals at `rate_per_s`;
     rectangular grids whose triangle count is log-uniform in `tri_range` and
-    aspect ratio uniform in `aspect_range`; steps uniform in `step_range`;
-    each dropped from a height in [0.3, 0.8] m onto its own sphere or box,
-    with the batch workload's physics (dt 0.01)."""
+    aspect ratio uniform in `aspect_range`, the longer side 1 m; steps uniform
+    in `step_range`; each dropped from a height in [0.3, 0.8] m onto its own
+    sphere or box, with the batch workload's physics (dt 0.01).
+
+    The longer side runs along the fast vertex index (vertex (ix, iy) is
+    ix * ny + iy, ny >= nx).  With the longer side along the slow index the
+    reference's natural-order ILU(0) overflows on the larger of these meshes
+    (489 x 151 vertices: a NaN solve; 151 x 489: finite), so this orientation
+    keeps every mesh of the range servable."""
     rng = SplitMix64(seed)
     t = 0.0
     out = []
@@ -227,17 +465,18 @@ def poisson_requests(count: int, rate_per_s: float, *, seed: int = 0, tri_range=
         tri = float(np.exp(np.log(tri_range[0]) +
                            (np.log(tri_range[1]) - np.log(tri_range[0])) * rng.uniform()))
         asp = aspect_range[0] + (aspect_range[1] - aspect_range[0]) * rng.uniform()
-        # 2 (nx-1)(ny-1) triangles with nx = aspect * ny
-        ny = max(3, int(round(np.sqrt(tri / (2.0 * asp)))) + 1)
-        nx = max(3, int(round(asp * (ny - 1))) + 1)
+        # 2 (n_short - 1)(n_long - 1) triangles with n_long - 1 = aspect (n_short - 1)
+        ns = max(3, int(round(np.sqrt(tri / (2.0 * asp)))) + 1)
+        nl = max(3, int(round(asp * (ns - 1))) + 1)
         steps = step_range[0] + int(rng.uniform() * (step_range[1] - step_range[0] + 1))
         h = 0.3 + 0.5 * rng.uniform()
         ob = ({"kind": "sphere", "center": [0.0, 0.0, 0.0], "radius": 0.3} if rng.uniform() < 0.5
               else {"kind": "box", "center": [0.0, 0.0, 0.0], "half": [0.25, 0.25, 0.25]})
         js = int(rng.next_u64() & 0x7FFFFFFF)
         out.append({"id": k, "steps": steps, "arrival": t,
-                    "scene": {"scene": "drop", "n": nx, "ny": ny, "height": h, "obstacle": ob,
-                              "seed": js, "pcg_maxit": pcg_maxit, "name": f"req{k}_{nx}x{ny}"}})
+                    "scene": {"scene": "drop", "n": ns, "ny": nl, "side": 1.0, "height": h,
+                              "obstacle": ob, "seed": js, "pcg_maxit": pcg_maxit,
+                              "name": f"req{k}_{ns}x{nl}"}})
     return out
 
 
@@ -333,16 +572,21 @@ def serve_jsonl(lines: Iterable[str], out: TextIO, *, hbm_budget_bytes: int,
         by_rid[key] = req
         reqs.append(dataclasses.replace(req, rid=key))
 
-    def emit(r: Result):
+    def emit(r: Result):  # called with the server's lock held: lines never interleave
         req = by_rid[r.rid]
         rec = result_record(dataclasses.replace(r, rid=req.rid), req)
         out.write(json.dumps(rec) + "\n")
         out.flush()
 
+    def emit_error(f: Failure):
+        out.write(json.dumps(error_record(by_rid[f.rid].rid, f.error)) + "\n")
+        out.flush()
+
     if not reqs:
-        return {"results": [], "wall_s": 0.0, "ticks": 0, "vertex_steps_per_s": 0.0,
-                "p50_step_ms": 0.0, "p99_step_ms": 0.0}
-    return Server(hbm_budget_bytes, on_result=emit).run(reqs, realtime=realtime)
+        return {"results": [], "failures": [], "wall_s": 0.0, "ticks": 0,
+                "vertex_steps_per_s": 0.0, "p50_step_ms": 0.0, "p99_step_ms": 0.0}
+    return Server(hbm_budget_bytes, on_result=emit, on_error=emit_error).run(reqs,
+                                                                           realtime=realtime)
 
 
 def main(argv=None) -> int:
@@ -379,7 +623,8 @@ def main(argv=None) -> int:
             fin.close()
         if fout is not sys.stdout:
             fout.close()
-    print(json.dumps({k: v for k, v in summary.items() if k != "results"}), file=sys.stderr)
+    print(json.dumps({k: v for k, v in summary.items() if k not in ("results", "failures")}),
+          file=sys.stderr)
     return 0
 
 

@@ -126,6 +126,10 @@ class DeviceCloth:
         else:
             check(_lib.load().cs_set_obstacle_values(self.h, dp(f64(pd)), dp(f64(n))))
 
+    # cs_set_option options (results are identical for every value)
+    OPT_SWEEP = 1  # 0 automatic, 1 row-by-row CSR sweep
+    OPT_PCG = 2    # 0 automatic, 1 one launch per PCG operation, 2 one-workgroup solve
+
     def set_option(self, option: int, value: int):
         check(_lib.load().cs_set_option(self.h, option, value))
 
@@ -171,7 +175,7 @@ class DeviceCloth:
 
     # -- timing
     PROFILE_KINDS = ("ilu_sweep", "pcg_reduce", "spmv", "vector_update", "ilu_factor",
-                     "assemble", "contact", "other")
+                     "assemble", "contact", "other", "pcg_workgroup")
 
     def profile(self, on: bool = True, sweep_only: bool = False) -> None:
         """Per-kernel-class HIP-event timing (cs_profile_enable): every class
@@ -362,6 +366,20 @@ class DeviceBatch:
         st = (_lib.StepStats * len(self.cloths))()
         check(_lib.load().cs_batch_step(self.h, n, C.cast(st, C.c_void_p)))
         return [s.as_dict() for s in st]
+
+    def profile(self, on: bool = True) -> None:
+        """Per-kernel-class HIP-event timing of the batch's shared launches."""
+        check(_lib.load().cs_batch_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        """{class: (total ms, launches)} since profiling was enabled."""
+        out = {}
+        for k, name in enumerate(DeviceCloth.PROFILE_KINDS):
+            ms = C.c_double()
+            n = C.c_longlong()
+            check(_lib.load().cs_batch_profile_read(self.h, k, C.byref(ms), C.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
 
 
 def bandwidth_probe(n_doubles: int = 1 << 26, reps: int = 20) -> float:

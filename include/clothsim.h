@@ -93,6 +93,9 @@ typedef struct {
   int n_ccd[5];           /* contacts per CCD pass */
   int n_riz_pass;         /* impact-zone passes run (<= 100) */
   int n_riz_vertices;     /* vertices in impact zones at the end */
+  int status;             /* CS_OK, or why this cloth's step stopped: CS_ERR_NONFINITE when
+                           * the solve produced an infinite or NaN displacement (the
+                           * contact pass is then skipped) */
 } cs_step_stats;
 
 const char* cs_last_error(void);
@@ -142,7 +145,11 @@ int cs_post_contact_update(cs_cloth* c);
 int cs_update_velocity(cs_cloth* c);
 /* n full StepTime iterations; stats of the last step (may be NULL).  Before
  * each step the swept vertices (cs_set_sweep) are placed at
- * rest + delta * clamp((t + dt) / duration, 0, 1); t then advances by dt. */
+ * rest + delta * clamp((t + dt) / duration, 0, 1); t then advances by dt.
+ * A step whose solve yields an infinite or NaN displacement (the reference's
+ * ILU(0) overflows on some large meshes) stops the loop with
+ * CS_ERR_NONFINITE before its contact pass; the state then holds that step's
+ * internal result. */
 int cs_step(cs_cloth* c, int n, cs_step_stats* stats);
 /* kinematic boundary: n vertices (which should carry bc = 1) moved linearly by
  * delta[3] over `duration` seconds, then held.  The reference has only fixed
@@ -163,19 +170,39 @@ typedef struct cs_batch cs_batch;
 int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out);
 int cs_batch_destroy(cs_batch* b);
 int cs_batch_size(cs_batch* b, int* n);
+/* per-kernel-class timing of a batch's shared launches with HIP events on the
+ * batch's stream (kinds and modes as cs_profile_enable; the contact passes,
+ * on the cloths' own streams, are not included) */
+int cs_batch_profile_enable(cs_batch* b, int on);
+int cs_batch_profile_read(cs_batch* b, int kind, double* total_ms, long long* launches);
 /* nsteps StepTime iterations of every cloth; stats[i] (n entries, may be
- * NULL) receives cloth i's last step */
+ * NULL) receives cloth i's last step.  The contact passes of the cloths run
+ * concurrently, each on its cloth's stream from a host worker thread.  A
+ * cloth whose solve is not finite gets stats[i].status = CS_ERR_NONFINITE
+ * and no contact pass; the others are unaffected and the call returns CS_OK
+ * (remove such a cloth from the batch: it stays non-finite). */
 int cs_batch_step(cs_batch* b, int nsteps, cs_step_stats* stats);
 
 /* implementation choices (results are identical for every value):
  * CS_OPT_SWEEP: 0 automatic (the packed four-lanes-per-row sweep when
- *               the mesh allows it), 1 row-by-row CSR sweep */
+ *               the mesh allows it), 1 row-by-row CSR sweep
+ * CS_OPT_PCG:   0 automatic (= 1), 1 one kernel launch per PCG operation
+ *               with a host poll every 8 iterations, 2 the whole solve in one
+ *               workgroup (one launch; needs the packed sweep and
+ *               3 np <= 65536; a batch uses it when all its cloths select it,
+ *               one workgroup per cloth, each iterating to its own
+ *               convergence).  Measured: 2 is not faster than 1 for a 32^2
+ *               cloth and slower for a 64-cloth batch of 24^2..128^2 cloths
+ *               at the iteration cap (each cloth's SpMV and reductions then
+ *               run on one CU) */
 #define CS_OPT_SWEEP 1
+#define CS_OPT_PCG 2
 int cs_set_option(cs_cloth* c, int option, int value);
 
 /* per-kernel-class timing with HIP events on the cloth's stream:
  * 0 ILU sweeps, 1 PCG reductions, 2 SpMV, 3 vector updates, 4 ILU factor,
- * 5 assembly, 6 contact pass, 7 other.  Enabling resets the totals.
+ * 5 assembly, 6 contact pass, 7 other, 8 the one-workgroup PCG solve (all
+ * of 0-3 in one kernel).  Enabling resets the totals.
  * on = CS_PROFILE_OFF, CS_PROFILE_ALL (every class, plus the reduction
  * counters of cs_reduction_stats) or CS_PROFILE_SWEEP (the ILU sweeps only:
  * two events per sweep launch, no counters, for timing that barely

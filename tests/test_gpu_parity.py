@@ -213,6 +213,7 @@ def test_profile_modes(sim_mod):
     cl = scene.fold_sweep(24, size=0.3)
     cl.pcg_maxit = 20
     with sim_mod.DeviceCloth(cl) as d:
+        d.set_option(2, 1)  # one launch per PCG operation: each class has its launches
         d.profile(True, sweep_only=True)
         d.step(1)
         p = d.profile_read()
@@ -224,6 +225,11 @@ def test_profile_modes(sim_mod):
         p = d.profile_read()
         assert p["ilu_sweep"][1] > 0 and p["pcg_reduce"][1] > 0 and p["spmv"][1] > 0
         assert d.reduction_stats()["sums"] > 0
+        d.set_option(2, 2)  # the one-workgroup solve is timed as one class
+        d.profile(True)
+        d.step(1)
+        p = d.profile_read()
+        assert p["pcg_workgroup"][1] == 1 and p["ilu_sweep"][1] == 0 and p["pcg_reduce"][1] == 0
         d.profile(False)
         with pytest.raises(Exception):
             from cloth_sim_self_collision_amd import _lib
@@ -303,10 +309,13 @@ def test_stages_bitwise(sim_mod, oracle_mod, name, steps, keep):
             assert np.array_equal(d.ilu_solve(vec), o.ilu_solve(ofd, ofc, vec)), k
             y = np.cos(np.arange(3 * cl.np_) * 0.11)
             assert np.array_equal(d.matvec(1.0, vec, 0.5, y), o.matvec(odia, ocrs, 1.0, vec, 0.5, y)), k
-            xs, rs, conv, it = d.pcg_solve(b, cl.pcg_tol, cl.pcg_maxit)
             oxs, ors, oconv, oit = o.pcg(odia, ocrs, ofd, ofc, ob, cl.pcg_tol, cl.pcg_maxit)
-            assert it == oit and conv == oconv, k
-            assert np.array_equal(xs, oxs) and np.array_equal(rs, ors), k
+            for pcg in (1, 2):  # one launch per operation; the one-workgroup solve
+                d.set_option(2, pcg)
+                xs, rs, conv, it = d.pcg_solve(b, cl.pcg_tol, cl.pcg_maxit)
+                assert it == oit and conv == oconv, (k, pcg)
+                assert np.array_equal(xs, oxs) and np.array_equal(rs, ors), (k, pcg)
+            d.set_option(2, 0)
             px = d.contacts_proximity(x, cl.clearance)
             assert np.array_equal(px, o.prox_set(x, cl.clearance)), k
             vv = v * 3.0
@@ -336,17 +345,21 @@ def test_velocity_update_and_host_obstacle_values(sim_mod, oracle_mod):
         assert (pd + cl.clearance > 0).any()  # the obstacle term is active
 
 
-@pytest.mark.parametrize("name,steps,variant", [("drape32", 100, 0), ("refdemo0", 60, 0),
-                                                ("refdemo1", 60, 0), ("fold24", 60, 0),
-                                                ("drape32", 60, 1), ("fold24", 40, 1),
-                                                ("box24", 60, 0), ("bar24", 70, 0),
-                                                ("stack16", 60, 0)])
-def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant):
+# (scene, steps, sweep variant, PCG variant): sweep 0 = packed quad sweep,
+# 1 = row-by-row; PCG 0 = automatic (one launch per PCG operation), 2 = the
+# whole solve in one workgroup
+@pytest.mark.parametrize("name,steps,variant,pcg", [
+    ("drape32", 100, 0, 0), ("drape32", 100, 0, 2), ("refdemo0", 60, 0, 0),
+    ("refdemo1", 60, 0, 2), ("fold24", 60, 0, 0), ("fold24", 60, 0, 2),
+    ("drape32", 60, 1, 0), ("fold24", 40, 1, 0), ("box24", 60, 0, 0), ("bar24", 70, 0, 2),
+    ("stack16", 60, 0, 0), ("stack16", 60, 0, 2)])
+def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant, pcg):
     cl = _scenes()[name]
     o = oracle_mod.Oracle(cl, "port")
     x, v = cl.xyz.copy(), cl.uvw.copy()
     with sim_mod.DeviceCloth(cl) as d:
         d.set_option(1, variant)
+        d.set_option(2, pcg)
         for k in range(steps):
             x_before = x.copy()
             scene.apply_sweep(cl, x, d.time + cl.dt)
@@ -392,16 +405,21 @@ def test_cpp_shim_step_loop_matches_reference(sim_mod, oracle_mod, tmp_path):
     assert np.array_equal(res[:3 * cl.np_], x) and np.array_equal(res[3 * cl.np_:], v)
 
 
-def test_batch_matches_each_cloth_alone(sim_mod, oracle_mod):
-    """Cloths of different sizes stepped by shared launches (cs_batch_step):
-    every cloth follows its own oracle trajectory bit for bit, including a
-    swept fold and PCG runs that stop at different iterations."""
+@pytest.mark.parametrize("pcg", [0, 2])
+def test_batch_matches_each_cloth_alone(sim_mod, oracle_mod, pcg):
+    """Cloths of different sizes stepped together (cs_batch_step): every cloth
+    follows its own oracle trajectory bit for bit, including a swept fold and
+    PCG runs that stop at different iterations; with shared launches per PCG
+    operation (pcg 0) and with the one-workgroup solve (one workgroup per
+    cloth, pcg 2)."""
     cls = [scene.drape_over_sphere(16), scene.fold_sweep(24, size=0.3), scene.drape_over_sphere(32),
            scene.reference_demo(0)]
     for cl in cls:
         cl.pcg_tol, cl.pcg_maxit = 1e-6, 1000
     steps = 25
     devs = [sim_mod.DeviceCloth(cl) for cl in cls]
+    for d in devs:
+        d.set_option(2, pcg)
     try:
         with sim_mod.DeviceBatch(devs) as b:
             st = b.step(steps)
@@ -477,10 +495,11 @@ def test_cloth_from_setup_cache_steps_identically(sim_mod, tmp_path):
     assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
 
 
-def test_wire_served_results_equal_alone(sim_mod):
+def test_wire_served_results_equal_alone(sim_mod, oracle_mod):
     """Serving over the JSON-lines wire format: each decoded final state equals
     stepping that request's cloth alone, bit for bit (a swept fold, a bar over
-    the ground, a drape); an unservable request gets an error record."""
+    the ground, a drape), and the drape's decoded state equals the oracle's;
+    an unservable request gets an error record."""
     import io
     import json
     from cloth_sim_self_collision_amd import serve
@@ -510,6 +529,15 @@ def test_wire_served_results_equal_alone(sim_mod):
         assert np.array_equal(serve.decode_array(rec["x"]).reshape(-1), x), r["id"]
         if "v" in r.get("return", []):
             assert np.array_equal(serve.decode_array(rec["v"]).reshape(-1), v), r["id"]
+    # the decoded drape against the oracle itself
+    cl = scene.from_config(reqs[0]["scene"])
+    o = oracle_mod.Oracle(cl, "port")
+    ox, ov = cl.xyz.copy(), cl.uvw.copy()
+    for _ in range(reqs[0]["steps"]):
+        ox, ov, *_ = o.step(ox, ov)
+    rec = recs[json.dumps("drape")]
+    assert np.array_equal(serve.decode_array(rec["x"]).reshape(-1), ox)
+    assert np.array_equal(serve.decode_array(rec["v"]).reshape(-1), ov)
 
 
 @pytest.mark.parametrize("name,maxit", [("fold256", 30), ("stack4x512", 3)])
@@ -541,3 +569,51 @@ def test_full_size_configs(sim_mod, oracle_mod, name, maxit):
             gx, gv = d.get_state()
         assert st["pcg_iters"] == it and st["impulse_applied"] == imp
         assert np.array_equal(gx, x) and np.array_equal(gv, v)
+
+
+def test_impact_zone_vertex_count_reported(sim_mod):
+    """Steps that run rigid impact zones report how many vertices the zones
+    hold (the drape over the sphere reaches them within 100 steps)."""
+    cl = scene.drape_over_sphere(32)
+    riz = []
+    with sim_mod.DeviceCloth(cl) as d:
+        for _ in range(100):
+            st = d.step(1)
+            if st["n_riz_pass"] > 0:
+                riz.append((st["n_riz_pass"], st["n_riz_vertices"]))
+    assert riz, "no step reached the impact zones"
+    assert all(nv > 0 for _, nv in riz), riz
+
+
+def test_nonfinite_solve_is_reported_not_stepped(sim_mod, oracle_mod):
+    """A mesh on which the reference's ILU(0) overflows (489 x 151 vertices,
+    the fast vertex index along the short side): the internal solve is NaN,
+    as in the oracle, and the step returns CS_ERR_NONFINITE before its
+    contact pass; in a batch only that cloth is flagged and the other steps
+    exactly as alone."""
+    from cloth_sim_self_collision_amd import _lib
+    bad = scene.drop_on_obstacle(489, 0.5, scene.sphere((0, 0, 0), 0.3), ny=151, seed=3,
+                                 pcg_maxit=3)
+    o = oracle_mod.Oracle(bad, "port")
+    _, ov, oconv, _ = o.step_internal(bad.xyz.copy(), bad.uvw.copy())
+    assert not np.isfinite(ov).all() and np.isnan(oconv)
+    o.close()
+    with sim_mod.DeviceCloth(bad) as d:
+        with pytest.raises(_lib.ClothSimError) as e:
+            d.step(1)
+        assert e.value.status == 6  # CS_ERR_NONFINITE
+    good = scene.drape_over_sphere(16)
+    good.pcg_maxit = bad.pcg_maxit = 20  # a batch shares the PCG settings
+    with sim_mod.DeviceCloth(good) as d:
+        d.step(2)
+        want = d.get_state()
+    devs = [sim_mod.DeviceCloth(good), sim_mod.DeviceCloth(bad)]
+    try:
+        with sim_mod.DeviceBatch(devs) as b:
+            st = b.step(2)
+        assert st[0]["status"] == 0 and st[1]["status"] == 6
+        gx, gv = devs[0].get_state()
+        assert np.array_equal(gx, want[0]) and np.array_equal(gv, want[1])
+    finally:
+        for dv in devs:
+            dv.close()

@@ -1,0 +1,184 @@
+"""The serving loop's host logic (admission by HBM budget, lanes, per-request
+errors, cancel, evict and resume) with the device replaced by a stand-in
+whose state after k steps is a known function of k, so that every result is
+checkable without a GPU.  The device-backed server is tested in
+tests/test_gpu_parity.py."""
+import io
+import json
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from cloth_sim_self_collision_amd import _lib, scene, serve
+
+
+class FakeCloth:
+    """Device stand-in: its state after k steps is (x0 + k, v0 - k)."""
+    created = []
+    fail_create = set()   # cloth names whose creation fails
+    oom_once = set()      # cloth names whose first creation runs out of memory
+    nonfinite_at = {}     # cloth name -> step index whose solve turns non-finite
+    lock = threading.Lock()
+
+    def __init__(self, cloth):
+        name = cloth.name
+        with FakeCloth.lock:
+            if name in FakeCloth.fail_create:
+                raise _lib.ClothSimError(1, "bad mesh")
+            if name in FakeCloth.oom_once:
+                FakeCloth.oom_once.discard(name)
+                raise _lib.ClothSimError(3, "out of memory")
+            FakeCloth.created.append(name)
+        self.cloth = cloth
+        self.np_ = cloth.np_
+        self.k = 0
+        self.time = 0.0
+        self.closed = False
+
+    def memory(self):
+        return self.np_ * 4096
+
+    def get_state(self):
+        return self.cloth.xyz + self.k, self.cloth.uvw - self.k
+
+    def set_state(self, x, v):
+        self.k = int(round(float(x[0] - self.cloth.xyz[0])))
+
+    def close(self):
+        self.closed = True
+
+
+class FakeBatch:
+    def __init__(self, devs):
+        assert all(not d.closed for d in devs)
+        self.devs = list(devs)
+
+    def step(self, n):
+        time.sleep(0.001)  # a tick takes time, so that control actions land mid-run
+        out = []
+        for d in self.devs:
+            assert not d.closed
+            bad = FakeCloth.nonfinite_at.get(d.cloth.name)
+            st = {"status": 0, "pcg_iters": 3, "impulse_applied": False, "n_prox": 0,
+                  "n_ccd_pass": 0, "n_ccd": [0] * 5}
+            if bad is not None and d.k >= bad:
+                st["status"] = 6
+            else:
+                d.k += 1
+                d.time += d.cloth.dt
+            out.append(st)
+        return out
+
+    def close(self):
+        pass
+
+
+@pytest.fixture
+def fake(monkeypatch):
+    monkeypatch.setattr(serve, "DeviceCloth", FakeCloth)
+    monkeypatch.setattr(serve, "DeviceBatch", FakeBatch)
+    FakeCloth.created = []
+    FakeCloth.fail_create = set()
+    FakeCloth.oom_once = set()
+    FakeCloth.nonfinite_at = {}
+    return FakeCloth
+
+
+def _req(rid, n, steps, arrival=0.0, name=None):
+    cl = scene.drape_over_sphere(n)
+    cl.name = name or f"c{rid}"
+    return serve.Request(rid, cl, steps=steps, arrival=arrival)
+
+
+def _check(res, req):
+    assert len(res.step_ms) == req.steps
+    assert np.array_equal(res.xyz, req.cloth.xyz + req.steps)
+    assert np.array_equal(res.uvw, req.cloth.uvw - req.steps)
+
+
+def test_requests_in_several_lanes_all_finish(fake):
+    reqs = [_req(0, 8, 5), _req(1, 60, 3, 0.01), _req(2, 20, 7, 0.02), _req(3, 160, 2, 0.03)]
+    out = serve.Server(1 << 40, lanes=(500, 10000)).run(reqs, realtime=True)
+    assert not out["failures"]
+    by = {r.rid: r for r in out["results"]}
+    for q in reqs:
+        _check(by[q.rid], q)
+    assert sum(1 for t in out["lanes"] if t > 0) == 3  # three lanes ticked
+
+
+def test_budget_and_creation_failures_are_per_request(fake):
+    fake.fail_create.add("broken")
+    reqs = [_req(0, 8, 4), _req(1, 300, 2, name="huge"), _req(2, 8, 3, name="broken"),
+            _req(3, 10, 2)]
+    budget = 200 * 200 * serve.BYTES_PER_VERTEX_ESTIMATE  # request 1 (90,000 vertices) cannot fit
+    out = serve.Server(budget).run(reqs, realtime=False)
+    fails = {f.rid: f.error for f in out["failures"]}
+    assert set(fails) == {1, 2}
+    assert "budget" in fails[1] and "bad mesh" in fails[2]
+    assert sorted(r.rid for r in out["results"]) == [0, 3]
+
+
+def test_tight_budget_queues_and_out_of_memory_retries(fake):
+    fake.oom_once.add("c1")
+    reqs = [_req(k, 12, 3 + k) for k in range(4)]
+    budget = 2 * 12 * 12 * serve.BYTES_PER_VERTEX_ESTIMATE + 1  # two resident cloths at a time
+    out = serve.Server(budget, lanes=()).run(reqs, realtime=False)
+    assert not out["failures"]
+    for q in reqs:
+        _check({r.rid: r for r in out["results"]}[q.rid], q)
+    assert fake.created.count("c1") == 1  # the out-of-memory creation was retried
+
+
+def test_nonfinite_step_fails_only_that_request(fake):
+    fake.nonfinite_at["c1"] = 2
+    reqs = [_req(0, 10, 5), _req(1, 10, 5), _req(2, 10, 5)]
+    out = serve.Server(1 << 40, lanes=()).run(reqs, realtime=False)
+    assert [f.rid for f in out["failures"]] == [1] and "NaN" in out["failures"][0].error
+    assert sorted(r.rid for r in out["results"]) == [0, 2]
+
+
+def test_cancel_and_evict_resume(fake):
+    reqs = [_req(0, 10, 400), _req(1, 10, 300), _req(2, 10, 6)]
+    state = {"evicted": False, "cancelled": False}
+
+    def control(srv, now):
+        done = {r.rid for r in srv.results}
+        if 2 in done and not state["evicted"]:
+            srv.evict(1)
+            state["evicted"] = True
+        if 2 in done and not state["cancelled"]:
+            srv.cancel(0)
+            state["cancelled"] = True
+
+    out = serve.Server(1 << 40, lanes=()).run(reqs, realtime=False, control=control)
+    assert [(f.rid, f.error) for f in out["failures"]] == [(0, "cancelled")]
+    by = {r.rid: r for r in out["results"]}
+    _check(by[1], reqs[1])  # evicted, admitted again, continued from its saved state
+    _check(by[2], reqs[2])
+    assert fake.created.count("c1") == 2
+
+
+def test_wire_run_mixes_error_and_result_lines(fake):
+    good = {"scene": "drape", "n": 8}
+    lines = [json.dumps({"id": "a", "scene": good, "steps": 3}),
+             json.dumps({"id": "big", "scene": {"scene": "drape", "n": 400}, "steps": 2}),
+             json.dumps({"id": "b", "scene": good, "steps": 2, "return": ["x", "v"]})]
+    out = io.StringIO()
+    serve.serve_jsonl(lines, out, hbm_budget_bytes=64 * 64 * serve.BYTES_PER_VERTEX_ESTIMATE,
+                      realtime=False)
+    recs = {r["id"]: r for r in map(json.loads, out.getvalue().splitlines())}
+    assert recs["big"]["status"] == "error" and "budget" in recs["big"]["error"]
+    assert recs["a"]["status"] == "ok" and recs["a"]["steps"] == 3
+    assert recs["b"]["status"] == "ok" and "v" in recs["b"]
+    x = serve.decode_array(recs["b"]["x"]).reshape(-1)
+    assert np.array_equal(x, scene.from_config(good).xyz + 2)
+
+
+def test_poisson_requests_keep_the_long_side_on_the_fast_index():
+    for r in serve.poisson_requests(40, 5.0, seed=1):
+        sc = r["scene"]
+        assert sc["ny"] >= sc["n"] and sc["side"] == 1.0
+        nt = 2 * (sc["n"] - 1) * (sc["ny"] - 1)
+        assert 700 <= nt <= 2.2e5
