@@ -111,6 +111,7 @@ SIGNATURES = {
     "cs_element_contact": (C.c_int, [C.c_int, _D, C.c_double, C.c_double, C.c_int, _D, _D, _D]),
     "cs_selftest_divsqrt": (C.c_int, [C.c_int, _D, _D, _D]),
     "cs_selftest_ordered_sum": (C.c_int, [C.c_int, _D, _D]),
+    "cs_ordered_sum_noalloc": (C.c_int, [C.c_int, _V, _V, _V]),
     "cs_inspect_setup": (C.c_int, [C.POINTER(Mesh), _I, _I, _I, _I, _I]),
 }
 

@@ -22,7 +22,7 @@ OUT = os.path.join(HERE, "libclothsim.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
 SOURCES = ["cs_setup.cpp", "cs_setup_cache.cpp", "cs_api.cpp", "cs_solve.hip", "cs_contact.hip",
-           "cs_elements.hip"]
+           "cs_elements.hip", "cs_noalloc.hip"]
 HEADERS = ["cs_math.hpp", "cs_ordered_sum.hpp", "cs_setup.hpp", "cs_device.hpp", "cs_solve.hpp", "cs_contact.hpp"]
 
 ARCH = os.environ.get("CLOTHSIM_ARCH", "gfx950")

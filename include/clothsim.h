@@ -288,6 +288,18 @@ int cs_inspect_setup(const cs_mesh* mesh, int* sizes, int* rowptr, int* colind, 
 int cs_selftest_divsqrt(int n, const double* a, const double* b, double* out);
 /* the PCG reduction on the device: *out = ((0.0 + t[0]) + t[1]) + ... */
 int cs_selftest_ordered_sum(int n, const double* t, double* out);
+/* cs_ordered_sum_noalloc has no counterpart in the reference.  It is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the reduction of
+ * cs_selftest_ordered_sum, *out = ((0.0 + t[0]) + t[1]) + ... bit for bit,
+ * on device pointers t (n doubles) and out, enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream), allocating nothing.  Its scratch is
+ * a static device array of CS_NOALLOC_MAX_TILES tile summaries (147 KB per
+ * GPU, held until the process ends), so n is at most
+ * CS_NOALLOC_MAX_TILES * 1024 = 4,194,304 (CS_ERR_CAPACITY beyond).  All
+ * calls on one GPU share that array: calls that overlap on one GPU race. */
+#define CS_NOALLOC_MAX_TILES 4096
+int cs_ordered_sum_noalloc(int n, const double* t, double* out, void* stream);
 /* device copy bandwidth: reps + 1 launches of a streaming copy of n doubles
  * (n even) between two fresh buffers; the last reps are timed with events.
  * *gbs = 16 n reps / seconds / 1e9 (read + write) */
