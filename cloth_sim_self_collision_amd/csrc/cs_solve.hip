@@ -261,129 +261,128 @@ __global__ __launch_bounds__(1024) void k_ilu_factor(Sel sel, int copy_in) {
   }
 }
 
-// Block ILU(0) with each row staged in LDS.  The arithmetic is ilu_row's,
-// operation for operation, in the same order: the updates of row i come from
-// its precomputed list in the reference's loop order (so no column search),
-// the row's blocks live in LDS while they are updated, and the upper block
-// of the next update is loaded while the current one is applied (rows of
-// earlier levels are final).  One thread per row, one slot of nb blocks per
-// thread: the row's blocks from rowptr[i] at their index, the diagonal at
-// nb - 1.
+// Block ILU(0), one row per group of kFgLanes lanes with the row's blocks in
+// LDS.  The arithmetic is ilu_row's, operation for operation: the updates of
+// row i come from its precomputed list (the reference's loop order, so no
+// column search).  The updates that use the same lower block L_ik target
+// distinct blocks of row i, so they run at once, one per lane; the lower
+// blocks are taken in ascending k, which gives every target block its
+// updates in the reference's order and makes each L_ik final before it is
+// used (it is itself a target only of updates with smaller k).  Each lane
+// loads its update entries' upper blocks (rows of earlier levels, final)
+// together with the row.  LDS slot per group: the row's blocks from
+// rowptr[i] at their index, the diagonal at nb - 1.
 constexpr int kStageDoubles = 15360;  // 120 KiB of LDS for the row slots
-constexpr int kFuGroup = 4;           // updates whose upper blocks load together
+constexpr int kFgLanes = 16;          // lanes per row
+constexpr int kFgLayers = 2;          // update entries each lane keeps in registers
 
 int ilu_stage_blocks(int widest_row) {
   const int nb = widest_row + 1;
   // at least one wave of rows, and every block index fits the 8-bit codes
-  return (widest_row < kFuDiag && kStageDoubles / (9 * nb) >= 64) ? nb : 0;
+  return (widest_row < kFuDiag && kStageDoubles / (9 * nb) >= 64 / kFgLanes) ? nb : 0;
 }
-
-constexpr int kStageThreads = 256;  // at most: the pipelined loads need the registers
 
 static int ilu_stage_threads(int nb) {
-  return std::min(kStageThreads, kStageDoubles / (9 * nb) / 64 * 64);
+  return std::min(1024, kStageDoubles / (9 * nb) * kFgLanes / 64 * 64);
 }
 
-__device__ void ilu_row_staged(const DevCloth& c, int i, int* ising, double* sl, int nb) {
+// LDS written by one lane of a wave is read by another after this
+__device__ __forceinline__ void fg_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A_ij -= L_ik U'_kj for one update entry (code: ik - rowptr[i] | target << 8)
+__device__ __forceinline__ void fg_apply(double* sl, int nb, int code, const double* u) {
+  const double* l = sl + (code & 0xff) * 9;
+  const int tg = (code >> 8) & 0xff;
+  double* a = sl + (tg == kFuDiag ? nb - 1 : tg) * 9;
+  for (int r = 0; r < 3; r++) {
+    const double l0 = l[r * 3 + 0], l1 = l[r * 3 + 1], l2 = l[r * 3 + 2];
+    a[r * 3 + 0] -= l0 * u[0] + l1 * u[3] + l2 * u[6];
+    a[r * 3 + 1] -= l0 * u[1] + l1 * u[4] + l2 * u[7];
+    a[r * 3 + 2] -= l0 * u[2] + l1 * u[5] + l2 * u[8];
+  }
+}
+
+// row i by lane t of its group (every lane of the group calls this)
+__device__ void ilu_row_grouped(const DevCloth& c, int i, int* ising, double* sl, int nb, int t) {
   const int rb = c.rowptr[i], re = c.rowptr[i + 1], di = c.diaind[i];
   double* F = c.fcrs;
-  const int deg = re - rb;
-  for (int q = 0; q < 9 * deg; q++) sl[q] = F[(size_t)rb * 9 + q];
-  double* d = sl + (nb - 1) * 9;
-  for (int q = 0; q < 9; q++) d[q] = c.fdia[(size_t)i * 9 + q];
   const int u0 = c.fu_ptr[i], u1 = c.fu_ptr[i + 1];
-  if (u0 < u1) {
-    // groups of kFuGroup updates: while one group is applied, the upper
-    // blocks of the next are loading and the descriptors of the one after;
-    // loads past the list repeat its last entry (never applied), so every
-    // load is unconditional
-    const int last = u1 - 1;
-    int ka[kFuGroup], ca[kFuGroup], kb[kFuGroup], cb[kFuGroup];
+  int code[kFgLayers];
+  double ub[kFgLayers][9];
 #pragma unroll
-    for (int j = 0; j < kFuGroup; j++) {
-      ka[j] = c.fu_kj[min(u0 + j, last)];
-      ca[j] = c.fu_code[min(u0 + j, last)];
-      kb[j] = c.fu_kj[min(u0 + kFuGroup + j, last)];
-      cb[j] = c.fu_code[min(u0 + kFuGroup + j, last)];
-    }
-    double ua[kFuGroup][9];
+  for (int m = 0; m < kFgLayers; m++) {
+    const int e = u0 + t + m * kFgLanes;
+    code[m] = -1;
+    if (e < u1) {
+      code[m] = c.fu_code[e];
+      const size_t kj = (size_t)c.fu_kj[e];
 #pragma unroll
-    for (int j = 0; j < kFuGroup; j++)
-#pragma unroll
-      for (int q = 0; q < 9; q++) ua[j][q] = F[(size_t)ka[j] * 9 + q];
-    for (int g = u0; g < u1; g += kFuGroup) {
-      double ub[kFuGroup][9];
-      int kc[kFuGroup], cc[kFuGroup];
-#pragma unroll
-      for (int j = 0; j < kFuGroup; j++)
-#pragma unroll
-        for (int q = 0; q < 9; q++) ub[j][q] = F[(size_t)kb[j] * 9 + q];
-#pragma unroll
-      for (int j = 0; j < kFuGroup; j++) {
-        kc[j] = c.fu_kj[min(g + 2 * kFuGroup + j, last)];
-        cc[j] = c.fu_code[min(g + 2 * kFuGroup + j, last)];
-      }
-#pragma unroll
-      for (int j = 0; j < kFuGroup; j++) {
-        if (g + j > last) break;
-        const double* u = ua[j];
-        const double* l = sl + (ca[j] & 0xff) * 9;
-        const int tg = ca[j] >> 8;
-        double* a = sl + (tg == kFuDiag ? nb - 1 : tg) * 9;
-        for (int r = 0; r < 3; r++) {
-          const double l0 = l[r * 3 + 0], l1 = l[r * 3 + 1], l2 = l[r * 3 + 2];
-          a[r * 3 + 0] -= l0 * u[0] + l1 * u[3] + l2 * u[6];
-          a[r * 3 + 1] -= l0 * u[1] + l1 * u[4] + l2 * u[7];
-          a[r * 3 + 2] -= l0 * u[2] + l1 * u[5] + l2 * u[8];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kFuGroup; j++) {
-        ca[j] = cb[j];
-        kb[j] = kc[j];
-        cb[j] = cc[j];
-#pragma unroll
-        for (int q = 0; q < 9; q++) ua[j][q] = ub[j][q];
-      }
+      for (int q = 0; q < 9; q++) ub[m][q] = F[kj * 9 + q];
     }
   }
-  const double det = +d[0] * d[4] * d[8] + d[3] * d[7] * d[2] + d[6] * d[1] * d[5] -
-                     d[0] * d[7] * d[5] - d[6] * d[4] * d[2] - d[3] * d[1] * d[8];
-  double* fd = c.fdia + (size_t)i * 9;
-  if (fabs(det) > 1.0e-30) {
-    const double id = 1.0 / det;
-    double t[9];
-    for (int q = 0; q < 9; q++) t[q] = d[q];
-    d[0] = id * (t[4] * t[8] - t[5] * t[7]);
-    d[1] = id * (t[2] * t[7] - t[1] * t[8]);
-    d[2] = id * (t[1] * t[5] - t[2] * t[4]);
-    d[3] = id * (t[5] * t[6] - t[3] * t[8]);
-    d[4] = id * (t[0] * t[8] - t[2] * t[6]);
-    d[5] = id * (t[2] * t[3] - t[0] * t[5]);
-    d[6] = id * (t[3] * t[7] - t[4] * t[6]);
-    d[7] = id * (t[1] * t[6] - t[0] * t[7]);
-    d[8] = id * (t[0] * t[4] - t[1] * t[3]);
-  } else if (ising) {
-    ising[i] = 1;
-    atomicAdd(&c.flags[kNsing], 1);
+  const int deg = re - rb;
+  for (int q = t; q < 9 * deg; q += kFgLanes) sl[q] = F[(size_t)rb * 9 + q];
+  double* d = sl + (nb - 1) * 9;
+  for (int q = t; q < 9; q += kFgLanes) d[q] = c.fdia[(size_t)i * 9 + q];
+  fg_wave_sync();
+  for (int k = 0; k < di - rb; k++) {
+#pragma unroll
+    for (int m = 0; m < kFgLayers; m++)
+      if (code[m] >= 0 && (code[m] & 0xff) == k) fg_apply(sl, nb, code[m], ub[m]);
+    // entries beyond the registers (rows with more than kFgLayers * kFgLanes updates)
+    for (int e = u0 + t + kFgLayers * kFgLanes; e < u1; e += kFgLanes) {
+      const int cd = c.fu_code[e];
+      if ((cd & 0xff) != k) continue;
+      const size_t kj = (size_t)c.fu_kj[e];
+      double u[9];
+      for (int q = 0; q < 9; q++) u[q] = F[kj * 9 + q];
+      fg_apply(sl, nb, cd, u);
+    }
+    fg_wave_sync();
   }
-  for (int q = 0; q < 9; q++) fd[q] = d[q];
-  for (int q = 0; q < 9 * (di - rb); q++) F[(size_t)rb * 9 + q] = sl[q];  // lower blocks
-  for (int ij = di; ij < re; ij++) {
-    const double* t = sl + (ij - rb) * 9;
-    double* u = F + (size_t)ij * 9;
-    for (int r = 0; r < 3; r++) {
-      u[r * 3 + 0] = d[r * 3 + 0] * t[0] + d[r * 3 + 1] * t[3] + d[r * 3 + 2] * t[6];
-      u[r * 3 + 1] = d[r * 3 + 0] * t[1] + d[r * 3 + 1] * t[4] + d[r * 3 + 2] * t[7];
-      u[r * 3 + 2] = d[r * 3 + 0] * t[2] + d[r * 3 + 1] * t[5] + d[r * 3 + 2] * t[8];
+  if (t == 0) {
+    const double det = +d[0] * d[4] * d[8] + d[3] * d[7] * d[2] + d[6] * d[1] * d[5] -
+                       d[0] * d[7] * d[5] - d[6] * d[4] * d[2] - d[3] * d[1] * d[8];
+    if (fabs(det) > 1.0e-30) {
+      const double id = 1.0 / det;
+      double w[9];
+      for (int q = 0; q < 9; q++) w[q] = d[q];
+      d[0] = id * (w[4] * w[8] - w[5] * w[7]);
+      d[1] = id * (w[2] * w[7] - w[1] * w[8]);
+      d[2] = id * (w[1] * w[5] - w[2] * w[4]);
+      d[3] = id * (w[5] * w[6] - w[3] * w[8]);
+      d[4] = id * (w[0] * w[8] - w[2] * w[6]);
+      d[5] = id * (w[2] * w[3] - w[0] * w[5]);
+      d[6] = id * (w[3] * w[7] - w[4] * w[6]);
+      d[7] = id * (w[1] * w[6] - w[0] * w[7]);
+      d[8] = id * (w[0] * w[4] - w[1] * w[3]);
+    } else if (ising) {
+      ising[i] = 1;
+      atomicAdd(&c.flags[kNsing], 1);
     }
   }
+  fg_wave_sync();
+  for (int q = t; q < 9; q += kFgLanes) c.fdia[(size_t)i * 9 + q] = d[q];
+  for (int q = t; q < 9 * (di - rb); q += kFgLanes) F[(size_t)rb * 9 + q] = sl[q];  // lower blocks
+  for (int q = t; q < 3 * (re - di); q += kFgLanes) {  // upper blocks: D^-1 U, one block row each
+    const int b = q / 3, r = q - 3 * b;
+    const double* w = sl + (di - rb + b) * 9;
+    double* u = F + (size_t)(di + b) * 9;
+    u[r * 3 + 0] = d[r * 3 + 0] * w[0] + d[r * 3 + 1] * w[3] + d[r * 3 + 2] * w[6];
+    u[r * 3 + 1] = d[r * 3 + 0] * w[1] + d[r * 3 + 1] * w[4] + d[r * 3 + 2] * w[7];
+    u[r * 3 + 2] = d[r * 3 + 0] * w[2] + d[r * 3 + 1] * w[5] + d[r * 3 + 2] * w[8];
+  }
+  fg_wave_sync();  // the slot is reused by the group's next row
 }
 
 // copy_in as k_ilu_factor; nb: blocks per row slot (at least every row's
-// blocks plus one), blockDim.x * nb * 9 <= kStageDoubles
+// blocks plus one), blockDim.x / kFgLanes * nb * 9 <= kStageDoubles
 template <class Sel>
-__global__ __launch_bounds__(kStageThreads) void k_ilu_factor_staged(Sel sel, int copy_in, int nb) {
+__global__ __launch_bounds__(1024) void k_ilu_factor_staged(Sel sel, int copy_in, int nb) {
   const DevCloth c = sel.cloth();
   __shared__ double stage[kStageDoubles];
   if (copy_in) {
@@ -393,11 +392,12 @@ __global__ __launch_bounds__(kStageThreads) void k_ilu_factor_staged(Sel sel, in
     if (threadIdx.x == 0) c.flags[kNsing] = 0;
     __syncthreads();
   }
-  double* sl = stage + (size_t)threadIdx.x * nb * 9;
+  const int grp = threadIdx.x / kFgLanes, t = threadIdx.x % kFgLanes;
+  const int ngrp = blockDim.x / kFgLanes;
+  double* sl = stage + (size_t)grp * nb * 9;
   for (int l = 0; l < c.nlf; l++) {
     const int b = c.lf_ptr[l], e = c.lf_ptr[l + 1];
-    for (int t = b + (int)threadIdx.x; t < e; t += blockDim.x)
-      ilu_row_staged(c, c.lf_rows[t], c.ising, sl, nb);
+    for (int r = b + grp; r < e; r += ngrp) ilu_row_grouped(c, c.lf_rows[r], c.ising, sl, nb, t);
     __syncthreads();
   }
 }
