@@ -109,6 +109,8 @@ SIGNATURES = {
     "cs_element_cst": (C.c_int, [C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]),
     "cs_element_bend": (C.c_int, [C.c_int, _D, _D, C.c_double, _D, _D]),
     "cs_element_contact": (C.c_int, [C.c_int, _D, C.c_double, C.c_double, C.c_int, _D, _D, _D]),
+    "cs_element_cst_energy": (C.c_int, [C.c_int, _D, _D, C.c_double, C.c_double, _D]),
+    "cs_element_contact_depth": (C.c_int, [C.c_int, _D, _D, C.c_double, C.c_double, _D, _D, _D]),
     "cs_selftest_divsqrt": (C.c_int, [C.c_int, _D, _D, _D]),
     "cs_selftest_ordered_sum": (C.c_int, [C.c_int, _D, _D]),
     "cs_ordered_sum_noalloc": (C.c_int, [C.c_int, _V, _V, _V]),

@@ -46,6 +46,29 @@ struct Obst8 {
   double v[8];
 };
 
+// energy of each triangle, W = 0.5 A (E . S) (cloth_internal_physics.h:125)
+__global__ void k_elem_cst_energy(int n, const double* C, const double* x, double lam, double mu,
+                                  double* W) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  CstState s;
+  cst_state(C + 9 * (size_t)e, x + 9 * (size_t)e, lam, mu, s);
+  W[e] = 0.5 * s.area * (s.strain[0] * s.stress[0] + s.strain[1] * s.stress[1] +
+                         s.strain[2] * s.stress[2]);
+}
+
+// penalty of each point from its obstacle depth and direction as a
+// penetrationDepth callback reported them (cloth_internal_physics.h:254-279)
+__global__ void k_elem_contact_depth(int n, const double* pd, const double* nrm, double k,
+                                     double clearance, double* W, double* dW, double* ddW) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const bool on = contact_penalty(pd[e], nrm + 3 * (size_t)e, k, clearance, dW + 3 * (size_t)e,
+                                  ddW + 9 * (size_t)e);
+  const double d = pd[e] + clearance;
+  W[e] = on ? 0.5 * k * d * d : 0.0;
+}
+
 __global__ void k_elem_contact(int n, const double* p, double k, double clearance, int kind,
                                Obst8 o, double* dW, double* ddW) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -100,6 +123,41 @@ extern "C" int cs_element_cst(int n, const double* C, const double* c, double la
   CS_HIP(hipGetLastError());
   CS_HIP(hipMemcpy(dW, dg, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));
   CS_HIP(hipMemcpy(ddW, dh, sizeof(double) * 81 * n, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+extern "C" int cs_element_cst_energy(int n, const double* C, const double* c, double lambda,
+                                     double myu, double* W) {
+  if (n < 0 || (n > 0 && (!C || !c || !W))) return kBadArg;
+  if (n == 0) return kOk;
+  Scratch s;
+  double *dC, *dc, *dw;
+  if (s.get(&dC, 9L * n) || s.get(&dc, 9L * n) || s.get(&dw, (long)n)) return kHipError;
+  CS_HIP(hipMemcpy(dC, C, sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+  CS_HIP(hipMemcpy(dc, c, sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_elem_cst_energy, dim3(blocks(n)), dim3(256), 0, 0, n, dC, dc, lambda, myu, dw);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpy(W, dw, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+extern "C" int cs_element_contact_depth(int n, const double* pd, const double* nrm, double stiff,
+                                        double clearance, double* W, double* dW, double* ddW) {
+  if (n < 0 || (n > 0 && (!pd || !nrm || !W || !dW || !ddW))) return kBadArg;
+  if (n == 0) return kOk;
+  Scratch s;
+  double *dpd, *dn, *dw, *dg, *dh;
+  if (s.get(&dpd, (long)n) || s.get(&dn, 3L * n) || s.get(&dw, (long)n) || s.get(&dg, 3L * n) ||
+      s.get(&dh, 9L * n))
+    return kHipError;
+  CS_HIP(hipMemcpy(dpd, pd, sizeof(double) * n, hipMemcpyHostToDevice));
+  CS_HIP(hipMemcpy(dn, nrm, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_elem_contact_depth, dim3(blocks(n)), dim3(256), 0, 0, n, dpd, dn, stiff,
+                     clearance, dw, dg, dh);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpy(W, dw, sizeof(double) * n, hipMemcpyDeviceToHost));
+  CS_HIP(hipMemcpy(dW, dg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  CS_HIP(hipMemcpy(ddW, dh, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));
   return kOk;
 }
 

@@ -332,6 +332,30 @@ def element_contact(p, stiff, clearance, obstacle):
     return dW.reshape(n, 3), ddW.reshape(n, 9)
 
 
+def element_cst_energy(C_, c_, lam, mu):
+    """The energy W of WdWddW_CST on the device, per triangle."""
+    C_ = f64(C_).reshape(-1)
+    c_ = f64(c_).reshape(-1)
+    n = len(C_) // 9
+    W = np.zeros(n)
+    check(_lib.load().cs_element_cst_energy(n, dp(C_), dp(c_), lam, mu, dp(W)))
+    return W
+
+
+def element_contact_depth(pd, nrm, stiff, clearance):
+    """WdWddW_Contact on the device from callback depths and directions:
+    (W[n], dW[n,3], ddW[n,9])."""
+    pd = f64(pd).reshape(-1)
+    nrm = f64(nrm).reshape(-1)
+    n = len(pd)
+    W = np.zeros(n)
+    dW = np.zeros(3 * n)
+    ddW = np.zeros(9 * n)
+    check(_lib.load().cs_element_contact_depth(n, dp(pd), dp(nrm), stiff, clearance, dp(W),
+                                               dp(dW), dp(ddW)))
+    return W, dW.reshape(n, 3), ddW.reshape(n, 9)
+
+
 class DeviceBatch:
     """Several DeviceCloths stepped together by shared launches
     (cs_batch_*).  Each cloth's result equals stepping it alone."""

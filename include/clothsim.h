@@ -256,6 +256,9 @@ int cs_query_ccd(cs_cloth* c, const double* xyz, const double* uvw, double dt, d
 /* WdWddW_CST for n triangles: C, c 9 doubles each; W[n], dW[9n], ddW[81n] */
 int cs_element_cst(int n, const double* C, const double* c, double lambda, double myu,
                    double* dW, double* ddW);
+/* the energy W[n] of WdWddW_CST (cloth_internal_physics.h:125) */
+int cs_element_cst_energy(int n, const double* C, const double* c, double lambda, double myu,
+                          double* W);
 /* WdWddW_Bend for n stencils: C, c 12 doubles each; dW[12n], ddW[144n] */
 int cs_element_bend(int n, const double* C, const double* c, double stiff, double* dW,
                     double* ddW);
@@ -263,6 +266,11 @@ int cs_element_bend(int n, const double* C, const double* c, double stiff, doubl
  * cs_params, 8 doubles): dW[3n], ddW[9n] */
 int cs_element_contact(int n, const double* p, double stiff, double clearance, int kind,
                        const double* obstacle, double* dW, double* ddW);
+/* WdWddW_Contact (cloth_internal_physics.h:243-280) for n points whose
+ * penetrationDepth callback gave depth pd[n] and direction nrm[3n]:
+ * W[n], dW[3n], ddW[9n] */
+int cs_element_contact_depth(int n, const double* pd, const double* nrm, double stiff,
+                             double clearance, double* W, double* dW, double* ddW);
 
 /* On-disk cache of the per-mesh setup (pattern, level schedules, sweep
  * slots, gather lists, BVH topology), the work cs_cloth_create and

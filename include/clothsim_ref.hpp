@@ -86,6 +86,9 @@ class Cloth {
     conv_ = conv;
     iter_ = iter;
   }
+  // which CPreconditionerILU's factor the device holds now (clothsim_ref_ops.hpp);
+  // the internal step refactors on the device and clears it
+  const void* factor_owner = nullptr;
 
  private:
   cs_cloth* h_ = nullptr;
@@ -123,6 +126,7 @@ inline void StepTime_InternalDynamicsILU(
   double conv = 0.0;
   int iter = 0;
   check(cs_step_internal(cloth.handle(), &conv, &iter), "cs_step_internal");
+  cloth.factor_owner = nullptr;
   cloth.record(conv, iter);
   check(cs_get_state(cloth.handle(), aXYZ.data(), aUVW.data()), "cs_get_state");
 }

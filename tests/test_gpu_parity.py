@@ -140,6 +140,21 @@ def test_element_routines_bitwise(sim_mod, oracle_mod):
         for e in range(0, n, 17):
             _, g0, h0 = oracle_mod.element_contact(p[e], 1e3, 0.01, ob.as_array())
             assert np.array_equal(g[e], g0) and np.array_equal(h[e], h0)
+    # energies: W of WdWddW_CST, and WdWddW_Contact from callback values (a
+    # sphere's depth and direction computed here as the C callback does)
+    W = sim_mod.element_cst_energy(C, c, 576.9, 384.6)
+    for e in range(0, n, 37):
+        assert W[e] == oracle_mod.element_cst(C[e], c[e], 576.9, 384.6)[0]
+    ob = scene.sphere((0.05, -0.02, 0.01), 0.3)
+    d = p - np.array(ob.params[:3])
+    ln = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    nrm = d / ln[:, None]
+    pd = ob.params[3] - ln
+    Wc, gc, hc = sim_mod.element_contact_depth(pd, nrm, 1e3, 0.01)
+    assert (pd + 0.01 >= 0).any() and (pd + 0.01 < 0).any()
+    for e in range(n):
+        w0, g0, h0 = oracle_mod.element_contact(p[e], 1e3, 0.01, ob.as_array())
+        assert Wc[e] == w0 and np.array_equal(gc[e], g0) and np.array_equal(hc[e], h0), e
 
 
 @pytest.mark.parametrize("name", ["drape16", "refdemo0", "fold24"])
@@ -403,6 +418,106 @@ def test_cpp_shim_step_loop_matches_reference(sim_mod, oracle_mod, tmp_path):
     for _ in range(steps):
         x, v, *_ = o.step(x, v)
     assert np.array_equal(res[:3 * cl.np_], x) and np.array_equal(res[3 * cl.np_:], v)
+
+
+def _read_ops(path, np_, nnz):
+    """The results tests/shim/ops_check.cpp writes, in its order."""
+    raw = open(path, "rb").read()
+    pos = 0
+
+    def take(dtype, n):
+        nonlocal pos
+        a = np.frombuffer(raw, dtype, n, pos)
+        pos += a.nbytes
+        return a
+
+    out = {}
+    out["rejected"], out["accepted"], out["iteration"], out["n_singular"] = take(np.int32, 4)
+    out["conv"], out["W"] = take(np.float64, 2)
+    for k, n in (("dia", 9 * np_), ("crs", 9 * nnz), ("b", 3 * np_), ("fdia", 9 * np_),
+                 ("fcrs", 9 * nnz)):
+        out[k] = take(np.float64, n)
+    out["diaind"] = take(np.int32, np_)
+    for k in ("x", "r", "y", "z"):
+        out[k] = take(np.float64, 3 * np_)
+    nnode = int(take(np.int32, 1)[0])
+    out["nodes"] = take(np.int32, 3 * nnode).reshape(-1, 3)
+    out["bb_prx"] = take(np.float64, 6 * nnode).reshape(-1, 6)
+    out["bb_ccd"] = take(np.float64, 6 * nnode).reshape(-1, 6)
+    for k in ("prox", "ccd"):
+        n = int(take(np.int32, 1)[0])
+        out[k] = take(np.int32, 5 * n).reshape(-1, 5)
+    assert pos == len(raw)
+    return out
+
+
+@pytest.mark.parametrize("name,steps", [("drape16", 40), ("fold20", 45), ("stack16", 34)])
+def test_cpp_reference_named_operations_match_oracle(sim_mod, oracle_mod, tmp_path, name, steps):
+    """The reference's lower-level classes and functions written against
+    include/clothsim_ref_ops.hpp (tests/shim/ops_check.cpp, a separate
+    process), at a state mid-trajectory (a drape over a sphere, a swept fold
+    with pinned vertices, two sheets in contact): SetPattern takes the stencil pattern
+    and rejects a different one; the system assembled element by element
+    (WdWddW_CST/_Bend/_Contact, Mearge, the m_valDia mass poke,
+    SetBoundaryCondition) equals the oracle's assembly; the ILU factor,
+    m_diaInd, Solve_PCG (solution, residual, ratio, iteration), MatVec and
+    CPreconditionerILU::Solve equal the oracle's; the BVH topology, both
+    refits and both contact sets equal the oracle's."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "shim", "bin", "ops_check")
+    if not os.path.exists(exe):
+        pytest.fail("tests/shim/bin/ops_check is not built (make -C tests/shim)")
+    cl = {"drape16": lambda: scene.drape_over_sphere(16),
+          "fold20": lambda: scene.fold_sweep(20, size=0.3),
+          "stack16": lambda: scene.stacked_sheets(2, 16, size=0.5, dt=0.005, seed=3,
+                                                  obstacle=scene.sphere((0.0, 0.0, -0.3), 0.3))}[name]()
+    o = oracle_mod.Oracle(cl, "port")
+    x, v, t = cl.xyz.copy(), cl.uvw.copy(), 0.0
+    for _ in range(steps):
+        scene.apply_sweep(cl, x, t + cl.dt)
+        x, v, *_ = o.step(x, v)
+        t += cl.dt
+    scene.apply_sweep(cl, x, t + cl.dt)
+    m = cl.material
+    sphere = cl.obstacle.kind == scene.OBST_SPHERE
+    hdr = np.array([cl.np_, cl.nt, cl.nq, cl.pcg_maxit, 2 if sphere else 0], np.int32)
+    ob = list(cl.obstacle.params[:4]) if sphere else [0.0] * 4
+    prm = np.array([m.lam, m.mu, m.stiff_bend, *m.gravity, cl.mass_point, m.stiff_contact,
+                    cl.clearance, cl.dt, cl.pcg_tol, *ob], np.float64)
+    assert len(prm) == 15
+    src = tmp_path / "in.bin"
+    with open(src, "wb") as f:
+        for a in (hdr, prm, cl.xyz0, cl.xyz, x, v):
+            f.write(np.ascontiguousarray(a).tobytes())
+        for a in (cl.tri, cl.quad, cl.bc):
+            f.write(np.ascontiguousarray(a, np.int32).tobytes())
+    out = tmp_path / "out.bin"
+    r = subprocess.run([exe, str(src), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rp, ci, di = o.pattern()
+    g = _read_ops(out, cl.np_, len(ci))
+    assert g["rejected"] == 1 and g["accepted"] == 1
+    dia, crs, b = o.assemble(x, v)
+    assert np.array_equal(g["dia"], dia) and np.array_equal(g["crs"], crs)
+    assert np.array_equal(g["b"], b)
+    fd, fc = o.ilu(dia, crs)
+    assert np.array_equal(g["fdia"], fd) and np.array_equal(g["fcrs"], fc)
+    assert np.array_equal(g["diaind"], di)
+    xs, rr, conv, it = o.pcg(dia, crs, fd, fc, b, cl.pcg_tol, cl.pcg_maxit)
+    assert np.array_equal(g["x"], xs) and np.array_equal(g["r"], rr)
+    assert g["conv"] == conv and g["iteration"] == it
+    assert np.array_equal(g["y"], o.matvec(dia, crs, 1.5, xs, 0.5, b))
+    assert np.array_equal(g["z"], o.ilu_solve(fd, fc, b))
+    assert np.array_equal(g["nodes"], o.bvh())
+    vm = xs / cl.dt
+    assert np.array_equal(g["bb_prx"], o.refit_prx(x, cl.clearance))
+    assert np.array_equal(g["bb_ccd"], o.refit_ccd(x, vm, cl.dt))
+    prox = o.prox_set(x, cl.clearance)
+    assert np.array_equal(g["prox"], prox)
+    assert np.array_equal(g["ccd"], o.ccd_set(x, vm, cl.dt, cl.clearance))
+    if name == "stack16":
+        assert len(prox) > 0  # sheet on sheet: the state has self-contact
 
 
 @pytest.mark.parametrize("pcg", [0, 2])
