@@ -1,26 +1,46 @@
 """Benchmark of the implicit cloth step with self-contact on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fold256]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--requests N]
 
-A step is one StepTime (self_contact_sparse/main.cpp:213-263) of the whole
-cloth: assembly, ILU(0), ILU-PCG, proximity + continuous contact pass,
-post-contact velocity update, all device-resident in FP64 and bit-identical
-to the reference.  The default workload is the single large cloth: a
-256x256-vertex 2 m sheet (130,050 triangles), one edge pinned, the opposite
-edge swept to fold it onto itself, dt = 0.005, PCG tolerance 1e-6 / cap 1000.
+Headline: the serving stream.  N requests arrive as a Poisson process; each
+is a rectangular cloth whose triangle count is log-uniform in [1e3, 2e5]
+(aspect 1..4), dropped from a seeded height onto its own sphere or box, with
+dt 0.01 and ILU(0)-PCG at tolerance 1e-6 / cap 1000.  Each request runs
+min(drawn steps, K) steps (drawn uniformly in 50..300; the cap bounds the
+run time and is written into `config`).  One step is one StepTime
+(self_contact_sparse/main.cpp:213-263): assembly, ILU(0), ILU-PCG, proximity
++ continuous contact pass, post-contact velocity update, in FP64 and
+bit-identical to the reference.
 
-Prints one JSON line: whole-job vertex-steps/s, per-step latency percentiles,
-the roofline of the dominant kernel class (algorithmic bytes / measured
-average launch time, HIP events on the cloth's stream over the timed
-region), the position error of the first step against the CPU reference and
-the CPU baseline (the reference's own code, compiled from its sources,
-timed on one host core for one step of the same workload).
+The stream is served with continuous batching (cloth_sim_self_collision_amd.
+serve.Server: HBM admission, size lanes stepping their resident cloths
+together), then the same requests run one after another with the plain
+single-cloth step; `value` is the served whole-job vertex-steps/s (inputs
+resident once admitted; cloth creation on admission is inside the timed
+run, for both legs), `plain` the plain one, with per-step p50/p99 latencies.
+Every served result is compared bit for bit with the plain one, and the
+first step of each request in the CPU sample with the reference's own code
+(`pos_err_m`).
+
+`roofline`: the kernel class with the most device time in the served run
+(HIP events around the shared launches on each lane's stream), its
+algorithmic bytes (unique bytes its launches must move, summed over the
+cloths each launch works on) over its measured launch time.  `cpu_baseline`:
+the reference's step compiled from its sources (oracle/_ref, else the C
+restatement), one process per host core, each running one step of one
+request (largest first), throughput = vertex-steps / wall time.
+
+Also reported: the single 256x256 folding cloth (plain step), the batch of 64
+cloths (sides 24..128) stepped by one cs_batch_step, and the 32x32 drape,
+each with its CPU baseline.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import multiprocessing as mp
 import os
+import queue
 import sys
 import time
 
@@ -39,27 +59,12 @@ TRAFFIC_BYTES: dict = {
     # profiles/fold256_hbm_traffic_pmc.md: (34064.5 * 2.000 + 4608.0 * 1.000) KiB
     ("fold256", "ilu_sweep"): 74.48e6,
 }
-
-
-def workload(name: str) -> scene.Cloth:
-    """fold256 (default), foldN, drapeN, or a scene configuration file (.json)."""
-    if name.endswith(".json"):
-        cl = scene.load_config(name)
-        if isinstance(cl, list):
-            raise SystemExit("the bench steps one cloth; the batch workload is --batch-count")
-        return cl
-    if name == "fold256":
-        return scene.fold_sweep(256)
-    if name.startswith("fold"):
-        return scene.fold_sweep(int(name[4:]))
-    if name.startswith("drape"):
-        return scene.drape_over_sphere(int(name[5:]))
-    raise SystemExit(f"unknown workload {name}")
+METRIC = "vertex-steps/s + p99 step latency, served vs plain GPU, at ≤1e-5 m pos err"
 
 
 def algorithmic_bytes(kind: str, np_: int, nnz: int) -> float:
-    """Unique bytes one launch of a kernel class must move (FP64 values,
-    int32 indices); re-reads and caches are not counted."""
+    """Unique bytes one pass of a kernel class over one cloth must move (FP64
+    values, int32 indices); re-reads and caches are not counted."""
     mat = 72.0 * (nnz + np_)          # off-diagonal + diagonal 3x3 blocks
     idx = 4.0 * nnz + 4.0 * (np_ + 1)  # column indices + row offsets
     vec = 24.0 * np_                  # one 3-vector per vertex
@@ -76,232 +81,364 @@ def algorithmic_bytes(kind: str, np_: int, nnz: int) -> float:
     return 0.0
 
 
-def cpu_baseline(cloth: scene.Cloth, gx1: np.ndarray):
-    """Time one step of the same workload on the host (1 thread) with the
-    reference's compiled code when available, else the restatement."""
+def step_bytes(kind: str, np_: int, nnz: int, iters: int) -> float:
+    """Algorithmic bytes of one cloth's step in a kernel class, given its PCG
+    iterations: one factorisation; iters + 1 preconditioner sweeps, matrix
+    products and vector updates; three reductions per iteration; the
+    one-workgroup solve does all of these in one launch."""
+    passes = iters + 1
+    if kind == "ilu_factor":
+        return algorithmic_bytes(kind, np_, nnz)
+    if kind in ("ilu_sweep", "spmv", "vector_update"):
+        return passes * algorithmic_bytes(kind, np_, nnz)
+    if kind == "pcg_reduce":
+        return 3 * passes * algorithmic_bytes(kind, np_, nnz)
+    if kind == "pcg_workgroup":
+        return sum(step_bytes(k, np_, nnz, iters) for k in ("ilu_sweep", "spmv", "vector_update",
+                                                            "pcg_reduce"))
+    return 0.0
+
+
+def roofline_from_ticks(ticks: list, copy_gbs, traffic_key=None) -> dict:
+    """Dominant kernel class of a batched run from per-tick records
+    ({"prof": {class: (ms, launches)}, "members": [(np, nnz, iters, status)]}):
+    its algorithmic bytes summed over every member's work ÷ its total launch
+    time."""
+    ms, n, by = {}, {}, {}
+    for t in ticks:
+        for k, (m, c) in t["prof"].items():
+            if c <= 0:
+                continue
+            ms[k] = ms.get(k, 0.0) + m
+            n[k] = n.get(k, 0) + c
+            by[k] = by.get(k, 0.0) + sum(step_bytes(k, a, b, it) for a, b, it, st in t["members"]
+                                         if st == 0)
+    if not ms:
+        return None
+    dom = max(ms, key=ms.get)
+    share = ms[dom] / sum(ms.values())
+    ach = by[dom] / (ms[dom] * 1e-3) / 1e9 if by[dom] > 0 else None
+    return {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (ach / HBM_PEAK_GBS) if ach else None,
+            "traffic": TRAFFIC_BYTES.get(traffic_key) if traffic_key else None,
+            "bytes_per_launch": by[dom] / n[dom], "avg_launch_us": ms[dom] / n[dom] * 1e3,
+            "launches": n[dom], "share_of_profiled_gpu_time": share,
+            "class_ms": {k: round(v, 2) for k, v in sorted(ms.items(), key=lambda kv: -kv[1])},
+            "stream_copy_gbs": copy_gbs,
+            "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None}
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: one process per core, each stepping whole cloths
+
+
+def _cpu_worker(cpu, kind, jobs, out):
+    try:
+        os.sched_setaffinity(0, {cpu})
+    except (AttributeError, OSError):
+        pass
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    while True:
+        job = jobs.get()
+        if job is None:
+            return
+        key, cl, steps = job
+        o = pyoracle.Oracle(cl, kind)
+        x, v, t = cl.xyz.copy(), cl.uvw.copy(), 0.0
+        times, x1 = [], None
+        t_begin = time.time()
+        for k in range(steps):
+            scene.apply_sweep(cl, x, t + cl.dt)
+            t0 = time.perf_counter()
+            x, v, *_ = o.step(x, v)
+            times.append(time.perf_counter() - t0)
+            if k == 0:
+                x1 = x.copy()
+            t += cl.dt
+        t_end = time.time()
+        o.close()
+        out.put((key, cl.np_, times, x1, t_begin, t_end))
+
+
+def cpu_baseline(jobs: list, processes: int, what: str) -> tuple:
+    """Run `jobs` [(key, cloth, steps)] on `processes` pinned single-core
+    processes (largest cloth first); returns (baseline dict, {key: x after
+    the first step})."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     kind = "reference" if pyoracle.available("reference") else "port"
-    backend = "reference" if kind == "reference" else "port"
-    o = pyoracle.Oracle(cloth, backend)
-    x = cloth.xyz.copy()
-    v = cloth.uvw.copy()
-    scene.apply_sweep(cloth, x, cloth.dt)
-    t0 = time.perf_counter()
-    x1, v1, *_ = o.step(x, v)
-    dt_s = time.perf_counter() - t0
-    o.close()
-    err = float(np.abs(x1 - gx1).max()) if gx1 is not None else None
-    return {"value": cloth.np_ / dt_s, "unit": "vertex-steps/s", "cores": 1, "kind": kind,
-            "sample": f"1 step of {cloth.name} from the initial state ({dt_s:.1f} s)"}, err
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(
+        range(os.cpu_count() or 1))
+    procs_n = max(1, min(processes, len(cpus), len(jobs)))
+    ctx = mp.get_context("spawn")
+    q_in, q_out = ctx.Queue(), ctx.Queue()
+    for j in sorted(jobs, key=lambda j: -j[1].np_ * j[2]):
+        q_in.put(j)
+    for _ in range(procs_n):
+        q_in.put(None)
+    procs = [ctx.Process(target=_cpu_worker, args=(cpus[k], kind, q_in, q_out), daemon=True)
+             for k in range(procs_n)]
+    for p in procs:
+        p.start()
+    res = []
+    while len(res) < len(jobs):
+        try:
+            res.append(q_out.get(timeout=1.0))
+        except queue.Empty:
+            if any(p.exitcode not in (None, 0) for p in procs):
+                raise RuntimeError("a CPU baseline process failed")
+    for p in procs:
+        p.join()
+    wall = max(r[5] for r in res) - min(r[4] for r in res)
+    vsteps = sum(r[1] * len(r[2]) for r in res)
+    lat = np.array([t for r in res for t in r[2]]) * 1e3
+    base = {"value": vsteps / wall, "unit": "vertex-steps/s", "cores": procs_n, "kind": kind,
+            "sample": what, "wall_s": round(wall, 2),
+            "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+            "steps_timed": int(len(lat))}
+    return base, {r[0]: r[3] for r in res}
 
 
-def batch_measure(count: int, steps: int, seed: int = 0) -> dict:
-    """The batch workload (independent cloths of seeded sizes 24..128 dropped
-    onto their own sphere or box): the same cloths stepped one after another
-    with their own launches ("plain") and together by cs_batch_step."""
+# ---------------------------------------------------------------------------
+# GPU legs
+
+
+def serve_leg(args, copy_gbs) -> dict:
+    """The headline: the served stream and the same requests plain."""
+    from cloth_sim_self_collision_amd import serve
+    wire = serve.poisson_requests(args.requests, args.rate, seed=args.seed, tri_range=(1.0e3, 2.0e5),
+                                  aspect_range=(1.0, 4.0), step_range=(50, 300), pcg_maxit=1000)
+    drawn = [w["steps"] for w in wire]
+    for w in wire:
+        w["steps"] = min(w["steps"], args.steps)
+    reqs = [serve.parse_request(w, k) for k, w in enumerate(wire)]
+    lane_pcg = tuple(int(v) for v in args.lane_pcg.split(",")) if args.lane_pcg else None
+    # warm-up: the three smallest requests, served and plain, W steps each
+    small = sorted(reqs, key=lambda r: r.cloth.np_)[:3]
+    warm = [serve.Request(f"warm{k}", r.cloth, steps=max(1, args.warmup)) for k, r in enumerate(small)]
+    if args.warmup > 0:
+        serve.Server(64 << 30, lane_pcg=lane_pcg).run(warm, realtime=False)
+        serve.run_plain(warm[:1])
+    print(f"[bench] serve: {len(reqs)} requests, {sum(r.cloth.nt for r in reqs)} triangles, "
+          f"step cap {args.steps}", file=sys.stderr, flush=True)
+    solo = {"solo_from": None if args.solo_from == 0 else args.solo_from} if args.solo_from >= 0 else {}
+    srv = serve.Server(64 << 30, lane_pcg=lane_pcg, profile=not args.no_profile, **solo)
+    served = srv.run(reqs, realtime=True)
+    print(f"[bench] served in {served['wall_s']:.2f} s ({served['ticks']} ticks)", file=sys.stderr,
+          flush=True)
+    plain = serve.run_plain(reqs, first_states=True)
+    print(f"[bench] plain in {plain['wall_s']:.2f} s", file=sys.stderr, flush=True)
+    by_rid = {r.rid: r for r in served["results"]}
+    same = (not served["failures"] and not plain["failures"] and len(by_rid) == len(reqs) and
+            all(np.array_equal(by_rid[rid].xyz, x) and np.array_equal(by_rid[rid].uvw, v)
+                for rid, x, v in plain["results"]))
+    roof = roofline_from_ticks(srv.tick_log, copy_gbs, None) if srv.tick_log else None
+    return {"reqs": reqs, "drawn_steps": drawn, "served": served, "plain": plain, "same": same,
+            "roofline": roof, "lane_pcg": srv.lane_pcg, "lanes": list(serve.DEFAULT_LANES),
+            "solo_from": srv.solo_from}
+
+
+def fold_leg(steps: int, copy_gbs) -> dict:
+    """The single 256x256 folding cloth, plain step, with the ILU sweep's
+    launch time from HIP events on the cloth's stream."""
+    from cloth_sim_self_collision_amd.sim import DeviceCloth
+    cl = scene.fold_sweep(256)
+    with DeviceCloth(cl) as d:
+        d.step(1)
+        d.profile(True, sweep_only=True)
+        lat, its = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            its.append(d.step(1)["pcg_iters"])
+            lat.append(time.perf_counter() - t1)
+        wall = time.perf_counter() - t0
+        ms, n = d.profile_read()["ilu_sweep"]
+        d.profile(False)
+        b = algorithmic_bytes("ilu_sweep", d.np_, d.nnz)
+        np_ = d.np_
+    ach = b / (ms / n * 1e-3) / 1e9 if n else None
+    print(f"[bench] fold256: {steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
+    return {"workload": "fold256", "np": np_, "nt": cl.nt, "steps": steps,
+            "vertex_steps_per_s": np_ * steps / wall, "ms_per_step": wall / steps * 1e3,
+            "p50_step_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
+            "p99_step_ms": float(np.percentile(np.array(lat) * 1e3, 99)), "pcg_iters": its,
+            "roofline": {"kernel": "ilu_sweep", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS if ach else None, "bytes_per_launch": b,
+                         "avg_launch_us": ms / n * 1e3 if n else None, "launches": n,
+                         "traffic": TRAFFIC_BYTES.get(("fold256", "ilu_sweep")),
+                         "frac_of_stream_copy": ach / copy_gbs if (ach and copy_gbs) else None}}
+
+
+def batch_leg(count: int, steps: int, copy_gbs) -> dict:
+    """The batch of `count` cloths (sides 24..128, seed 0, each dropped onto
+    its own sphere or box): one plain round (each cloth one step alone), then
+    `steps` steps of all of them in one cs_batch_step each."""
     from cloth_sim_self_collision_amd.sim import DeviceBatch, DeviceCloth
-    cls = scene.batch_drop(count, seed=seed)
+    cls = scene.batch_drop(count, seed=0)
     nv = sum(c.np_ for c in cls)
     devs = [DeviceCloth(c) for c in cls]
     try:
-        for d in devs:  # warm-up: one step each
-            d.step(1)
         t0 = time.perf_counter()
         for d in devs:
             d.step(1)
         t_plain = time.perf_counter() - t0
         print(f"[bench] batch: plain round {t_plain:.2f} s", file=sys.stderr, flush=True)
+        ticks = []
         with DeviceBatch(devs) as b:
             b.step(1)  # warm-up of the batched kernels
+            b.profile(True)
             lat = []
             t0 = time.perf_counter()
             for _ in range(steps):
                 t1 = time.perf_counter()
                 st = b.step(1)
                 lat.append(time.perf_counter() - t1)
+                ticks.append({"prof": b.profile_read(),
+                              "members": [(d.np_, d.nnz, s["pcg_iters"], s["status"])
+                                          for d, s in zip(devs, st)]})
+                b.profile(True)
             t_batch = time.perf_counter() - t0
+            b.profile(False)
         print(f"[bench] batch: {steps} batched steps {t_batch:.2f} s", file=sys.stderr, flush=True)
-        lat_ms = np.array(lat) * 1e3
-        plain = nv / t_plain
-        batched = nv * steps / t_batch
-        return {"workload": f"batch_drop{count}", "cloths": count, "vertices": nv,
-                "plain_vertex_steps_per_s": plain, "batched_vertex_steps_per_s": batched,
-                "speedup": batched / plain, "batched_p50_step_ms": float(np.percentile(lat_ms, 50)),
-                "batched_p99_step_ms": float(np.percentile(lat_ms, 99)),
-                "pcg_iters_max": max(s["pcg_iters"] for s in st)}
     finally:
         for d in devs:
             d.close()
+    lat_ms = np.array(lat) * 1e3
+    plain = nv / t_plain
+    batched = nv * steps / t_batch
+    return {"workload": f"batch_drop{count}", "cloths": count, "vertices": nv, "steps": steps,
+            "plain_vertex_steps_per_s": plain, "batched_vertex_steps_per_s": batched,
+            "batched_vs_plain": batched / plain, "batched_p50_step_ms": float(np.percentile(lat_ms, 50)),
+            "batched_p99_step_ms": float(np.percentile(lat_ms, 99)),
+            "pcg_iters_max": max(s["pcg_iters"] for s in st),
+            "roofline": roofline_from_ticks(ticks, copy_gbs)}, cls
 
 
-def serve_measure(count: int, rate: float, seed: int = 0) -> dict:
-    """A bounded sample of the serving workload (Poisson arrivals, rectangular
-    cloths of log-uniform triangle counts and aspect 1..4, sphere or box):
-    served with continuous batching, then the same requests one after another
-    with the plain step."""
-    from cloth_sim_self_collision_amd import serve
-    reqs = serve.poisson_stream(count, rate, seed=seed, tri_range=(1.0e3, 6.0e3),
-                                step_range=(5, 15))
-    served = serve.Server(hbm_budget_bytes=64 << 30).run(reqs, realtime=True)
-    print(f"[bench] served {count} requests in {served['wall_s']:.2f} s", file=sys.stderr, flush=True)
-    plain = serve.run_plain(reqs)
-    print(f"[bench] plain {count} requests in {plain['wall_s']:.2f} s", file=sys.stderr, flush=True)
-    same = all(np.array_equal(a.xyz, b[1]) and np.array_equal(a.uvw, b[2])
-               for a, b in zip(served["results"], plain["results"]))
-    return {"workload": f"poisson{count}", "requests": count, "rate_per_s": rate,
-            "triangles": [int(r.cloth.nt) for r in reqs], "steps": [r.steps for r in reqs],
-            "served_vertex_steps_per_s": served["vertex_steps_per_s"],
-            "served_p50_step_ms": served["p50_step_ms"], "served_p99_step_ms": served["p99_step_ms"],
-            "served_wall_s": served["wall_s"],
-            "plain_vertex_steps_per_s": plain["vertex_steps_per_s"],
-            "plain_p50_step_ms": plain["p50_step_ms"], "plain_p99_step_ms": plain["p99_step_ms"],
-            "plain_wall_s": plain["wall_s"],
-            "served_vs_plain": served["vertex_steps_per_s"] / plain["vertex_steps_per_s"],
-            "results_identical": bool(same)}
+def drape_leg(steps: int) -> dict:
+    """The 32x32 drape over a sphere (1,024 vertices), plain step."""
+    from cloth_sim_self_collision_amd.sim import DeviceCloth
+    cl = scene.drape_over_sphere(32)
+    with DeviceCloth(cl) as d:
+        d.step(2)
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            d.step(1)
+            lat.append(time.perf_counter() - t1)
+        wall = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+    return {"workload": "drape32", "np": cl.np_, "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "vertex_steps_per_s": cl.np_ * steps / wall,
+            "p50_step_ms": float(np.percentile(lat_ms, 50)),
+            "p99_step_ms": float(np.percentile(lat_ms, 99))}, cl
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3,
+                    help="steps per request of the served stream (cap on the drawn 50..300)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fold256")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
-    ap.add_argument("--maxit", type=int, default=None,
-                    help="PCG iteration cap override (short profiling runs only)")
-    ap.add_argument("--sweep-variant", type=int, default=0,
-                    help="ILU sweep implementation (CS_OPT_SWEEP; results are identical)")
-    ap.add_argument("--batch-count", type=int, default=64,
-                    help="cloths of the batch workload measured beside the headline (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=2)
-    ap.add_argument("--serve-count", type=int, default=12,
-                    help="requests of the serving sample measured beside the headline (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, 1/s")
+    ap.add_argument("--requests", type=int, default=16, help="requests of the served stream")
+    ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate, 1/s")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--lane-pcg", default="", help="PCG variant per lane, e.g. 2,2,0")
+    ap.add_argument("--solo-from", type=int, default=-1,
+                    help="vertex count from which a request runs alone (-1: the server's default, "
+                         "0: never)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (HIP's default is 4; at most 32)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="CPU baseline processes (at most the cores this process may use)")
+    ap.add_argument("--no-profile", action="store_true", help="no per-class HIP events")
+    ap.add_argument("--fold-steps", type=int, default=3, help="steps of fold256 (0: skip)")
+    ap.add_argument("--batch-count", type=int, default=64, help="cloths of the batch (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--drape-steps", type=int, default=50, help="steps of drape32 (0: skip)")
     ap.add_argument("--copy-mb", type=float, default=512.0,
                     help="size of the streaming-copy probe that measures the copy peak")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(every box here has one GPU)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box this "
+              "runs on has one GPU; the sharded design is described but not built)", file=sys.stderr)
         return 2
 
+    # hardware queues: every lane and solo request steps on streams of its own
+    if args.hw_queues > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     from cloth_sim_self_collision_amd import _lib
-    from cloth_sim_self_collision_amd.sim import DeviceCloth
     if _lib.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         return 1
-
-    from cloth_sim_self_collision_amd.sim import bandwidth_probe, issue_probe, read_probe
+    from cloth_sim_self_collision_amd.sim import bandwidth_probe
     n_copy = int(args.copy_mb * 2**20 / 8) // 2 * 2
     copy_gbs = bandwidth_probe(n_copy, 10) if n_copy > 0 else None
-    # the one-workgroup sweep streams its factor through one CU: that CU's
-    # read ceiling, over a buffer of the factor's size
-    one_cu_gbs = read_probe(1 << 23, 5, 1) if n_copy > 0 else None
-    cloth = workload(args.workload)
-    if args.maxit is not None:
-        cloth.pcg_maxit = args.maxit
-    d = DeviceCloth(cloth)
-    d.set_option(1, args.sweep_variant)
-    # the sweep runs on one CU as a chain of dependent sub-levels, each paced by
-    # its waves' vector-load instructions: the issue floor prices them at the
-    # cost one load instruction has in a workgroup of the same shape
-    shape = d.sweep_shape()
-    issue_ns = issue_probe(shape["waves"]) if (shape["quad"] and n_copy > 0) else None
-    # warmup; the first step is also compared against the CPU reference
-    first_x = None
-    for k in range(max(args.warmup, 1)):
-        t0 = time.perf_counter()
-        d.step(1)
-        print(f"[bench] warmup step {k}: {time.perf_counter() - t0:.3f} s", file=sys.stderr,
-              flush=True)
-        if k == 0:
-            first_x, _ = d.get_state()
-    # HIP events around each launch of the ILU sweep (the dominant kernel) on
-    # the cloth's stream over the timed steps: two per launch.  Events around
-    # every kernel class would cost the step about 4%, so the per-class
-    # breakdown comes from one more, untimed step below.
-    if not args.no_profile:
-        d.profile(True, sweep_only=True)
-    lat = []
-    stats = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        stats.append(d.step(1))
-        lat.append(time.perf_counter() - t0)
-        print(f"[bench] step {len(lat) - 1}: {lat[-1]:.3f} s, pcg {stats[-1]['pcg_iters']}",
-              file=sys.stderr, flush=True)
-    t_total = time.perf_counter() - t_start
-    sweep_timed = d.profile_read()["ilu_sweep"] if not args.no_profile else None
-    prof, red = {}, None
-    if not args.no_profile:
-        d.profile(True)
-        t0 = time.perf_counter()
-        d.step(1)
-        print(f"[bench] profiled step (every kernel class): {time.perf_counter() - t0:.3f} s",
-              file=sys.stderr, flush=True)
-        prof = d.profile_read()
-        red = d.reduction_stats()
-    d.profile(False)
 
-    value = cloth.np_ * args.steps / t_total
-    lat_ms = np.array(lat) * 1e3
-    roof = None
-    if prof:
-        kinds = {k: v for k, v in prof.items() if v[1] > 0}
-        dom = max(kinds, key=lambda k: kinds[k][0])
-        share = kinds[dom][0] / sum(v[0] for v in kinds.values())
-        if dom == "ilu_sweep" and sweep_timed and sweep_timed[1] > 0:
-            ms, n = sweep_timed
-            over = "timed steps"
-        else:
-            ms, n = kinds[dom]
-            over = "profiled step"
-        b = algorithmic_bytes(dom, d.np_, d.nnz)
-        avg_s = ms / n * 1e-3
-        ach = b / avg_s / 1e9 if b > 0 else None
-        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": (ach / HBM_PEAK_GBS) if ach else None,
-                "traffic": TRAFFIC_BYTES.get((cloth.name, dom)),
-                "bytes_per_launch": b, "avg_launch_us": avg_s * 1e6, "launches": n,
-                "launches_timed_over": over, "share_of_gpu_time": share,
-                "stream_copy_gbs": copy_gbs,
-                "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None,
-                "one_cu_read_gbs": one_cu_gbs,
-                "frac_of_one_cu_read": (ach / one_cu_gbs) if (ach and one_cu_gbs) else None}
-        if dom == "ilu_sweep" and issue_ns:
-            loads = shape["wave_loads"]
-            floor_us = loads * issue_ns * 1e-3
-            roof.update({"sweep_shape": shape, "ns_per_wave_load": issue_ns,
-                         "wave_loads_per_launch": loads, "issue_floor_us": floor_us,
-                         "frac_of_issue_floor": floor_us / (avg_s * 1e6)})
-    d.close()
-    batch = batch_measure(args.batch_count, args.batch_steps) if args.batch_count > 0 else None
-    served = serve_measure(args.serve_count, args.serve_rate) if args.serve_count > 0 else None
-    cpu, err = (None, None) if args.no_cpu else cpu_baseline(workload(args.workload), first_x)
+    sv = serve_leg(args, copy_gbs)
+    served, plain, reqs = sv["served"], sv["plain"], sv["reqs"]
+    fold = fold_leg(args.fold_steps, copy_gbs) if args.fold_steps > 0 else None
+    batch, batch_cls = batch_leg(args.batch_count, args.batch_steps, copy_gbs) \
+        if args.batch_count > 0 else (None, None)
+    drape, drape_cl = drape_leg(args.drape_steps) if args.drape_steps > 0 else (None, None)
+
+    cpu, err = None, None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu, first = cpu_baseline([(r.rid, r.cloth, 1) for r in reqs], args.cpu_procs,
+                                  f"the first step of each of the {len(reqs)} requests, one "
+                                  "process per core, largest first")
+        err = max(float(np.abs(first[rid] - x).max()) for rid, x in plain["first"].items()
+                  if rid in first)
+        print(f"[bench] cpu served sample {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
+        if batch is not None:
+            batch["cpu_baseline"], _ = cpu_baseline(
+                [(k, c, 1) for k, c in enumerate(batch_cls)], args.cpu_procs,
+                f"one step of each of the {len(batch_cls)} cloths, one process per core")
+        if drape is not None:
+            drape["cpu_baseline"], _ = cpu_baseline(
+                [("drape32", drape_cl, args.drape_steps)], 1,
+                f"{args.drape_steps} steps of drape32 on one core")
+        print(f"[bench] cpu baselines done {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
+
+    nt = [r.cloth.nt for r in reqs]
+    by_rid = {r.rid: r for r in served["results"]}
     out = {
-        "metric": "vertex-steps/s + p99 step latency, served vs plain GPU, at <=1e-5 m pos err",
-        "value": value, "unit": "vertex-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": t_total / args.steps * 1e3,
+        "metric": METRIC, "value": served["vertex_steps_per_s"], "unit": "vertex-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": served["wall_s"] / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": cloth.name, "np": cloth.np_, "nt": cloth.nt, "nq": cloth.nq,
-                   "dt": cloth.dt, "pcg_tol": cloth.pcg_tol, "pcg_maxit": cloth.pcg_maxit,
-                   "path": "plain GPU step, FP64 bit-exact"},
-        "p50_step_ms": float(np.percentile(lat_ms, 50)),
-        "p99_step_ms": float(np.percentile(lat_ms, 99)),
+        "config": {"workload": "served Poisson stream of drop scenes", "requests": len(reqs),
+                   "arrival_rate_per_s": args.rate, "seed": args.seed,
+                   "tri_range": [1.0e3, 2.0e5], "aspect_range": [1.0, 4.0],
+                   "triangles_drawn": [min(nt), max(nt)], "triangles_total": int(sum(nt)),
+                   "vertices_total": int(sum(r.cloth.np_ for r in reqs)),
+                   "steps_drawn_range": [50, 300], "step_cap": args.steps,
+                   "steps_per_request": [r.steps for r in reqs],
+                   "dt": reqs[0].cloth.dt, "pcg_tol": reqs[0].cloth.pcg_tol,
+                   "pcg_maxit": reqs[0].cloth.pcg_maxit, "lanes_vertex_bounds": sv["lanes"],
+                   "lane_pcg": list(sv["lane_pcg"]), "solo_from_vertices": sv["solo_from"],
+                   "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+                   "precision": "f64, bit-exact"},
+        "p50_step_ms": served["p50_step_ms"], "p99_step_ms": served["p99_step_ms"],
+        "plain": {"vertex_steps_per_s": plain["vertex_steps_per_s"], "wall_s": plain["wall_s"],
+                  "p50_step_ms": plain["p50_step_ms"], "p99_step_ms": plain["p99_step_ms"]},
+        "served_wall_s": served["wall_s"],
+        "served_vs_plain": served["vertex_steps_per_s"] / plain["vertex_steps_per_s"],
+        "served_equals_plain_bitwise": bool(sv["same"]),
         "pos_err_m": err,
-        "pcg_iters": [s["pcg_iters"] for s in stats],
-        "contacts": {"n_prox": stats[-1]["n_prox"], "n_ccd": stats[-1]["n_ccd"],
-                     "n_riz_pass": stats[-1]["n_riz_pass"]},
-        "kernel_ms_profiled_step": {k: round(v[0], 3) for k, v in prof.items()} if prof else None,
-        "reduction_stats": red,
-        "roofline": roof,
+        "pos_err_scope": "first step of every request: GPU plain step vs the CPU baseline's step",
+        "pcg_iters": {str(r.rid): by_rid[r.rid].pcg_iters for r in reqs if r.rid in by_rid},
+        "lane_ticks": served["lanes"], "solo_ticks": served["solo"],
+        "roofline": sv["roofline"],
         "cpu_baseline": cpu,
-        "batch": batch,
-        "served": served,
+        "fold256": fold, "batch64": batch, "drape32": drape,
     }
     print(json.dumps(out))
     return 0

@@ -74,6 +74,13 @@ BYTES_PER_VERTEX_ESTIMATE = 8192
 # Cloths of the first two lanes run their PCG solve in one workgroup each
 # (3 np <= 65536); the last lane's cloths use one launch per PCG operation.
 DEFAULT_LANES = (2500, 21846)
+# PCG variant per default lane (CS_OPT_PCG)
+DEFAULT_LANE_PCG = (0, 0, 0)
+_OPT_PCG = 2
+# requests of at least this many vertices are stepped alone, each by its own
+# thread on its own stream, instead of joining a lane (None: never)
+DEFAULT_SOLO_FROM: Optional[int] = None
+_NONFINITE = "the solve produced an infinite or NaN displacement (the ILU(0)-PCG iterate overflowed)"
 
 
 @dataclasses.dataclass
@@ -125,9 +132,10 @@ class _Active:
 
 
 class _Lane:
-    """One host thread stepping a set of resident cloths together."""
+    """One host thread stepping a set of resident cloths together, or (solo)
+    one cloth alone with the plain step until it leaves."""
 
-    def __init__(self, server: "Server", index: int):
+    def __init__(self, server: "Server", index: int, solo: Optional[_Active] = None):
         self.server = server
         self.index = index
         self.inbox: List[_Active] = []
@@ -135,7 +143,9 @@ class _Lane:
         self.batch: Optional[DeviceBatch] = None
         self.members: tuple = ()
         self.ticks = 0
-        self.thread = threading.Thread(target=self._run, name=f"lane{index}", daemon=True)
+        self.solo = solo
+        target = self._run if solo is None else self._run_solo
+        self.thread = threading.Thread(target=target, name=f"lane{index}", daemon=True)
 
     def _close_batch(self):
         if self.batch is not None:
@@ -168,6 +178,8 @@ class _Lane:
                     self._close_batch()
                     self.batch = DeviceBatch([a.dev for a in self.active])
                     self.members = key
+                    if srv.profile:
+                        self.batch.profile(True)
                 ts = srv.clock()
                 try:
                     stats = self.batch.step(1)
@@ -181,18 +193,22 @@ class _Lane:
                     continue
                 dt_ms = (srv.clock() - ts) * 1e3
                 self.ticks += 1
+                if srv.profile:
+                    # per-class device time of this tick's shared launches, and
+                    # what each member did in it
+                    prof = self.batch.profile_read()
+                    self.batch.profile(True)
+                    rec = {"lane": self.index, "ms": dt_ms, "prof": prof,
+                           "members": [(a.dev.np_, a.dev.nnz, st["pcg_iters"], st["status"])
+                                       for a, st in zip(self.active, stats)]}
+                    with srv.cv:
+                        srv.tick_log.append(rec)
                 leaving = []
                 for a, st in zip(list(self.active), stats):
                     if st["status"] != 0:
-                        leaving.append((a, "the solve produced an infinite or NaN displacement "
-                                           "(the ILU(0)-PCG iterate overflowed)", False))
+                        leaving.append((a, _NONFINITE, False))
                         continue
-                    a.done += 1
-                    a.step_ms.append(dt_ms)
-                    a.pcg_iters.append(st["pcg_iters"])
-                    a.impulse_applied.append(bool(st["impulse_applied"]))
-                    a.n_prox.append(int(st["n_prox"]))
-                    a.n_ccd.append([int(c) for c in st["n_ccd"][:st["n_ccd_pass"]]])
+                    self._record(a, st, dt_ms)
                     if a.done >= a.req.steps:
                         leaving.append((a, None, True))
                 if leaving:
@@ -206,6 +222,52 @@ class _Lane:
                 srv.cv.notify_all()
         finally:
             self._close_batch()
+
+    def _record(self, a: _Active, st: dict, dt_ms: float) -> None:
+        a.done += 1
+        a.step_ms.append(dt_ms)
+        a.pcg_iters.append(st["pcg_iters"])
+        a.impulse_applied.append(bool(st["impulse_applied"]))
+        a.n_prox.append(int(st["n_prox"]))
+        a.n_ccd.append([int(c) for c in st["n_ccd"][:st["n_ccd_pass"]]])
+
+    def _run_solo(self):
+        """Step one cloth alone (its own stream) until it finishes, fails, is
+        cancelled or evicted."""
+        srv = self.server
+        a = self.solo
+        try:
+            if srv.profile:
+                a.dev.profile(True)
+            while True:
+                if a.cancel or a.evict:
+                    self._leave(a, "cancelled" if a.cancel else None)
+                    return
+                ts = srv.clock()
+                try:
+                    st = a.dev.step(1)
+                except _lib.ClothSimError as e:
+                    self._leave(a, f"step failed: {e}")
+                    return
+                dt_ms = (srv.clock() - ts) * 1e3
+                self.ticks += 1
+                if srv.profile:
+                    rec = {"lane": self.index, "ms": dt_ms, "prof": a.dev.profile_read(),
+                           "members": [(a.dev.np_, a.dev.nnz, st["pcg_iters"], st["status"])]}
+                    a.dev.profile(True)
+                    with srv.cv:
+                        srv.tick_log.append(rec)
+                if st["status"] != 0:
+                    self._leave(a, _NONFINITE)
+                    return
+                self._record(a, st, dt_ms)
+                if a.done >= a.req.steps:
+                    self._leave(a, None, finished=True)
+                    return
+        except BaseException as e:  # noqa: BLE001 - reported, the server stops
+            with srv.cv:
+                srv.fatal = e
+                srv.cv.notify_all()
 
     def _leave(self, a: _Active, error: Optional[str], finished: bool = False):
         """Retire a request from this lane: result, error, or eviction."""
@@ -243,12 +305,34 @@ class Server:
 
     hbm_budget_bytes: device memory the resident cloths may hold;
     lanes: vertex-count boundaries of the lanes (one lane per interval; () is
-    a single lane that steps every resident cloth together)."""
+    a single lane that steps every resident cloth together);
+    lane_pcg: the PCG variant (CS_OPT_PCG) of each lane's cloths, one value
+    per lane (0 automatic, 2 the whole solve in one workgroup per cloth, for
+    cloths that allow it); None: DEFAULT_LANE_PCG when lanes is
+    DEFAULT_LANES, else automatic everywhere;
+    profile: time each tick's shared launches by kernel class (HIP events on
+    the lane's stream) and keep a record per tick in tick_log;
+    solo_from: requests of at least this many vertices are stepped alone by a
+    thread of their own (plain steps on the cloth's stream) instead of
+    joining a lane, so a large cloth never paces smaller ones (None: never).
+    Set GPU_MAX_HW_QUEUES (HIP's hardware queues per process, 4 by default)
+    to at least the number of lanes plus solo requests before HIP starts, or
+    streams share queues and their kernels run one after another."""
 
     def __init__(self, hbm_budget_bytes: int, clock: Callable[[], float] = time.perf_counter,
                  on_result: Optional[Callable[["Result"], None]] = None,
                  on_error: Optional[Callable[["Failure"], None]] = None,
-                 lanes: tuple = DEFAULT_LANES):
+                 lanes: tuple = DEFAULT_LANES, lane_pcg: Optional[tuple] = None,
+                 profile: bool = False, solo_from: Optional[int] = DEFAULT_SOLO_FROM):
+        if lane_pcg is None:
+            lane_pcg = DEFAULT_LANE_PCG if tuple(lanes) == DEFAULT_LANES else (0,) * (len(lanes) + 1)
+        if len(lane_pcg) != len(lanes) + 1:
+            raise ValueError("lane_pcg needs one value per lane")
+        self.lane_pcg = tuple(int(v) for v in lane_pcg)
+        self.profile = bool(profile)
+        self.solo_from = solo_from
+        self.solo_lanes: List[_Lane] = []
+        self.tick_log: List[dict] = []
         self.budget = int(hbm_budget_bytes)
         self.clock = clock
         self.on_result = on_result
@@ -319,6 +403,7 @@ class Server:
         if given, is called from the admission loop with the elapsed time
         (for cancel/evict)."""
         self.realtime = realtime
+        self.solo_lanes = []
         self.pending = [_Active(req=r) for r in sorted(requests, key=lambda r: (r.arrival, str(r.rid)))]
         self._by_rid = {a.req.rid: a for a in self.pending}
         self.live = {id(a) for a in self.pending}
@@ -368,15 +453,18 @@ class Server:
                 for ln in lanes:
                     for a in ln.active + ln.inbox:
                         a.cancel = True
+                for ln in self.solo_lanes:
+                    ln.solo.cancel = True
                 self.cv.notify_all()
-            for ln in lanes:
+            for ln in lanes + self.solo_lanes:
                 ln.thread.join()
         wall = self.clock() - self.t0
         lat = np.array([m for r in self.results for m in r.step_ms])
         vsteps = sum(r.vertices * len(r.step_ms) for r in self.results)
         return {"results": sorted(self.results, key=lambda r: str(r.rid)),
                 "failures": list(self.failures), "wall_s": wall,
-                "ticks": sum(ln.ticks for ln in lanes), "lanes": [ln.ticks for ln in lanes],
+                "ticks": sum(ln.ticks for ln in lanes + self.solo_lanes),
+                "lanes": [ln.ticks for ln in lanes], "solo": [ln.ticks for ln in self.solo_lanes],
                 "vertex_steps_per_s": vsteps / wall if wall > 0 else 0.0,
                 "p50_step_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
                 "p99_step_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0}
@@ -403,6 +491,12 @@ class Server:
                     self._fail(a.req.rid, f"cannot create the cloth: {e}")
                 self.cv.notify_all()
             return
+        lane = self._lane_of(a.req.cloth.np_)
+        if self.lane_pcg[lane] != 0:
+            try:
+                dev.set_option(_OPT_PCG, self.lane_pcg[lane])
+            except _lib.ClothSimError:
+                pass  # a cloth the variant does not apply to keeps the automatic one
         with self.cv:
             mem = dev.memory()
             self.resident += max(mem, a.memory) - a.memory
@@ -410,25 +504,38 @@ class Server:
             a.dev = dev
             if a.admitted_s < 0:
                 a.admitted_s = self.clock() - self.t0
-            lanes[self._lane_of(a.req.cloth.np_)].inbox.append(a)
+            if self.solo_from is not None and a.req.cloth.np_ >= self.solo_from:
+                ln = _Lane(self, len(lanes) + len(self.solo_lanes), solo=a)
+                self.solo_lanes.append(ln)
+                ln.thread.start()
+            else:
+                lanes[lane].inbox.append(a)
             self.cv.notify_all()
 
 
-def run_plain(requests: List[Request]) -> dict:
+def run_plain(requests: List[Request], first_states: bool = False) -> dict:
     """The same requests one after another, each cloth stepped alone (the
     plain GPU step), for comparison.  A request whose step fails is recorded
-    in "failures" and contributes no steps."""
+    in "failures" and contributes no steps.  first_states: also return each
+    request's positions after its first step ("first": rid -> x), read
+    outside the timed steps."""
     t0 = time.perf_counter()
     results, lat, failures = [], [], []
+    first = {}
+    t_read = 0.0
     vsteps = 0
     for r in sorted(requests, key=lambda r: (r.arrival, str(r.rid))):
         try:
             with DeviceCloth(r.cloth) as d:
                 ms = []
-                for _ in range(r.steps):
+                for k in range(r.steps):
                     t1 = time.perf_counter()
                     d.step(1)
                     ms.append((time.perf_counter() - t1) * 1e3)
+                    if first_states and k == 0:
+                        t2 = time.perf_counter()
+                        first[r.rid] = d.get_state()[0]
+                        t_read += time.perf_counter() - t2
                 x, v = d.get_state()
         except _lib.ClothSimError as e:
             failures.append(Failure(r.rid, str(e)))
@@ -436,9 +543,9 @@ def run_plain(requests: List[Request]) -> dict:
         lat += ms
         vsteps += r.cloth.np_ * r.steps
         results.append((r.rid, x, v))
-    wall = time.perf_counter() - t0
+    wall = time.perf_counter() - t0 - t_read
     lat = np.array(lat)
-    return {"results": results, "failures": failures, "wall_s": wall,
+    return {"results": results, "failures": failures, "wall_s": wall, "first": first,
             "vertex_steps_per_s": vsteps / wall if wall > 0 else 0.0,
             "p50_step_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
             "p99_step_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0}

@@ -40,6 +40,15 @@ class FakeCloth:
     def memory(self):
         return self.np_ * 4096
 
+    def set_option(self, option, value):
+        self.option = (option, value)
+
+    nnz = 0
+
+    def step(self, n):
+        """The plain step of one cloth (solo requests)."""
+        return FakeBatch([self]).step(n)[0]
+
     def get_state(self):
         return self.cloth.xyz + self.k, self.cloth.uvw - self.k
 
@@ -106,6 +115,27 @@ def test_requests_in_several_lanes_all_finish(fake):
     for q in reqs:
         _check(by[q.rid], q)
     assert sum(1 for t in out["lanes"] if t > 0) == 3  # three lanes ticked
+
+
+def test_solo_requests_step_alone_beside_the_lanes(fake):
+    fake.nonfinite_at["c4"] = 1
+    reqs = [_req(0, 8, 5), _req(1, 60, 4, 0.01), _req(2, 20, 7, 0.02), _req(3, 70, 3, 0.03),
+            _req(4, 65, 3), _req(5, 80, 300)]
+    state = {"cancelled": False}
+
+    def control(srv, now):
+        if {r.rid for r in srv.results} >= {0, 1, 2, 3} and not state["cancelled"]:
+            srv.cancel(5)
+            state["cancelled"] = True
+
+    out = serve.Server(1 << 40, lanes=(500,), solo_from=3600).run(reqs, realtime=True,
+                                                                 control=control)
+    by = {r.rid: r for r in out["results"]}
+    for q in reqs[:4]:
+        _check(by[q.rid], q)
+    fails = {f.rid: f.error for f in out["failures"]}
+    assert set(fails) == {4, 5} and "NaN" in fails[4] and fails[5] == "cancelled"
+    assert len(out["solo"]) == 4  # requests 1, 3, 4, 5 (>= 3600 vertices) ran alone
 
 
 def test_budget_and_creation_failures_are_per_request(fake):
