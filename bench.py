@@ -52,12 +52,15 @@ sys.path.insert(0, ROOT)
 from cloth_sim_self_collision_amd import scene  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc, each
-# scaled by what the counter reads for k_stream_copy of known size), keyed by
-# (workload, kernel class); the passes and the scaling are in profiles/
-TRAFFIC_BYTES: dict = {
-    # profiles/fold256_hbm_traffic_pmc.md: (34064.5 * 2.000 + 4608.0 * 1.000) KiB
-    ("fold256", "ilu_sweep"): 74.48e6,
+# HBM bytes per launch over algorithmic bytes per launch, keyed by (workload,
+# kernel class): FETCH_SIZE + WRITE_SIZE from two rocprofv3 --pmc passes, each
+# scaled by what the counter reads for k_stream_copy of known size; the passes
+# and the scaling are in profiles/
+TRAFFIC_RATIO: dict = {
+    # profiles/fold256_hbm_traffic_pmc.md: (34065.4 * 2.000 + 4608.0) KiB / 59.98 MB
+    ("fold256", "ilu_sweep"): 1.242,
+    # profiles/served_stream_hbm_traffic_pmc.md: batched sweeps, 1.166 + 0.079
+    ("served", "ilu_sweep"): 1.245,
 }
 METRIC = "vertex-steps/s + p99 step latency, served vs plain GPU, at ≤1e-5 m pos err"
 
@@ -99,7 +102,7 @@ def step_bytes(kind: str, np_: int, nnz: int, iters: int) -> float:
     return 0.0
 
 
-def roofline_from_ticks(ticks: list, copy_gbs, traffic_key=None) -> dict:
+def roofline_from_ticks(ticks: list, copy_gbs, workload=None) -> dict:
     """Dominant kernel class of a batched run from per-tick records
     ({"prof": {class: (ms, launches)}, "members": [(np, nnz, iters, status)]}):
     its algorithmic bytes summed over every member's work ÷ its total launch
@@ -120,10 +123,13 @@ def roofline_from_ticks(ticks: list, copy_gbs, traffic_key=None) -> dict:
     ach = by[dom] / (ms[dom] * 1e-3) / 1e9 if by[dom] > 0 else None
     return {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (ach / HBM_PEAK_GBS) if ach else None,
-            "traffic": TRAFFIC_BYTES.get(traffic_key) if traffic_key else None,
+            "traffic": (TRAFFIC_RATIO[(workload, dom)] * by[dom] / n[dom]
+                        if (workload, dom) in TRAFFIC_RATIO else None),
             "bytes_per_launch": by[dom] / n[dom], "avg_launch_us": ms[dom] / n[dom] * 1e3,
             "launches": n[dom], "share_of_profiled_gpu_time": share,
             "class_ms": {k: round(v, 2) for k, v in sorted(ms.items(), key=lambda kv: -kv[1])},
+            "classes": {k: {"launches": n[k], "avg_launch_us": ms[k] / n[k] * 1e3,
+                            "bytes_per_launch": by[k] / n[k]} for k in ms},
             "stream_copy_gbs": copy_gbs,
             "frac_of_stream_copy": (ach / copy_gbs) if (ach and copy_gbs) else None}
 
@@ -208,42 +214,48 @@ def serve_leg(args, copy_gbs) -> dict:
     """The headline: the served stream and the same requests plain."""
     from cloth_sim_self_collision_amd import serve
     wire = serve.poisson_requests(args.requests, args.rate, seed=args.seed, tri_range=(1.0e3, 2.0e5),
-                                  aspect_range=(1.0, 4.0), step_range=(50, 300), pcg_maxit=1000)
+                                  aspect_range=(1.0, 4.0), step_range=(50, 300),
+                                  pcg_maxit=args.pcg_maxit)
     drawn = [w["steps"] for w in wire]
     for w in wire:
         w["steps"] = min(w["steps"], args.steps)
     reqs = [serve.parse_request(w, k) for k, w in enumerate(wire)]
-    lane_pcg = tuple(int(v) for v in args.lane_pcg.split(",")) if args.lane_pcg else None
+    kw = {"solo_from": None if args.solo_from == 0 else args.solo_from} if args.solo_from >= 0 else {}
+    if args.lanes:
+        kw["lanes"] = tuple(int(v) for v in args.lanes.split(","))
+    if args.lane_pcg:
+        kw["lane_pcg"] = tuple(int(v) for v in args.lane_pcg.split(","))
     # warm-up: the three smallest requests, served and plain, W steps each
     small = sorted(reqs, key=lambda r: r.cloth.np_)[:3]
     warm = [serve.Request(f"warm{k}", r.cloth, steps=max(1, args.warmup)) for k, r in enumerate(small)]
     if args.warmup > 0:
-        serve.Server(64 << 30, lane_pcg=lane_pcg).run(warm, realtime=False)
+        serve.Server(64 << 30, **kw).run(warm, realtime=False)
         serve.run_plain(warm[:1])
     print(f"[bench] serve: {len(reqs)} requests, {sum(r.cloth.nt for r in reqs)} triangles, "
           f"step cap {args.steps}", file=sys.stderr, flush=True)
-    solo = {"solo_from": None if args.solo_from == 0 else args.solo_from} if args.solo_from >= 0 else {}
-    srv = serve.Server(64 << 30, lane_pcg=lane_pcg, profile=not args.no_profile, **solo)
+    srv = serve.Server(64 << 30, profile=not args.no_profile, **kw)
     served = srv.run(reqs, realtime=True)
     print(f"[bench] served in {served['wall_s']:.2f} s ({served['ticks']} ticks)", file=sys.stderr,
           flush=True)
-    plain = serve.run_plain(reqs, first_states=True)
-    print(f"[bench] plain in {plain['wall_s']:.2f} s", file=sys.stderr, flush=True)
-    by_rid = {r.rid: r for r in served["results"]}
-    same = (not served["failures"] and not plain["failures"] and len(by_rid) == len(reqs) and
-            all(np.array_equal(by_rid[rid].xyz, x) and np.array_equal(by_rid[rid].uvw, v)
-                for rid, x, v in plain["results"]))
-    roof = roofline_from_ticks(srv.tick_log, copy_gbs, None) if srv.tick_log else None
+    plain, same = None, None
+    if not args.serve_only:
+        plain = serve.run_plain(reqs, first_states=True)
+        print(f"[bench] plain in {plain['wall_s']:.2f} s", file=sys.stderr, flush=True)
+        by_rid = {r.rid: r for r in served["results"]}
+        same = (not served["failures"] and not plain["failures"] and len(by_rid) == len(reqs) and
+                all(np.array_equal(by_rid[rid].xyz, x) and np.array_equal(by_rid[rid].uvw, v)
+                    for rid, x, v in plain["results"]))
+    roof = roofline_from_ticks(srv.tick_log, copy_gbs, "served") if srv.tick_log else None
     return {"reqs": reqs, "drawn_steps": drawn, "served": served, "plain": plain, "same": same,
-            "roofline": roof, "lane_pcg": srv.lane_pcg, "lanes": list(serve.DEFAULT_LANES),
+            "roofline": roof, "lane_pcg": srv.lane_pcg, "lanes": list(srv.bounds),
             "solo_from": srv.solo_from}
 
 
-def fold_leg(steps: int, copy_gbs) -> dict:
+def fold_leg(steps: int, copy_gbs, maxit: int) -> dict:
     """The single 256x256 folding cloth, plain step, with the ILU sweep's
     launch time from HIP events on the cloth's stream."""
     from cloth_sim_self_collision_amd.sim import DeviceCloth
-    cl = scene.fold_sweep(256)
+    cl = scene.fold_sweep(256, pcg_maxit=maxit)
     with DeviceCloth(cl) as d:
         d.step(1)
         d.profile(True, sweep_only=True)
@@ -267,16 +279,18 @@ def fold_leg(steps: int, copy_gbs) -> dict:
             "roofline": {"kernel": "ilu_sweep", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS if ach else None, "bytes_per_launch": b,
                          "avg_launch_us": ms / n * 1e3 if n else None, "launches": n,
-                         "traffic": TRAFFIC_BYTES.get(("fold256", "ilu_sweep")),
+                         "traffic": TRAFFIC_RATIO[("fold256", "ilu_sweep")] * b,
                          "frac_of_stream_copy": ach / copy_gbs if (ach and copy_gbs) else None}}
 
 
-def batch_leg(count: int, steps: int, copy_gbs) -> dict:
+def batch_leg(count: int, steps: int, copy_gbs, maxit: int) -> dict:
     """The batch of `count` cloths (sides 24..128, seed 0, each dropped onto
     its own sphere or box): one plain round (each cloth one step alone), then
     `steps` steps of all of them in one cs_batch_step each."""
     from cloth_sim_self_collision_amd.sim import DeviceBatch, DeviceCloth
     cls = scene.batch_drop(count, seed=0)
+    for c in cls:
+        c.pcg_maxit = maxit
     nv = sum(c.np_ for c in cls)
     devs = [DeviceCloth(c) for c in cls]
     try:
@@ -316,10 +330,11 @@ def batch_leg(count: int, steps: int, copy_gbs) -> dict:
             "roofline": roofline_from_ticks(ticks, copy_gbs)}, cls
 
 
-def drape_leg(steps: int) -> dict:
+def drape_leg(steps: int, maxit: int) -> dict:
     """The 32x32 drape over a sphere (1,024 vertices), plain step."""
     from cloth_sim_self_collision_amd.sim import DeviceCloth
     cl = scene.drape_over_sphere(32)
+    cl.pcg_maxit = maxit
     with DeviceCloth(cl) as d:
         d.step(2)
         lat = []
@@ -339,12 +354,14 @@ def drape_leg(steps: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3,
+    ap.add_argument("--steps", type=int, default=2,
                     help="steps per request of the served stream (cap on the drawn 50..300)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--requests", type=int, default=16, help="requests of the served stream")
+    ap.add_argument("--requests", type=int, default=64, help="requests of the served stream")
     ap.add_argument("--rate", type=float, default=20.0, help="Poisson arrival rate, 1/s")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--lanes", default="", help="lane vertex-count bounds, e.g. 2500,21846 "
+                                                "(default: the server's)")
     ap.add_argument("--lane-pcg", default="", help="PCG variant per lane, e.g. 2,2,0")
     ap.add_argument("--solo-from", type=int, default=-1,
                     help="vertex count from which a request runs alone (-1: the server's default, "
@@ -359,6 +376,12 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=64, help="cloths of the batch (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--drape-steps", type=int, default=50, help="steps of drape32 (0: skip)")
+    ap.add_argument("--pcg-maxit", type=int, default=1000,
+                    help="PCG iteration cap of every workload (1000; lower only for profiling "
+                         "runs, whose line is then not the benchmark)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="the served stream alone (profiling runs): no plain leg, no other "
+                         "workloads, no CPU baselines")
     ap.add_argument("--copy-mb", type=float, default=512.0,
                     help="size of the streaming-copy probe that measures the copy peak")
     args = ap.parse_args()
@@ -378,23 +401,30 @@ def main() -> int:
     n_copy = int(args.copy_mb * 2**20 / 8) // 2 * 2
     copy_gbs = bandwidth_probe(n_copy, 10) if n_copy > 0 else None
 
-    sv = serve_leg(args, copy_gbs)
-    served, plain, reqs = sv["served"], sv["plain"], sv["reqs"]
-    fold = fold_leg(args.fold_steps, copy_gbs) if args.fold_steps > 0 else None
-    batch, batch_cls = batch_leg(args.batch_count, args.batch_steps, copy_gbs) \
+    if args.requests <= 0:  # profiling runs of the other workloads
+        print("[bench] no served stream (--requests 0): no headline value", file=sys.stderr)
+    sv = serve_leg(args, copy_gbs) if args.requests > 0 else None
+    if args.serve_only:
+        args.fold_steps = args.batch_count = args.drape_steps = 0
+        args.no_cpu = True
+    fold = fold_leg(args.fold_steps, copy_gbs, args.pcg_maxit) if args.fold_steps > 0 else None
+    batch, batch_cls = batch_leg(args.batch_count, args.batch_steps, copy_gbs, args.pcg_maxit) \
         if args.batch_count > 0 else (None, None)
-    drape, drape_cl = drape_leg(args.drape_steps) if args.drape_steps > 0 else (None, None)
+    drape, drape_cl = drape_leg(args.drape_steps, args.pcg_maxit) if args.drape_steps > 0 \
+        else (None, None)
 
     cpu, err = None, None
     if not args.no_cpu:
         t0 = time.perf_counter()
-        cpu, first = cpu_baseline([(r.rid, r.cloth, 1) for r in reqs], args.cpu_procs,
-                                  f"the first step of each of the {len(reqs)} requests, one "
-                                  "process per core, largest first")
-        err = max(float(np.abs(first[rid] - x).max()) for rid, x in plain["first"].items()
-                  if rid in first)
-        print(f"[bench] cpu served sample {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-              flush=True)
+        if sv is not None:
+            reqs, plain = sv["reqs"], sv["plain"]
+            cpu, first = cpu_baseline([(r.rid, r.cloth, 1) for r in reqs], args.cpu_procs,
+                                      f"the first step of each of the {len(reqs)} requests, one "
+                                      "process per core, largest first")
+            err = max(float(np.abs(first[rid] - x).max()) for rid, x in plain["first"].items()
+                      if rid in first)
+            print(f"[bench] cpu served sample {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
         if batch is not None:
             batch["cpu_baseline"], _ = cpu_baseline(
                 [(k, c, 1) for k, c in enumerate(batch_cls)], args.cpu_procs,
@@ -406,9 +436,23 @@ def main() -> int:
         print(f"[bench] cpu baselines done {time.perf_counter() - t0:.1f} s", file=sys.stderr,
               flush=True)
 
+    out = headline(args, sv, err) if sv is not None else {
+        "metric": METRIC, "value": None, "unit": "vertex-steps/s", "n_gpus": 1,
+        "note": "no served stream (--requests 0): a profiling run of the other workloads"}
+    out.update({"cpu_baseline": cpu, "fold256": fold, "batch64": batch, "drape32": drape})
+    # vs_baseline stays null: the reference publishes no number for this metric
+    if cpu and out.get("value"):
+        out["value_vs_cpu_baseline"] = out["value"] / cpu["value"]
+    print(json.dumps(out))
+    return 0
+
+
+def headline(args, sv, err) -> dict:
+    """The served stream's part of the bench line."""
+    served, plain, reqs = sv["served"], sv["plain"], sv["reqs"]
     nt = [r.cloth.nt for r in reqs]
     by_rid = {r.rid: r for r in served["results"]}
-    out = {
+    return {
         "metric": METRIC, "value": served["vertex_steps_per_s"], "unit": "vertex-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": served["wall_s"] / args.steps * 1e3,
@@ -428,20 +472,17 @@ def main() -> int:
                    "precision": "f64, bit-exact"},
         "p50_step_ms": served["p50_step_ms"], "p99_step_ms": served["p99_step_ms"],
         "plain": {"vertex_steps_per_s": plain["vertex_steps_per_s"], "wall_s": plain["wall_s"],
-                  "p50_step_ms": plain["p50_step_ms"], "p99_step_ms": plain["p99_step_ms"]},
+                  "p50_step_ms": plain["p50_step_ms"], "p99_step_ms": plain["p99_step_ms"]}
+        if plain else None,
         "served_wall_s": served["wall_s"],
-        "served_vs_plain": served["vertex_steps_per_s"] / plain["vertex_steps_per_s"],
-        "served_equals_plain_bitwise": bool(sv["same"]),
+        "served_vs_plain": served["vertex_steps_per_s"] / plain["vertex_steps_per_s"] if plain else None,
+        "served_equals_plain_bitwise": sv["same"],
         "pos_err_m": err,
         "pos_err_scope": "first step of every request: GPU plain step vs the CPU baseline's step",
         "pcg_iters": {str(r.rid): by_rid[r.rid].pcg_iters for r in reqs if r.rid in by_rid},
         "lane_ticks": served["lanes"], "solo_ticks": served["solo"],
         "roofline": sv["roofline"],
-        "cpu_baseline": cpu,
-        "fold256": fold, "batch64": batch, "drape32": drape,
     }
-    print(json.dumps(out))
-    return 0
 
 
 if __name__ == "__main__":

@@ -5,12 +5,14 @@ every admitted simulation resident on the device (mesh, matrix, factor, BVH,
 state) until it finishes.  Admitted requests join one of a few *lanes* by
 size (vertex count); each lane is a host thread that steps its resident
 cloths together, one ``cs_batch_step(1)`` per tick, and picks up newly
-admitted requests at the next tick.  Lanes run concurrently on their own
-streams, so small cloths tick at their own pace instead of waiting for the
-largest resident cloth, and the work of all lanes overlaps on the device.
-Within a lane every cloth whose size allows it runs its whole PCG solve in
-its own workgroup, so cloths of one lane do not wait for each other's solver
-iterations either.
+admitted requests at the next tick.  Optionally, large requests
+(``solo_from`` vertices and up) run alone instead, each on a thread and
+stream of its own with the plain step.  Lanes (and solo requests) run
+concurrently, so small cloths tick at their own pace instead of waiting for
+the largest resident cloth, and the work of all of them overlaps on the
+device.  Cloths are
+created (host setup and upload) by a small pool of threads, so a large
+cloth's setup does not hold back the admission of the ones after it.
 
 Admission is by resident device footprint: a request is admitted once it has
 arrived and its estimated footprint (BYTES_PER_VERTEX_ESTIMATE per vertex, or
@@ -52,6 +54,7 @@ Results are written as requests finish, one line each, in completion order.
 from __future__ import annotations
 
 import base64
+import concurrent.futures as cf
 import dataclasses
 import json
 import sys
@@ -70,16 +73,23 @@ from .sim import DeviceBatch, DeviceCloth
 # the resident footprint (measured ~5.5 KB/vertex at 65k vertices, rounded up)
 BYTES_PER_VERTEX_ESTIMATE = 8192
 
-# lane boundaries by vertex count: [0, 2500), [2500, 21846), [21846, inf).
-# Cloths of the first two lanes run their PCG solve in one workgroup each
-# (3 np <= 65536); the last lane's cloths use one launch per PCG operation.
-DEFAULT_LANES = (2500, 21846)
-# PCG variant per default lane (CS_OPT_PCG)
-DEFAULT_LANE_PCG = (0, 0, 0)
+# lane boundaries by vertex count: six lanes, [0, 2500), [2500, 8000), ...,
+# [60000, inf).  A lane's tick lasts as long as its slowest member's step, so
+# members of similar size waste little; measured on a 64-request stream of the
+# serving workload (1e3..2e5 triangles): 141 000 vertex-steps/s with these
+# six lanes against 129 000 with five (2500, 8000, 21846, 50000), 122 000
+# with eight and 84 000 with every cloth of 21846 vertices or more alone.
+DEFAULT_LANES = (2500, 8000, 16000, 30000, 60000)
+# PCG variant per default lane (CS_OPT_PCG): the two smallest lanes run each
+# cloth's whole solve in one workgroup (one launch per step instead of about
+# seven per PCG iteration), the others one launch per PCG operation
+DEFAULT_LANE_PCG = (2, 2, 0, 0, 0, 0)
 _OPT_PCG = 2
 # requests of at least this many vertices are stepped alone, each by its own
 # thread on its own stream, instead of joining a lane (None: never)
 DEFAULT_SOLO_FROM: Optional[int] = None
+# cloths created at once on admission (host setup and upload)
+CREATORS = 4
 _NONFINITE = "the solve produced an infinite or NaN displacement (the ILU(0)-PCG iterate overflowed)"
 
 
@@ -408,6 +418,8 @@ class Server:
         self._by_rid = {a.req.rid: a for a in self.pending}
         self.live = {id(a) for a in self.pending}
         lanes = [_Lane(self, k) for k in range(len(self.bounds) + 1)]
+        pool = cf.ThreadPoolExecutor(max_workers=CREATORS, thread_name_prefix="create")
+        creating: list = []
         self.t0 = self.clock()
         for ln in lanes:
             ln.thread.start()
@@ -421,6 +433,9 @@ class Server:
                         self._fail(a.req.rid, f"needs about {need} bytes of device memory, more "
                                               f"than the budget of {self.budget}")
             while True:
+                for f in [f for f in creating if f.done()]:
+                    f.result()  # an unexpected error in a creation stops the server
+                    creating.remove(f)
                 now = self.clock() - self.t0
                 if control is not None:
                     control(self, now)
@@ -446,8 +461,9 @@ class Server:
                         nxt = [a.req.arrival - now for a in self.pending[:1] if realtime]
                         self.cv.wait(max(0.0, min([0.02] + nxt)))
                     continue
-                self._admit(cand, lanes)
+                creating.append(pool.submit(self._admit, cand, lanes))
         finally:
+            pool.shutdown(wait=True)
             with self.cv:
                 self.stopping = True
                 for ln in lanes:

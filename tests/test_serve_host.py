@@ -109,7 +109,7 @@ def _check(res, req):
 
 def test_requests_in_several_lanes_all_finish(fake):
     reqs = [_req(0, 8, 5), _req(1, 60, 3, 0.01), _req(2, 20, 7, 0.02), _req(3, 160, 2, 0.03)]
-    out = serve.Server(1 << 40, lanes=(500, 10000)).run(reqs, realtime=True)
+    out = serve.Server(1 << 40, lanes=(500, 10000), solo_from=None).run(reqs, realtime=True)
     assert not out["failures"]
     by = {r.rid: r for r in out["results"]}
     for q in reqs:
