@@ -195,6 +195,7 @@ struct Many {
   __device__ __forceinline__ Params params() const { return ptab[blockIdx.y]; }
 };
 
+
 // Optional per-kernel-class timing with HIP events recorded on the cloth's
 // stream around each launch (cs_profile_enable / cs_profile_read).
 enum ProfKind : int {

@@ -48,6 +48,46 @@
 
 namespace cs {
 
+// Loads and stores with global-memory instructions.  The compiler proves a
+// kernel argument's pointers global, but not the pointers of a Many table
+// entry, and accesses them with flat instructions; a flat access also counts
+// against the LDS wait counter, so in a loop that keeps loads in flight
+// across LDS traffic (the sweeps, the reductions) every LDS wait would wait
+// for those loads too.  Every buffer of a cloth is device (global) memory.
+template <class T>
+struct GlobalWord {
+  typedef T type;
+};
+template <>
+struct GlobalWord<double2> {
+  typedef double __attribute__((ext_vector_type(2))) type;
+};
+template <>
+struct GlobalWord<int4> {
+  typedef int __attribute__((ext_vector_type(4))) type;
+};
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+  typedef typename GlobalWord<T>::type W;
+  const W w = *(const __attribute__((address_space(1))) W*)p;
+  T v;
+  __builtin_memcpy(&v, &w, sizeof(T));
+  return v;
+}
+template <class T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+// the same pointer in the global address space (for scalar element types)
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 constexpr int kSumBlock = 256;                  // threads per block
 constexpr int kSumPer = 4;                      // consecutive elements per thread
 constexpr int kSumTile = kSumBlock * kSumPer;   // elements per tile
@@ -426,7 +466,7 @@ __device__ __forceinline__ double chain_tiles(double s, int b0, int wend, const 
 #pragma unroll
       for (int r = 0; r < kSumTile / 64; r++) {
         const int i = r * 64 + lane;
-        buf[i] = t[begin + min(i, len - 1)];
+        buf[i] = gload(t + begin + min(i, len - 1));
       }
       wave_lds_sync();
       s = wave_walk(buf, len, s, stats);
@@ -463,7 +503,7 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_tiles(Src src) {
   double a = 0.0;
 #pragma unroll
   for (int j = 0; j < kSumPer; j++)
-    if (base + j < n) a += t[base + j];
+    if (base + j < n) a += gload(t + base + j);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
   if (lane == 0) wd[wid] = a;
@@ -471,7 +511,7 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_tiles(Src src) {
   if (tid == 0) {
     double s = wd[0];
     for (int k = 1; k < kSumWaves; k++) s += wd[k];
-    job.w.tsum[blockIdx.x] = s;
+    gstore(job.w.tsum + blockIdx.x, s);
   }
 }
 
@@ -491,7 +531,7 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
   {
     // approximate running sum at the start of this tile -> binade guess
     double p = 0.0;
-    for (int j = tid; j < b; j += kSumBlock) p += w.tsum[j];
+    for (int j = tid; j < b; j += kSumBlock) p += gload(w.tsum + j);
     p = block_sum_approx(p, sh);
     int code = -1, scale = 0;
     if (fabs(p) >= DBL_MIN && fabs(p) <= DBL_MAX) {
@@ -504,7 +544,7 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
 #pragma unroll
     for (int j = 0; j < kSumPer; j++) {
       const int idx = b * kSumTile + tid * kSumPer + j;
-      const double x = idx < n ? t[idx] : 0.0;
+      const double x = idx < n ? gload(t + idx) : 0.0;
       if (ok) {
         const double q = ldexp(x, scale);
         const double r = rint(q);
@@ -526,10 +566,10 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
     }
     block_minmax64(mn, mx, sh);
     if (tid == 0) {
-      w.code[b] = ok ? code : -1;
-      w.tot[b] = total;
-      w.pmin[b] = mn;
-      w.pmax[b] = mx;
+      gstore(w.code + b, ok ? code : -1);
+      gstore(w.tot + b, total);
+      gstore(w.pmin + b, mn);
+      gstore(w.pmax + b, mx);
     }
   }
   // the last block to finish chains the tiles, on its first wave
@@ -553,10 +593,10 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int j = min(j0 + r * 64 + lane, wend - 1);
-        cd[r] = w.code[j];
-        tt[r] = w.tot[j];
-        mn[r] = w.pmin[j];
-        mx[r] = w.pmax[j];
+        cd[r] = gload(w.code + j);
+        tt[r] = gload(w.tot + j);
+        mn[r] = gload(w.pmin + j);
+        mx[r] = gload(w.pmax + j);
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -633,7 +673,7 @@ __device__ __forceinline__ double wg_ordered_sum(const double* __restrict__ t, i
 #pragma unroll 4
     for (int j = 0; j < kSumTile / 64; j++) {
       const int idx = b * kSumTile + j * 64 + lane;
-      if (idx < n) a += t[idx];
+      if (idx < n) a += gload(t + idx);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
@@ -659,7 +699,7 @@ __device__ __forceinline__ double wg_ordered_sum(const double* __restrict__ t, i
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
       const int idx = b * kSumTile + lane * kPer + j;
-      const double x = idx < n ? t[idx] : 0.0;
+      const double x = idx < n ? gload(t + idx) : 0.0;
       if (ok) {
         const double q = ldexp(x, scale);
         const double r = rint(q);

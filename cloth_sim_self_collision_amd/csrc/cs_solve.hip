@@ -308,9 +308,12 @@ __device__ __forceinline__ void fg_apply(double* sl, int nb, int code, const dou
 
 // row i by lane t of its group (every lane of the group calls this)
 __device__ void ilu_row_grouped(const DevCloth& c, int i, int* ising, double* sl, int nb, int t) {
-  const int rb = c.rowptr[i], re = c.rowptr[i + 1], di = c.diaind[i];
-  double* F = c.fcrs;
-  const int u0 = c.fu_ptr[i], u1 = c.fu_ptr[i + 1];
+  const int rb = gptr(c.rowptr)[i], re = gptr(c.rowptr)[i + 1], di = gptr(c.diaind)[i];
+  const auto F = gptr(c.fcrs);
+  const auto fdia = gptr(c.fdia);
+  const auto fu_code = gptr(c.fu_code);
+  const auto fu_kj = gptr(c.fu_kj);
+  const int u0 = gptr(c.fu_ptr)[i], u1 = gptr(c.fu_ptr)[i + 1];
   int code[kFgLayers];
   double ub[kFgLayers][9];
 #pragma unroll
@@ -318,8 +321,8 @@ __device__ void ilu_row_grouped(const DevCloth& c, int i, int* ising, double* sl
     const int e = u0 + t + m * kFgLanes;
     code[m] = -1;
     if (e < u1) {
-      code[m] = c.fu_code[e];
-      const size_t kj = (size_t)c.fu_kj[e];
+      code[m] = fu_code[e];
+      const size_t kj = (size_t)fu_kj[e];
 #pragma unroll
       for (int q = 0; q < 9; q++) ub[m][q] = F[kj * 9 + q];
     }
@@ -327,7 +330,7 @@ __device__ void ilu_row_grouped(const DevCloth& c, int i, int* ising, double* sl
   const int deg = re - rb;
   for (int q = t; q < 9 * deg; q += kFgLanes) sl[q] = F[(size_t)rb * 9 + q];
   double* d = sl + (nb - 1) * 9;
-  for (int q = t; q < 9; q += kFgLanes) d[q] = c.fdia[(size_t)i * 9 + q];
+  for (int q = t; q < 9; q += kFgLanes) d[q] = fdia[(size_t)i * 9 + q];
   fg_wave_sync();
   for (int k = 0; k < di - rb; k++) {
 #pragma unroll
@@ -335,9 +338,9 @@ __device__ void ilu_row_grouped(const DevCloth& c, int i, int* ising, double* sl
       if (code[m] >= 0 && (code[m] & 0xff) == k) fg_apply(sl, nb, code[m], ub[m]);
     // entries beyond the registers (rows with more than kFgLayers * kFgLanes updates)
     for (int e = u0 + t + kFgLayers * kFgLanes; e < u1; e += kFgLanes) {
-      const int cd = c.fu_code[e];
+      const int cd = fu_code[e];
       if ((cd & 0xff) != k) continue;
-      const size_t kj = (size_t)c.fu_kj[e];
+      const size_t kj = (size_t)fu_kj[e];
       double u[9];
       for (int q = 0; q < 9; q++) u[q] = F[kj * 9 + q];
       fg_apply(sl, nb, cd, u);
@@ -366,12 +369,12 @@ __device__ void ilu_row_grouped(const DevCloth& c, int i, int* ising, double* sl
     }
   }
   fg_wave_sync();
-  for (int q = t; q < 9; q += kFgLanes) c.fdia[(size_t)i * 9 + q] = d[q];
+  for (int q = t; q < 9; q += kFgLanes) fdia[(size_t)i * 9 + q] = d[q];
   for (int q = t; q < 9 * (di - rb); q += kFgLanes) F[(size_t)rb * 9 + q] = sl[q];  // lower blocks
   for (int q = t; q < 3 * (re - di); q += kFgLanes) {  // upper blocks: D^-1 U, one block row each
     const int b = q / 3, r = q - 3 * b;
     const double* w = sl + (di - rb + b) * 9;
-    double* u = F + (size_t)(di + b) * 9;
+    const auto u = F + (size_t)(di + b) * 9;
     u[r * 3 + 0] = d[r * 3 + 0] * w[0] + d[r * 3 + 1] * w[3] + d[r * 3 + 2] * w[6];
     u[r * 3 + 1] = d[r * 3 + 0] * w[1] + d[r * 3 + 1] * w[4] + d[r * 3 + 2] * w[7];
     u[r * 3 + 2] = d[r * 3 + 0] * w[2] + d[r * 3 + 1] * w[5] + d[r * 3 + 2] * w[8];
@@ -619,8 +622,8 @@ __device__ __forceinline__ QuadSrc<kFwd, K> quad_src(const DevCloth& c, const do
 // scalar base plus a 32-bit per-lane offset
 template <class T>
 __device__ __forceinline__ T ld32(const T* base, unsigned i) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
-                                     i * (unsigned)sizeof(T));
+  return gload(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                          i * (unsigned)sizeof(T)));
 }
 
 // Sub-level cuts of one sweep, read at wave-uniform indices.  CutsLds reads
@@ -728,7 +731,7 @@ __device__ __forceinline__ void quad_compute(double (*ring)[kRingLen], double* o
   }
   const bool own = q.has && lane < 3;
   ring[lane < 3 ? lane : 2][own ? (q.s & kMask) : kRingDummy] = z;
-  if (own) out[3 * q.out + lane] = z;  // forward: backward-slot order; backward: row order
+  if (own) gstore(out + 3 * q.out + lane, z);  // forward: backward-slot order; backward: row order
 }
 
 // sub-level cuts of one component's sweep kept in LDS (a sweep with more
@@ -809,7 +812,7 @@ __device__ __forceinline__ void quad_component(const DevCloth& c, int comp, doub
   const int f0 = c.qf_comp[comp], f1 = c.qf_comp[comp + 1];
   const bool fl = f1 - f0 <= kCutsLds;
   if (fl)
-    for (int i = threadIdx.x; i <= f1 - f0; i += blockDim.x) cuts[i] = c.qf_ptr[f0 + i];
+    for (int i = threadIdx.x; i <= f1 - f0; i += blockDim.x) cuts[i] = gload(c.qf_ptr + f0 + i);
   __syncthreads();
   if (fl)
     quad_sweep<true, K>(c, CutsLds{cuts, f0}, f0, f1, nullptr, c.sb, ring);
@@ -819,7 +822,7 @@ __device__ __forceinline__ void quad_component(const DevCloth& c, int comp, doub
   const int b0 = c.qb_comp[comp], b1 = c.qb_comp[comp + 1];
   const bool bl = b1 - b0 <= kCutsLds;
   if (bl)
-    for (int i = threadIdx.x; i <= b1 - b0; i += blockDim.x) cuts[i] = c.qb_ptr[b0 + i];
+    for (int i = threadIdx.x; i <= b1 - b0; i += blockDim.x) cuts[i] = gload(c.qb_ptr + b0 + i);
   __syncthreads();
   if (bl)
     quad_sweep<false, K>(c, CutsLds{cuts, b0}, b0, b1, c.sb, dst, ring);
@@ -845,13 +848,14 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
   quad_component<K>(c, comp, dst, ring, cuts);
   if (!prod) return;
   if (c.ncomp == 1) {
-    for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = src_row[e] * dst[e];
+    for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x)
+      gstore(prod + e, gload(src_row + e) * gload(dst + e));
   } else {  // this component's rows: its forward slots
     const int s0 = c.qf_ptr[c.qf_comp[comp]], s1 = c.qf_ptr[c.qf_comp[comp + 1]];
     for (int k = threadIdx.x; k < 3 * (s1 - s0); k += blockDim.x) {
       const int sl = s0 + k / 3;
-      const int e = 3 * c.f_rci[4 * sl] + (k - 3 * (k / 3));
-      prod[e] = src_row[e] * dst[e];
+      const int e = 3 * gload(c.f_rci + 4 * sl) + (k - 3 * (k / 3));
+      gstore(prod + e, gload(src_row + e) * gload(dst + e));
     }
   }
 }
@@ -940,13 +944,17 @@ __global__ void k_matvec(int np, const int* rowptr, const int* colind, const dou
 // one row of k_spmv_dir: p_new[i] (= z[i] or z[i] + beta p_old[i]), Ap[i]
 // and p_new[i] .* Ap[i]
 __device__ __forceinline__ void spmv_dir_row(const DevCloth& c, int i, bool first, double beta,
-                                             const double* __restrict__ pold,
-                                             double* __restrict__ pnew) {
-  const double* z = c.z;
+                                             const double* __restrict__ pold_,
+                                             double* __restrict__ pnew_) {
+  const auto z = gptr(c.z);
+  const auto pold = gptr(pold_);
+  const auto pnew = gptr(pnew_);
+  const auto rowptr = gptr(c.rowptr);
+  const auto colind = gptr(c.colind);
   double y0 = z[3 * i] * 0.0, y1 = z[3 * i + 1] * 0.0, y2 = z[3 * i + 2] * 0.0;
-  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; k++) {
-    const int j = c.colind[k];
-    const double* a = c.crs + (size_t)k * 9;
+  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+    const int j = colind[k];
+    const auto a = gptr(c.crs) + (size_t)k * 9;
     double x0, x1, x2;
     if (first) {
       x0 = z[3 * j];
@@ -971,19 +979,21 @@ __device__ __forceinline__ void spmv_dir_row(const DevCloth& c, int i, bool firs
     x1 = z[3 * i + 1] + beta * pold[3 * i + 1];
     x2 = z[3 * i + 2] + beta * pold[3 * i + 2];
   }
-  const double* a = c.dia + (size_t)i * 9;
+  const auto a = gptr(c.dia) + (size_t)i * 9;
   y0 += 1.0 * (a[0] * x0 + a[1] * x1 + a[2] * x2);
   y1 += 1.0 * (a[3] * x0 + a[4] * x1 + a[5] * x2);
   y2 += 1.0 * (a[6] * x0 + a[7] * x1 + a[8] * x2);
-  c.ap[3 * i] = y0;
-  c.ap[3 * i + 1] = y1;
-  c.ap[3 * i + 2] = y2;
+  const auto ap = gptr(c.ap);
+  ap[3 * i] = y0;
+  ap[3 * i + 1] = y1;
+  ap[3 * i + 2] = y2;
   pnew[3 * i] = x0;
   pnew[3 * i + 1] = x1;
   pnew[3 * i + 2] = x2;
-  c.prod[3 * i] = x0 * y0;
-  c.prod[3 * i + 1] = x1 * y1;
-  c.prod[3 * i + 2] = x2 * y2;
+  const auto prod = gptr(c.prod);
+  prod[3 * i] = x0 * y0;
+  prod[3 * i + 1] = x1 * y1;
+  prod[3 * i + 2] = x2 * y2;
 }
 
 template <class Sel>
@@ -1002,13 +1012,15 @@ __global__ void k_spmv_dir(Sel sel, int it) {
 // the next forward sweep's right-hand side
 __device__ __forceinline__ void update_rx_elem(const DevCloth& c, int e, double alpha,
                                                const double* __restrict__ p) {
-  const double r = c.r[e] + -alpha * c.ap[e];
-  c.r[e] = r;
-  c.dx[e] = c.dx[e] + alpha * p[e];
-  c.prod2[e] = r * r;
+  const auto rr = gptr(c.r);
+  const double r = rr[e] + -alpha * gptr(c.ap)[e];
+  rr[e] = r;
+  const auto dx = gptr(c.dx);
+  dx[e] = dx[e] + alpha * gptr(p)[e];
+  gptr(c.prod2)[e] = r * r;
   if (c.fslot) {
     const int i = e / 3;
-    c.f_dinv[fwd_rhs_index(c.np, c.fslot[i], e - 3 * i)] = r;  // the next sweep's input
+    gptr(c.f_dinv)[fwd_rhs_index(c.np, gptr(c.fslot)[i], e - 3 * i)] = r;  // the next sweep's input
   }
 }
 
@@ -1176,7 +1188,9 @@ template <int K>
 __device__ __forceinline__ void wg_precondition(const DevCloth& c, double (*ring)[kRingLen],
                                                 int* cuts) {
   for (int comp = 0; comp < c.ncomp; comp++) quad_component<K>(c, comp, c.z, ring, cuts);
-  for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) c.prod[e] = c.r[e] * c.z[e];
+  const auto r = gptr(c.r), z = gptr(c.z);
+  const auto prod = gptr(c.prod);
+  for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x) prod[e] = r[e] * z[e];
   __syncthreads();
 }
 
@@ -1190,12 +1204,12 @@ __global__ __launch_bounds__(kWgThreads) void k_pcg_wg(Sel sel, double tol2, int
   if (tid < 3) ring[tid][kRingZero] = 0.0;  // never written again
   // r = b, dx = 0, b .* b, r as the forward sweep's right-hand side (k_pcg_init)
   for (int e = tid; e < n3; e += nth) {
-    const double b = c.b[e];
-    c.r[e] = b;
-    c.dx[e] = 0;
-    c.prod[e] = b * b;
+    const double b = gptr(c.b)[e];
+    gptr(c.r)[e] = b;
+    gptr(c.dx)[e] = 0;
+    gptr(c.prod)[e] = b * b;
     const int i = e / 3;
-    c.f_dinv[fwd_rhs_index(c.np, c.fslot[i], e - 3 * i)] = b;
+    gptr(c.f_dinv)[fwd_rhs_index(c.np, gptr(c.fslot)[i], e - 3 * i)] = b;
   }
   __syncthreads();
   double conv = 0.0, inv_rr0 = 0.0, rz = 0.0, beta = 0.0;
@@ -1231,7 +1245,7 @@ __global__ __launch_bounds__(kWgThreads) void k_pcg_wg(Sel sel, double tol2, int
   }
   if (!done) {
     // cap reached: the achieved ratio (ilu_sparse.cpp:706-707)
-    for (int e = tid; e < n3; e += nth) c.prod[e] = c.r[e] * c.r[e];
+    for (int e = tid; e < n3; e += nth) gptr(c.prod)[e] = gptr(c.r)[e] * gptr(c.r)[e];
     __syncthreads();
     conv = sqrt(wg_ordered_sum(c.prod, n3, ws) * inv_rr0);
     iters = maxit;
