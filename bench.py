@@ -164,13 +164,13 @@ def _cpu_worker(cpu, kind, jobs, out):
             t += cl.dt
         t_end = time.time()
         o.close()
-        out.put((key, cl.np_, times, x1, t_begin, t_end))
+        out.put((key, cl.np_, times, x1, t_begin, t_end, x))
 
 
 def cpu_baseline(jobs: list, processes: int, what: str) -> tuple:
     """Run `jobs` [(key, cloth, steps)] on `processes` pinned single-core
     processes (largest cloth first); returns (baseline dict, {key: x after
-    the first step})."""
+    the first step}, {key: x after the last step})."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     kind = "reference" if pyoracle.available("reference") else "port"
@@ -203,7 +203,7 @@ def cpu_baseline(jobs: list, processes: int, what: str) -> tuple:
             "sample": what, "wall_s": round(wall, 2),
             "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
             "steps_timed": int(len(lat))}
-    return base, {r[0]: r[3] for r in res}
+    return base, {r[0]: r[3] for r in res}, {r[0]: r[6] for r in res}
 
 
 # ---------------------------------------------------------------------------
@@ -225,6 +225,8 @@ def serve_leg(args, copy_gbs) -> dict:
         kw["lanes"] = tuple(int(v) for v in args.lanes.split(","))
     if args.lane_pcg:
         kw["lane_pcg"] = tuple(int(v) for v in args.lane_pcg.split(","))
+    if args.autotune:
+        kw["autotune"] = True
     # warm-up: the three smallest requests, served and plain, W steps each
     small = sorted(reqs, key=lambda r: r.cloth.np_)[:3]
     warm = [serve.Request(f"warm{k}", r.cloth, steps=max(1, args.warmup)) for k, r in enumerate(small)]
@@ -331,24 +333,32 @@ def batch_leg(count: int, steps: int, copy_gbs, maxit: int) -> dict:
 
 
 def drape_leg(steps: int, maxit: int) -> dict:
-    """The 32x32 drape over a sphere (1,024 vertices), plain step."""
+    """The 32x32 drape over a sphere (1,024 vertices), plain step, from its
+    initial state: FP64 and mixed precision (FP32 sweep coefficients); the
+    final positions are kept for the comparison with the reference."""
     from cloth_sim_self_collision_amd.sim import DeviceCloth
     cl = scene.drape_over_sphere(32)
     cl.pcg_maxit = maxit
-    with DeviceCloth(cl) as d:
-        d.step(2)
-        lat = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            t1 = time.perf_counter()
-            d.step(1)
-            lat.append(time.perf_counter() - t1)
-        wall = time.perf_counter() - t0
-    lat_ms = np.array(lat) * 1e3
-    return {"workload": "drape32", "np": cl.np_, "steps": steps, "ms_per_step": wall / steps * 1e3,
-            "vertex_steps_per_s": cl.np_ * steps / wall,
-            "p50_step_ms": float(np.percentile(lat_ms, 50)),
-            "p99_step_ms": float(np.percentile(lat_ms, 99))}, cl
+    out = {"workload": "drape32", "np": cl.np_, "steps": steps}
+    for leg, prec in (("f64", 0), ("mixed", 1)):
+        with DeviceCloth(cl) as d:
+            d.set_option(DeviceCloth.OPT_PRECISION, prec)
+            lat = []
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                t1 = time.perf_counter()
+                d.step(1)
+                lat.append(time.perf_counter() - t1)
+            wall = time.perf_counter() - t0
+            x, _ = d.get_state()
+        lat_ms = np.array(lat) * 1e3
+        out[leg] = {"ms_per_step": wall / steps * 1e3, "vertex_steps_per_s": cl.np_ * steps / wall,
+                    "p50_step_ms": float(np.percentile(lat_ms, 50)),
+                    "p99_step_ms": float(np.percentile(lat_ms, 99)), "x": x}
+    # the workload's own figures are the FP64 ones
+    out.update({k: out["f64"][k] for k in ("ms_per_step", "vertex_steps_per_s", "p50_step_ms",
+                                            "p99_step_ms")})
+    return out, cl
 
 
 def main() -> int:
@@ -363,6 +373,8 @@ def main() -> int:
     ap.add_argument("--lanes", default="", help="lane vertex-count bounds, e.g. 2500,21846 "
                                                 "(default: the server's)")
     ap.add_argument("--lane-pcg", default="", help="PCG variant per lane, e.g. 2,2,0")
+    ap.add_argument("--autotune", action="store_true",
+                    help="lanes pick their PCG variant from measured ticks")
     ap.add_argument("--solo-from", type=int, default=-1,
                     help="vertex count from which a request runs alone (-1: the server's default, "
                          "0: never)")
@@ -418,7 +430,7 @@ def main() -> int:
         t0 = time.perf_counter()
         if sv is not None:
             reqs, plain = sv["reqs"], sv["plain"]
-            cpu, first = cpu_baseline([(r.rid, r.cloth, 1) for r in reqs], args.cpu_procs,
+            cpu, first, _ = cpu_baseline([(r.rid, r.cloth, 1) for r in reqs], args.cpu_procs,
                                       f"the first step of each of the {len(reqs)} requests, one "
                                       "process per core, largest first")
             err = max(float(np.abs(first[rid] - x).max()) for rid, x in plain["first"].items()
@@ -426,19 +438,25 @@ def main() -> int:
             print(f"[bench] cpu served sample {time.perf_counter() - t0:.1f} s", file=sys.stderr,
                   flush=True)
         if batch is not None:
-            batch["cpu_baseline"], _ = cpu_baseline(
+            batch["cpu_baseline"], _, _ = cpu_baseline(
                 [(k, c, 1) for k, c in enumerate(batch_cls)], args.cpu_procs,
                 f"one step of each of the {len(batch_cls)} cloths, one process per core")
         if drape is not None:
-            drape["cpu_baseline"], _ = cpu_baseline(
+            drape["cpu_baseline"], _, last = cpu_baseline(
                 [("drape32", drape_cl, args.drape_steps)], 1,
                 f"{args.drape_steps} steps of drape32 on one core")
+            # position error after all the steps against the reference's own run
+            for leg in ("f64", "mixed"):
+                drape[leg]["pos_err_m"] = float(np.abs(drape[leg].pop("x") - last["drape32"]).max())
         print(f"[bench] cpu baselines done {time.perf_counter() - t0:.1f} s", file=sys.stderr,
               flush=True)
 
     out = headline(args, sv, err) if sv is not None else {
         "metric": METRIC, "value": None, "unit": "vertex-steps/s", "n_gpus": 1,
         "note": "no served stream (--requests 0): a profiling run of the other workloads"}
+    if drape is not None:
+        for leg in ("f64", "mixed"):
+            drape[leg].pop("x", None)
     out.update({"cpu_baseline": cpu, "fold256": fold, "batch64": batch, "drape32": drape})
     # vs_baseline stays null: the reference publishes no number for this metric
     if cpu and out.get("value"):
@@ -467,7 +485,8 @@ def headline(args, sv, err) -> dict:
                    "steps_per_request": [r.steps for r in reqs],
                    "dt": reqs[0].cloth.dt, "pcg_tol": reqs[0].cloth.pcg_tol,
                    "pcg_maxit": reqs[0].cloth.pcg_maxit, "lanes_vertex_bounds": sv["lanes"],
-                   "lane_pcg": list(sv["lane_pcg"]), "solo_from_vertices": sv["solo_from"],
+                   "lane_pcg": list(sv["lane_pcg"]), "lane_autotune": bool(args.autotune),
+                   "solo_from_vertices": sv["solo_from"],
                    "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                    "precision": "f64, bit-exact"},
         "p50_step_ms": served["p50_step_ms"], "p99_step_ms": served["p99_step_ms"],

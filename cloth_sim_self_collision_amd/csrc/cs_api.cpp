@@ -656,16 +656,30 @@ int cs_set_option(cs_cloth* p, int option, int value) {
   if (!p) return kBadArg;
   Cloth* c = H(p);
   if (option == CS_OPT_SWEEP && value >= 0 && value <= 1) {
+    if (value == 1 && c->d.precision != 0) {
+      set_error("the row-by-row sweep is FP64 only: set CS_OPT_PRECISION to 0 first");
+      return kBadArg;
+    }
     c->d.sweep_variant = value;
     return kOk;
   }
   if (option == CS_OPT_PCG && value >= 0 && value <= 2) {
     if (value == 2 && !pcg_wg_ok(c->d)) {
-      set_error("the one-workgroup PCG solve needs the packed sweep and 3 np <= 65536");
+      set_error("the one-workgroup PCG solve needs the packed sweep, FP64 and 3 np <= 65536");
       return kBadArg;
     }
     c->d.pcg_variant = value;
     return kOk;
+  }
+  if (option == CS_OPT_PRECISION && value >= 0 && value <= 1) {
+    if (value == 1 && (!quad_ok(c->d) || c->d.sweep_variant == 1 || c->d.pcg_variant == 2)) {
+      set_error("mixed precision needs the packed sweep and the multi-launch PCG solve");
+      return kBadArg;
+    }
+    c->d.precision = value;
+    // the packed coefficients change format: repack the current factor
+    if (int st = launch_pack_factor(c->d, c->stream)) return st;
+    return sync(c);
   }
   set_error("unknown option or value");
   return kBadArg;
@@ -1135,6 +1149,17 @@ extern "C" int cs_batch_step(cs_batch* p, int nsteps, cs_step_stats* stats) {
   // member cloths may have pending work on their own streams
   for (Cloth* c : b->cl) CS_HIP(hipStreamSynchronize(c->stream));
   if (int st = batch_refresh(b)) return st;
+  // the PCG variant follows the members' current options (results are the
+  // same either way): the one-workgroup solve when every member selects it
+  b->view.wg = b->view.piped;
+  for (Cloth* c : b->cl) b->view.wg = b->view.wg && pcg_wg_ok(c->d) && c->d.pcg_variant == 2;
+  // one sweep instance serves every member: their coefficient formats agree
+  b->view.f32 = b->cl[0]->d.precision != 0;
+  for (Cloth* c : b->cl)
+    if ((c->d.precision != 0) != b->view.f32) {
+      set_error("cs_batch_step: the cloths of a batch must share the precision");
+      return kBadArg;
+    }
   const size_t n = b->cl.size();
   for (int k = 0; k < nsteps; k++) {
     for (Cloth* c : b->cl)

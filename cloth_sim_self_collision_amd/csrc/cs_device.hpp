@@ -152,6 +152,7 @@ struct DevCloth {
   int wmax;           // widest level of either sweep
   int sweep_variant;  // 0 automatic (quad sweep), 1 row-by-row
   int pcg_variant;    // 0 automatic, 1 one launch per PCG operation, 2 one-workgroup solve
+  int precision;      // 0 FP64 (bit-exact), 1 mixed: the packed sweep's coefficients in FP32
   const int* f_rci;  // {row, block count, backward slot, 0} per forward slot
   const int* b_rc;   // {row, block count} per backward slot
   // sweep metadata per slot: kpack 16-bit ring byte offsets of the column

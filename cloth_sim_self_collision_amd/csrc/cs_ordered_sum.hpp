@@ -66,6 +66,10 @@ template <>
 struct GlobalWord<int4> {
   typedef int __attribute__((ext_vector_type(4))) type;
 };
+template <>
+struct GlobalWord<float4> {
+  typedef float __attribute__((ext_vector_type(4))) type;
+};
 template <class T>
 __device__ __forceinline__ T gload(const T* p) {
   typedef typename GlobalWord<T>::type W;

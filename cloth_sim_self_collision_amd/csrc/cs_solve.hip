@@ -512,6 +512,13 @@ __host__ __device__ constexpr int pack_pairs(int K) { return (3 * K + 1) / 2; }
 __device__ __forceinline__ size_t pack_index(int v, int sl, int r, int np) {
   return ((((size_t)(v >> 1) * np + sl) * 3 + r) << 1) + (v & 1);
 }
+// mixed precision: the same values rounded to FP32, four per 16 bytes, in the
+// same buffer (it holds the FP64 pairs otherwise): value v of (slot, r) at
+// float index [(((v / 4) * np + slot) * 3 + r) * 4 + v % 4]
+__host__ __device__ constexpr int pack_quads(int K) { return (3 * K + 3) / 4; }
+__device__ __forceinline__ size_t pack_index32(int v, int sl, int r, int np) {
+  return ((((size_t)(v >> 2) * np + sl) * 3 + r) << 2) + (v & 3);
+}
 
 template <class Sel>
 __global__ void k_pack_factor(Sel sel) {
@@ -522,11 +529,19 @@ __global__ void k_pack_factor(Sel sel) {
   {
     const int i = c.f_rci[4 * sl], cnt = c.f_rci[4 * sl + 1];
     const double* src = c.fcrs + (size_t)c.rowptr[i] * 9;
-    for (int v = 0; v < 2 * pack_pairs(c.kpack); v++)
-      for (int r = 0; r < 3; r++) {
-        const int k = v / 3, j = v % 3;
-        c.f_blk[pack_index(v, sl, r, np)] = (k < cnt && k < c.kpack) ? src[k * 9 + r * 3 + j] : 0.0;
-      }
+    if (c.precision)
+      for (int v = 0; v < 4 * pack_quads(c.kpack); v++)
+        for (int r = 0; r < 3; r++) {
+          const int k = v / 3, j = v % 3;
+          reinterpret_cast<float*>(c.f_blk)[pack_index32(v, sl, r, np)] =
+              (k < cnt && k < c.kpack) ? (float)src[k * 9 + r * 3 + j] : 0.0f;
+        }
+    else
+      for (int v = 0; v < 2 * pack_pairs(c.kpack); v++)
+        for (int r = 0; r < 3; r++) {
+          const int k = v / 3, j = v % 3;
+          c.f_blk[pack_index(v, sl, r, np)] = (k < cnt && k < c.kpack) ? src[k * 9 + r * 3 + j] : 0.0;
+        }
     const double* d = c.fdia + (size_t)i * 9;
     for (int v = 0; v < 4; v++)
       for (int r = 0; r < 3; r++) c.f_dinv[pack_index(v, sl, r, np)] = v < 3 ? d[r * 3 + v] : 0.0;
@@ -534,11 +549,19 @@ __global__ void k_pack_factor(Sel sel) {
   {
     const int i = c.b_rc[2 * sl], cnt = c.b_rc[2 * sl + 1];
     const double* src = c.fcrs + (size_t)c.diaind[i] * 9;
-    for (int v = 0; v < 2 * pack_pairs(c.kpack); v++)
-      for (int r = 0; r < 3; r++) {
-        const int k = v / 3, j = v % 3;
-        c.b_blk[pack_index(v, sl, r, np)] = (k < cnt && k < c.kpack) ? src[k * 9 + r * 3 + j] : 0.0;
-      }
+    if (c.precision)
+      for (int v = 0; v < 4 * pack_quads(c.kpack); v++)
+        for (int r = 0; r < 3; r++) {
+          const int k = v / 3, j = v % 3;
+          reinterpret_cast<float*>(c.b_blk)[pack_index32(v, sl, r, np)] =
+              (k < cnt && k < c.kpack) ? (float)src[k * 9 + r * 3 + j] : 0.0f;
+        }
+    else
+      for (int v = 0; v < 2 * pack_pairs(c.kpack); v++)
+        for (int r = 0; r < 3; r++) {
+          const int k = v / 3, j = v % 3;
+          c.b_blk[pack_index(v, sl, r, np)] = (k < cnt && k < c.kpack) ? src[k * 9 + r * 3 + j] : 0.0;
+        }
   }
 }
 
@@ -593,18 +616,20 @@ struct QuadRow {
 // Bases of one sweep's operands, formed once per launch: every array is
 // addressed as base[unsigned 32-bit index], which the loads take as a
 // scalar base plus a per-lane offset (no 64-bit address arithmetic per load).
-template <bool kFwd, int K>
+// F32: the coefficients as FP32 quads (mixed precision), else FP64 pairs
+template <bool kFwd, int K, bool F32>
 struct QuadSrc {
   const int4* meta[meta_groups(K)];  // sweep metadata, 16-byte groups per slot
   const double* rhs;
   const double2* blk[pack_pairs(K)];
+  const float4* blk4[pack_quads(K)];
   const double2* d0;  // D^-1 values 0-1 and 2-3 (forward)
   const double2* d1;
 };
 
-template <bool kFwd, int K>
-__device__ __forceinline__ QuadSrc<kFwd, K> quad_src(const DevCloth& c, const double* rhs) {
-  QuadSrc<kFwd, K> q;
+template <bool kFwd, int K, bool F32>
+__device__ __forceinline__ QuadSrc<kFwd, K, F32> quad_src(const DevCloth& c, const double* rhs) {
+  QuadSrc<kFwd, K, F32> q;
   const size_t np = (size_t)c.np;
   const int4* meta = reinterpret_cast<const int4*>(kFwd ? c.f_meta : c.b_meta);
 #pragma unroll
@@ -613,6 +638,9 @@ __device__ __forceinline__ QuadSrc<kFwd, K> quad_src(const DevCloth& c, const do
   const double2* blk = reinterpret_cast<const double2*>(kFwd ? c.f_blk : c.b_blk);
 #pragma unroll
   for (int p = 0; p < pack_pairs(K); p++) q.blk[p] = blk + p * np * 3;
+  const float4* blk4 = reinterpret_cast<const float4*>(kFwd ? c.f_blk : c.b_blk);
+#pragma unroll
+  for (int p = 0; p < pack_quads(K); p++) q.blk4[p] = blk4 + p * np * 3;
   q.d0 = reinterpret_cast<const double2*>(c.f_dinv);
   q.d1 = q.d0 + np * 3;
   return q;
@@ -648,8 +676,8 @@ struct CutsGlobal {
 // and is only marked (i = -1), lane 3 loads lane 2's right-hand side.  Loads
 // on one path only (even a wave-uniform skip) make the compiler wait for every
 // outstanding load where the paths merge, defeating the prefetch.
-template <bool kFwd, int K, class Cuts>
-__device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const Cuts& cuts, int q1,
+template <bool kFwd, int K, bool F32, class Cuts>
+__device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K, F32>& src, const Cuts& cuts, int q1,
                                           int l, QuadRow& q) {
   const int lane = threadIdx.x & 3, qi = threadIdx.x >> 2;
   const int lc = l < q1 ? l : cuts.any();  // past the last sub-level: marked empty
@@ -678,11 +706,22 @@ __device__ __forceinline__ void quad_load(const QuadSrc<kFwd, K>& src, const Cut
   const unsigned lr = lane < 3 ? lane : 2;  // lane 3 shadows lane 2 (same cache line)
   const unsigned e = 3 * sl + lr;
   if (!kFwd) q.t = ld32(src.rhs, e);  // forward: with the last D^-1 entry below
+  if constexpr (F32) {
 #pragma unroll
-  for (int p = 0; p < pack_pairs(K); p++) {
-    const double2 w = ld32(src.blk[p], e);
-    q.a[2 * p] = w.x;
-    if (2 * p + 1 < 3 * K) q.a[2 * p + 1] = w.y;
+    for (int p = 0; p < pack_quads(K); p++) {
+      const float4 w = ld32(src.blk4[p], e);
+      q.a[4 * p] = (double)w.x;
+      if (4 * p + 1 < 3 * K) q.a[4 * p + 1] = (double)w.y;
+      if (4 * p + 2 < 3 * K) q.a[4 * p + 2] = (double)w.z;
+      if (4 * p + 3 < 3 * K) q.a[4 * p + 3] = (double)w.w;
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < pack_pairs(K); p++) {
+      const double2 w = ld32(src.blk[p], e);
+      q.a[2 * p] = w.x;
+      if (2 * p + 1 < 3 * K) q.a[2 * p + 1] = w.y;
+    }
   }
   if (kFwd) {
     const double2 w0 = ld32(src.d0, e);
@@ -753,15 +792,15 @@ __device__ __forceinline__ bool wave_idle3(const Cuts& cuts, int q1, int s) {
 // one sweep over the sub-levels; three register sets rotate so that the
 // loads of sub-levels l + 1 and l + 2 are in flight while l is computed
 // over the sub-levels q0 .. q1 - 1 (one component's)
-template <bool kFwd, int K, class Cuts>
+template <bool kFwd, int K, bool F32, class Cuts>
 __device__ __forceinline__ void quad_sweep(const DevCloth& c, const Cuts& cuts, int q0, int q1,
                                            const double* rhs, double* out,
                                            double (*ring)[kRingLen]) {
-  const QuadSrc<kFwd, K> src = quad_src<kFwd, K>(c, rhs);
+  const QuadSrc<kFwd, K, F32> src = quad_src<kFwd, K, F32>(c, rhs);
   QuadRow qa, qb, qc;
-  quad_load<kFwd, K>(src, cuts, q1, q0, qa);
-  quad_load<kFwd, K>(src, cuts, q1, q0 + 1, qb);
-  quad_load<kFwd, K>(src, cuts, q1, q0 + 2, qc);
+  quad_load<kFwd, K, F32>(src, cuts, q1, q0, qa);
+  quad_load<kFwd, K, F32>(src, cuts, q1, q0 + 1, qb);
+  quad_load<kFwd, K, F32>(src, cuts, q1, q0 + 2, qc);
   // all three thirds unconditional (a count that is not a multiple of three
   // ends with empty sub-levels): a skipped third would change which loads are
   // the newest on one path, and the compiler would then wait for all loads at
@@ -785,13 +824,13 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const Cuts& cuts, 
       }
     }
     quad_compute<kFwd, K>(ring, out, qa);
-    quad_load<kFwd, K>(src, cuts, q1, l + 3, qa);
+    quad_load<kFwd, K, F32>(src, cuts, q1, l + 3, qa);
     lds_barrier();
     quad_compute<kFwd, K>(ring, out, qb);
-    quad_load<kFwd, K>(src, cuts, q1, l + 4, qb);
+    quad_load<kFwd, K, F32>(src, cuts, q1, l + 4, qb);
     lds_barrier();
     quad_compute<kFwd, K>(ring, out, qc);
-    quad_load<kFwd, K>(src, cuts, q1, l + 5, qc);
+    quad_load<kFwd, K, F32>(src, cuts, q1, l + 5, qc);
     lds_barrier();
   }
 }
@@ -806,7 +845,7 @@ __device__ __forceinline__ void quad_sweep(const DevCloth& c, const Cuts& cuts, 
 // the forward result into c.sb (backward-slot order), the backward result
 // into dst (row order).  ring: the LDS ring, its zero entry already set;
 // cuts: LDS for the component's sub-level cuts (kCutsLds + 1 entries).
-template <int K>
+template <int K, bool F32 = false>
 __device__ __forceinline__ void quad_component(const DevCloth& c, int comp, double* dst,
                                                double (*ring)[kRingLen], int* cuts) {
   const int f0 = c.qf_comp[comp], f1 = c.qf_comp[comp + 1];
@@ -815,9 +854,9 @@ __device__ __forceinline__ void quad_component(const DevCloth& c, int comp, doub
     for (int i = threadIdx.x; i <= f1 - f0; i += blockDim.x) cuts[i] = gload(c.qf_ptr + f0 + i);
   __syncthreads();
   if (fl)
-    quad_sweep<true, K>(c, CutsLds{cuts, f0}, f0, f1, nullptr, c.sb, ring);
+    quad_sweep<true, K, F32>(c, CutsLds{cuts, f0}, f0, f1, nullptr, c.sb, ring);
   else
-    quad_sweep<true, K>(c, CutsGlobal{c.qf_ptr}, f0, f1, nullptr, c.sb, ring);
+    quad_sweep<true, K, F32>(c, CutsGlobal{c.qf_ptr}, f0, f1, nullptr, c.sb, ring);
   __syncthreads();  // the backward sweep reads the component's complete forward result
   const int b0 = c.qb_comp[comp], b1 = c.qb_comp[comp + 1];
   const bool bl = b1 - b0 <= kCutsLds;
@@ -825,13 +864,13 @@ __device__ __forceinline__ void quad_component(const DevCloth& c, int comp, doub
     for (int i = threadIdx.x; i <= b1 - b0; i += blockDim.x) cuts[i] = gload(c.qb_ptr + b0 + i);
   __syncthreads();
   if (bl)
-    quad_sweep<false, K>(c, CutsLds{cuts, b0}, b0, b1, c.sb, dst, ring);
+    quad_sweep<false, K, F32>(c, CutsLds{cuts, b0}, b0, b1, c.sb, dst, ring);
   else
-    quad_sweep<false, K>(c, CutsGlobal{c.qb_ptr}, b0, b1, c.sb, dst, ring);
+    quad_sweep<false, K, F32>(c, CutsGlobal{c.qb_ptr}, b0, b1, c.sb, dst, ring);
   __syncthreads();
 }
 
-template <int K, class Sel>
+template <int K, bool F32, class Sel>
 __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const double* src_row,
                                                                  double* dst, double* prod) {
   const DevCloth c = sel.cloth();
@@ -845,7 +884,7 @@ __global__ __launch_bounds__(kQuadThreads) void k_ilu_apply_quad(Sel sel, const 
   __shared__ double ring[3][kRingLen];  // + the dummy and the zero entry
   __shared__ int cuts[kCutsLds + 1];
   if (threadIdx.x < 3) ring[threadIdx.x][kRingZero] = 0.0;  // never written again
-  quad_component<K>(c, comp, dst, ring, cuts);
+  quad_component<K, F32>(c, comp, dst, ring, cuts);
   if (!prod) return;
   if (c.ncomp == 1) {
     for (int e = threadIdx.x; e < 3 * c.np; e += blockDim.x)
@@ -867,20 +906,30 @@ inline int quad_threads(int qwidth) {
   return 4 * ((rows + 15) / 16 * 16);
 }
 
+template <bool F32, class Sel>
+static void launch_quad_p(int kq, dim3 grid, dim3 tb, const Sel& sel, const double* src,
+                          double* dst, double* prod, hipStream_t s) {
+  switch (kq) {
+    case 1: hipLaunchKernelGGL((k_ilu_apply_quad<1, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 2: hipLaunchKernelGGL((k_ilu_apply_quad<2, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 3: hipLaunchKernelGGL((k_ilu_apply_quad<3, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 4: hipLaunchKernelGGL((k_ilu_apply_quad<4, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 5: hipLaunchKernelGGL((k_ilu_apply_quad<5, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 6: hipLaunchKernelGGL((k_ilu_apply_quad<6, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    case 7: hipLaunchKernelGGL((k_ilu_apply_quad<7, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+    default: hipLaunchKernelGGL((k_ilu_apply_quad<8, F32, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
+  }
+}
+
+// f32: the cloths' packed coefficients are FP32 (mixed precision)
 template <class Sel>
-static void launch_quad(int kq, int qwidth, dim3 grid, const Sel& sel, const double* src,
+static void launch_quad(int kq, bool f32, int qwidth, dim3 grid, const Sel& sel, const double* src,
                         double* dst, double* prod, hipStream_t s) {
   const dim3 tb(quad_threads(qwidth));
-  switch (kq) {
-    case 1: hipLaunchKernelGGL((k_ilu_apply_quad<1, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-    case 2: hipLaunchKernelGGL((k_ilu_apply_quad<2, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-    case 3: hipLaunchKernelGGL((k_ilu_apply_quad<3, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-    case 4: hipLaunchKernelGGL((k_ilu_apply_quad<4, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-    case 5: hipLaunchKernelGGL((k_ilu_apply_quad<5, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-    case 6: hipLaunchKernelGGL((k_ilu_apply_quad<6, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-    case 7: hipLaunchKernelGGL((k_ilu_apply_quad<7, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-    default: hipLaunchKernelGGL((k_ilu_apply_quad<8, Sel>), grid, tb, 0, s, sel, src, dst, prod); break;
-  }
+  if (f32)
+    launch_quad_p<true>(kq, grid, tb, sel, src, dst, prod, s);
+  else
+    launch_quad_p<false>(kq, grid, tb, sel, src, dst, prod, s);
 }
 
 bool sweep_shape(const DevCloth& c, int* waves, int* loads_fwd, int* loads_bwd) {
@@ -889,8 +938,9 @@ bool sweep_shape(const DevCloth& c, int* waves, int* loads_fwd, int* loads_bwd) 
   *waves = quad_threads(c.qwidth) / 64;
   // per lane: the metadata groups and coefficient pairs, then two D^-1 pairs
   // (forward, the right-hand side in the second) or the right-hand side
-  *loads_fwd = meta_groups(K) + pack_pairs(K) + 2;
-  *loads_bwd = meta_groups(K) + pack_pairs(K) + 1;
+  const int blk = c.precision ? pack_quads(K) : pack_pairs(K);
+  *loads_fwd = meta_groups(K) + blk + 2;
+  *loads_bwd = meta_groups(K) + blk + 1;
   return true;
 }
 
@@ -904,7 +954,7 @@ static void launch_sweep(const DevCloth& c, const double* src, double* dst, doub
   if (quad_ok(c) && c.sweep_variant != 1) {
     if (!rhs_ready)
       hipLaunchKernelGGL(k_gather_fslots<One>, dim3(grid_for(3L * c.np)), dim3(kTB), 0, s, one, src);
-    launch_quad(c.kpack, c.qwidth, dim3(c.ncomp), one, src, dst, prod, s);
+    launch_quad(c.kpack, c.precision != 0, c.qwidth, dim3(c.ncomp), one, src, dst, prod, s);
   } else {
     hipLaunchKernelGGL(k_ilu_apply<One>, dim3(1), dim3(1024), 0, s, one, src, dst, prod);
   }
@@ -1267,7 +1317,7 @@ __global__ __launch_bounds__(kWgThreads) void k_pcg_wg(Sel sel, double tol2, int
 // the one-workgroup solve applies: the packed sweep with its slot-order
 // right-hand side, and reductions of at most kWgMaxTiles tiles
 bool pcg_wg_ok(const DevCloth& c) {
-  return quad_ok(c) && c.sweep_variant != 1 && c.fslot != nullptr &&
+  return quad_ok(c) && c.sweep_variant != 1 && c.fslot != nullptr && c.precision == 0 &&
          sum_tiles(3 * c.np) <= kWgMaxTiles;
 }
 
@@ -1510,7 +1560,7 @@ static void launch_sweep_batch(const BatchView& b, hipStream_t s) {
   const Many m{b.tab, b.ptab};
   if (b.piped) {
     // every cloth packed: r is already in slot order (k_pcg_init / k_update_rx)
-    launch_quad(b.kpack, b.max_qwidth, dim3(b.max_ncomp, b.n), m, nullptr, nullptr, nullptr, s);
+    launch_quad(b.kpack, b.f32, b.max_qwidth, dim3(b.max_ncomp, b.n), m, nullptr, nullptr, nullptr, s);
   } else {
     hipLaunchKernelGGL(k_ilu_apply<Many>, dim3(1, b.n), dim3(1024), 0, s, m, nullptr, nullptr, nullptr);
   }

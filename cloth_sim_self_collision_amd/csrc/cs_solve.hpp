@@ -55,6 +55,7 @@ struct BatchView {
   bool piped;  // every cloth can use the packed (quad) sweep ...
   int kpack;   // ... with this many packed blocks per row
   bool wg;     // every cloth takes the one-workgroup PCG solve (one workgroup per cloth)
+  bool f32;    // every cloth's packed sweep coefficients are FP32 (mixed precision)
 };
 // StepTime_InternalDynamicsILU for every cloth of the batch (start positions
 // saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); dgather/hgather hold n

@@ -38,7 +38,9 @@ Wire format (JSON lines, `python -m cloth_sim_self_collision_amd.serve`):
   request  {"id": any JSON value, "scene": a scene configuration
             (scene.from_config; one cloth), "steps": int >= 1,
             "arrival": seconds after the start (optional, default 0),
-            "precision": "f64" (optional; the only precision served),
+            "precision": "f64" (default, bit-identical to the reference)
+                         or "mixed" (FP32 preconditioner coefficients; served
+                         only while every step's ILU(0)-PCG converges),
             "return": subset of ["x", "v"] (optional, default ["x"])}
   result   {"id", "status": "ok", "vertices", "triangles", "steps",
             "arrival", "queued_s", "admitted_s", "finished_s",
@@ -47,7 +49,8 @@ Wire format (JSON lines, `python -m cloth_sim_self_collision_amd.serve`):
             "x": {"dtype": "<f8", "shape": [np, 3], "data": base64}, "v": ...}
            or {"id", "status": "error", "error": message} for a request
            that cannot be served (bad record, unknown scene, precision, too
-           large for the budget, a failed or non-finite step, cancelled).
+           large for the budget, a failed or non-finite step, a mixed-precision
+           step whose solve did not converge, cancelled).
 
 Results are written as requests finish, one line each, in completion order.
 """
@@ -91,6 +94,19 @@ DEFAULT_SOLO_FROM: Optional[int] = None
 # cloths created at once on admission (host setup and upload)
 CREATORS = 4
 _NONFINITE = "the solve produced an infinite or NaN displacement (the ILU(0)-PCG iterate overflowed)"
+_OPT_PRECISION = 3
+
+
+def _step_error(req: "Request", st: dict) -> Optional[str]:
+    """Why a step's result cannot be served, or None."""
+    if st["status"] != 0:
+        return _NONFINITE
+    if (req.precision == "mixed" and st["pcg_iters"] >= req.cloth.pcg_maxit and
+            not st["pcg_conv"] < req.cloth.pcg_tol):
+        return ("mixed precision is served only where ILU(0)-PCG converges: this step stopped "
+                f"at the cap of {req.cloth.pcg_maxit} iterations (residual ratio "
+                f"{st['pcg_conv']:.3g}); request precision f64")
+    return None
 
 
 @dataclasses.dataclass
@@ -100,6 +116,7 @@ class Request:
     steps: int
     arrival: float = 0.0  # seconds after the start of the run
     returns: tuple = ("x",)  # arrays of the final state in the wire result
+    precision: str = "f64"  # or "mixed"
 
 
 @dataclasses.dataclass
@@ -145,7 +162,8 @@ class _Lane:
     """One host thread stepping a set of resident cloths together, or (solo)
     one cloth alone with the plain step until it leaves."""
 
-    def __init__(self, server: "Server", index: int, solo: Optional[_Active] = None):
+    def __init__(self, server: "Server", index: int, solo: Optional[_Active] = None,
+                 tune: bool = False):
         self.server = server
         self.index = index
         self.inbox: List[_Active] = []
@@ -154,6 +172,7 @@ class _Lane:
         self.members: tuple = ()
         self.ticks = 0
         self.solo = solo
+        self.tuner = _VariantTuner() if tune and solo is None else None
         target = self._run if solo is None else self._run_solo
         self.thread = threading.Thread(target=target, name=f"lane{index}", daemon=True)
 
@@ -190,6 +209,7 @@ class _Lane:
                     self.members = key
                     if srv.profile:
                         self.batch.profile(True)
+                variant = self._choose_variant()
                 ts = srv.clock()
                 try:
                     stats = self.batch.step(1)
@@ -203,6 +223,8 @@ class _Lane:
                     continue
                 dt_ms = (srv.clock() - ts) * 1e3
                 self.ticks += 1
+                if variant is not None:
+                    self.tuner.record(variant, dt_ms / sum(a.dev.np_ for a in self.active))
                 if srv.profile:
                     # per-class device time of this tick's shared launches, and
                     # what each member did in it
@@ -215,8 +237,9 @@ class _Lane:
                         srv.tick_log.append(rec)
                 leaving = []
                 for a, st in zip(list(self.active), stats):
-                    if st["status"] != 0:
-                        leaving.append((a, _NONFINITE, False))
+                    err = _step_error(a.req, st)
+                    if err is not None:
+                        leaving.append((a, err, False))
                         continue
                     self._record(a, st, dt_ms)
                     if a.done >= a.req.steps:
@@ -232,6 +255,22 @@ class _Lane:
                 srv.cv.notify_all()
         finally:
             self._close_batch()
+
+    def _choose_variant(self) -> Optional[int]:
+        """With autotuning, the PCG variant of this tick for every member
+        (None: not tuned, the members keep the lane's variant).  Only lanes
+        whose members all allow the one-workgroup solve are tuned."""
+        if self.tuner is None:
+            return None
+        v = self.tuner.pick()
+        try:
+            for a in self.active:
+                a.dev.set_option(_OPT_PCG, v)
+        except _lib.ClothSimError:
+            for a in self.active:  # a member too large for one workgroup
+                a.dev.set_option(_OPT_PCG, 0)
+            return None
+        return v
 
     def _record(self, a: _Active, st: dict, dt_ms: float) -> None:
         a.done += 1
@@ -267,8 +306,9 @@ class _Lane:
                     a.dev.profile(True)
                     with srv.cv:
                         srv.tick_log.append(rec)
-                if st["status"] != 0:
-                    self._leave(a, _NONFINITE)
+                err = _step_error(a.req, st)
+                if err is not None:
+                    self._leave(a, err)
                     return
                 self._record(a, st, dt_ms)
                 if a.done >= a.req.steps:
@@ -310,6 +350,31 @@ class _Lane:
             srv.cv.notify_all()
 
 
+class _VariantTuner:
+    """Picks a lane's PCG variant from measured ticks: the one-workgroup solve
+    (2) or one launch per PCG operation (0), by the lower recent tick time per
+    member vertex.  Each variant is tried on the first ticks, and the one not
+    in use again every `explore` ticks, so the choice follows the lane's
+    membership.  Results are the same with either variant."""
+
+    def __init__(self, explore: int = 8, keep: int = 4):
+        self.samples = {0: [], 2: []}
+        self.explore = explore
+        self.keep = keep
+        self.n = 0
+
+    def pick(self) -> int:
+        self.n += 1
+        for v in (2, 0):
+            if not self.samples[v]:
+                return v
+        best = min(self.samples, key=lambda v: sum(self.samples[v]) / len(self.samples[v]))
+        return (2 if best == 0 else 0) if self.n % self.explore == 0 else best
+
+    def record(self, v: int, ms_per_vertex: float) -> None:
+        self.samples[v] = (self.samples[v] + [ms_per_vertex])[-self.keep:]
+
+
 class Server:
     """Continuous batching over resident simulations in concurrent lanes.
 
@@ -322,6 +387,8 @@ class Server:
     DEFAULT_LANES, else automatic everywhere;
     profile: time each tick's shared launches by kernel class (HIP events on
     the lane's stream) and keep a record per tick in tick_log;
+    autotune: each lane picks its PCG variant from its measured ticks
+    (_VariantTuner) instead of lane_pcg;
     solo_from: requests of at least this many vertices are stepped alone by a
     thread of their own (plain steps on the cloth's stream) instead of
     joining a lane, so a large cloth never paces smaller ones (None: never).
@@ -333,15 +400,18 @@ class Server:
                  on_result: Optional[Callable[["Result"], None]] = None,
                  on_error: Optional[Callable[["Failure"], None]] = None,
                  lanes: tuple = DEFAULT_LANES, lane_pcg: Optional[tuple] = None,
-                 profile: bool = False, solo_from: Optional[int] = DEFAULT_SOLO_FROM):
+                 profile: bool = False, solo_from: Optional[int] = DEFAULT_SOLO_FROM,
+                 autotune: bool = False):
         if lane_pcg is None:
             lane_pcg = DEFAULT_LANE_PCG if tuple(lanes) == DEFAULT_LANES else (0,) * (len(lanes) + 1)
         if len(lane_pcg) != len(lanes) + 1:
             raise ValueError("lane_pcg needs one value per lane")
         self.lane_pcg = tuple(int(v) for v in lane_pcg)
         self.profile = bool(profile)
+        self.autotune = bool(autotune)
         self.solo_from = solo_from
         self.solo_lanes: List[_Lane] = []
+        self.lanes: dict = {}
         self.tick_log: List[dict] = []
         self.budget = int(hbm_budget_bytes)
         self.clock = clock
@@ -417,12 +487,12 @@ class Server:
         self.pending = [_Active(req=r) for r in sorted(requests, key=lambda r: (r.arrival, str(r.rid)))]
         self._by_rid = {a.req.rid: a for a in self.pending}
         self.live = {id(a) for a in self.pending}
-        lanes = [_Lane(self, k) for k in range(len(self.bounds) + 1)]
+        # lanes are created as requests need them, one per (size interval,
+        # precision, PCG tolerance and cap): a batch's cloths share all of these
+        self.lanes = {}
         pool = cf.ThreadPoolExecutor(max_workers=CREATORS, thread_name_prefix="create")
         creating: list = []
         self.t0 = self.clock()
-        for ln in lanes:
-            ln.thread.start()
         try:
             with self.cv:
                 for a in list(self.pending):
@@ -461,11 +531,12 @@ class Server:
                         nxt = [a.req.arrival - now for a in self.pending[:1] if realtime]
                         self.cv.wait(max(0.0, min([0.02] + nxt)))
                     continue
-                creating.append(pool.submit(self._admit, cand, lanes))
+                creating.append(pool.submit(self._admit, cand))
         finally:
             pool.shutdown(wait=True)
             with self.cv:
                 self.stopping = True
+                lanes = list(self.lanes.values())
                 for ln in lanes:
                     for a in ln.active + ln.inbox:
                         a.cancel = True
@@ -485,7 +556,7 @@ class Server:
                 "p50_step_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
                 "p99_step_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0}
 
-    def _admit(self, a: _Active, lanes: List[_Lane]) -> None:
+    def _admit(self, a: _Active) -> None:
         """Create the cloth of an admitted request (outside the lock) and hand
         it to its lane; creation failures become errors or a later retry."""
         try:
@@ -508,7 +579,19 @@ class Server:
                 self.cv.notify_all()
             return
         lane = self._lane_of(a.req.cloth.np_)
-        if self.lane_pcg[lane] != 0:
+        if a.req.precision == "mixed":
+            try:
+                dev.set_option(_OPT_PRECISION, 1)
+            except _lib.ClothSimError as e:
+                dev.close()
+                with self.cv:
+                    self.resident -= a.memory
+                    a.memory = 0
+                    self.live.discard(id(a))
+                    self._fail(a.req.rid, f"mixed precision is not available for this cloth: {e}")
+                    self.cv.notify_all()
+                return
+        elif self.lane_pcg[lane] != 0:
             try:
                 dev.set_option(_OPT_PCG, self.lane_pcg[lane])
             except _lib.ClothSimError:
@@ -521,11 +604,20 @@ class Server:
             if a.admitted_s < 0:
                 a.admitted_s = self.clock() - self.t0
             if self.solo_from is not None and a.req.cloth.np_ >= self.solo_from:
-                ln = _Lane(self, len(lanes) + len(self.solo_lanes), solo=a)
+                ln = _Lane(self, len(self.lanes) + len(self.solo_lanes), solo=a)
                 self.solo_lanes.append(ln)
                 ln.thread.start()
             else:
-                lanes[lane].inbox.append(a)
+                cl = a.req.cloth
+                key = (lane, a.req.precision, cl.pcg_tol, cl.pcg_maxit)
+                ln = self.lanes.get(key)
+                if ln is None:
+                    # the one-workgroup solve that the tuner tries is FP64 only
+                    ln = _Lane(self, len(self.lanes) + len(self.solo_lanes),
+                               tune=self.autotune and a.req.precision == "f64")
+                    self.lanes[key] = ln
+                    ln.thread.start()
+                ln.inbox.append(a)
             self.cv.notify_all()
 
 
@@ -611,7 +703,7 @@ def poisson_stream(count: int, rate_per_s: float, **kw) -> List[Request]:
 # ---------------------------------------------------------------------------
 # wire format
 
-PRECISIONS = ("f64",)
+PRECISIONS = ("f64", "mixed")
 
 
 def encode_array(a: np.ndarray) -> dict:
@@ -650,11 +742,13 @@ def parse_request(obj: dict, index: int = 0) -> Request:
     cloth = scene.from_config(obj["scene"])
     if isinstance(cloth, list):
         raise ValueError("a request carries one cloth (use one request per cloth)")
-    return Request(rid=rid, cloth=cloth, steps=steps, arrival=arrival, returns=returns)
+    return Request(rid=rid, cloth=cloth, steps=steps, arrival=arrival, returns=returns,
+                   precision=prec)
 
 
 def result_record(r: Result, req: Optional[Request] = None) -> dict:
-    out = {"id": r.rid, "status": "ok", "vertices": r.vertices,
+    out = {"id": r.rid, "status": "ok", "precision": req.precision if req is not None else None,
+           "vertices": r.vertices,
            "triangles": req.cloth.nt if req is not None else None, "steps": len(r.step_ms),
            "arrival": req.arrival if req is not None else None,
            "queued_s": r.queued_s, "admitted_s": r.admitted_s, "finished_s": r.finished_s,

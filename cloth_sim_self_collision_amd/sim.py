@@ -126,9 +126,10 @@ class DeviceCloth:
         else:
             check(_lib.load().cs_set_obstacle_values(self.h, dp(f64(pd)), dp(f64(n))))
 
-    # cs_set_option options (results are identical for every value)
+    # cs_set_option options (results are identical for every value of the first two)
     OPT_SWEEP = 1  # 0 automatic, 1 row-by-row CSR sweep
     OPT_PCG = 2    # 0 automatic, 1 one launch per PCG operation, 2 one-workgroup solve
+    OPT_PRECISION = 3  # 0 FP64 (bit-exact), 1 mixed: FP32 sweep coefficients (not bit-exact)
 
     def set_option(self, option: int, value: int):
         check(_lib.load().cs_set_option(self.h, option, value))

@@ -189,14 +189,26 @@ int cs_batch_step(cs_batch* b, int nsteps, cs_step_stats* stats);
  * CS_OPT_PCG:   0 automatic (= 1), 1 one kernel launch per PCG operation
  *               with a host poll every 8 iterations, 2 the whole solve in one
  *               workgroup (one launch; needs the packed sweep and
- *               3 np <= 65536; a batch uses it when all its cloths select it,
+ *               3 np <= 65536; a batch uses it for a step when all its cloths
+ *               select it at that step,
  *               one workgroup per cloth, each iterating to its own
  *               convergence).  Measured: 2 is not faster than 1 for a 32^2
  *               cloth and slower for a 64-cloth batch of 24^2..128^2 cloths
  *               at the iteration cap (each cloth's SpMV and reductions then
  *               run on one CU) */
+/* CS_OPT_PRECISION: 0 FP64, bit-identical to the reference (default); 1
+ *               mixed: the packed ILU sweep reads the factor's coefficients
+ *               rounded to FP32 (D^-1, the vectors, SpMV, the reductions and
+ *               everything else stay FP64).  Not bit-identical: the position
+ *               error against the FP64 reference is small only where
+ *               ILU(0)-PCG converges (a non-converging solve stops at its
+ *               iteration cap on a different iterate).  Needs the packed sweep
+ *               and CS_OPT_PCG != 2; all cloths of a batch share it.  Measured:
+ *               no faster than FP64 (the sweep waits on its instruction
+ *               stream, and the conversions add to it). */
 #define CS_OPT_SWEEP 1
 #define CS_OPT_PCG 2
+#define CS_OPT_PRECISION 3
 int cs_set_option(cs_cloth* c, int option, int value);
 
 /* per-kernel-class timing with HIP events on the cloth's stream:

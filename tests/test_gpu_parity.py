@@ -557,7 +557,8 @@ def test_batch_matches_each_cloth_alone(sim_mod, oracle_mod, pcg):
 def test_served_results_equal_alone(sim_mod, oracle_mod):
     """Continuous batching: requests of different sizes arriving while others
     run end bit-identical to the same simulations stepped alone, with a
-    generous HBM budget (all overlap) and a tight one (requests queue)."""
+    generous HBM budget (all overlap), a tight one (requests queue), and one
+    lane whose PCG variant the autotuner changes from tick to tick."""
     from cloth_sim_self_collision_amd import serve
     sph = scene.sphere((0.0, 0.0, 0.0), 0.3)
     bx = scene.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25))
@@ -577,13 +578,109 @@ def test_served_results_equal_alone(sim_mod, oracle_mod):
         x, v, *_ = o.step(x, v)
     assert np.array_equal(alone[1][0], x) and np.array_equal(alone[1][1], v)
     biggest = max(r.cloth.np_ for r in reqs) * serve.BYTES_PER_VERTEX_ESTIMATE
-    for budget, realtime in ((1 << 36, True), (biggest + 1, False)):
-        out = serve.Server(budget).run(reqs, realtime=realtime)
+    for budget, realtime, tune in ((1 << 36, True, False), (biggest + 1, False, False),
+                                   (1 << 36, False, True)):
+        # the last run: one lane whose PCG variant the tuner switches between ticks
+        kw = {"lanes": (), "autotune": True} if tune else {}
+        out = serve.Server(budget, **kw).run(reqs, realtime=realtime)
         assert [r.rid for r in out["results"]] == [0, 1, 2]
         for res in out["results"]:
             assert len(res.step_ms) == reqs[res.rid].steps
             ax, av = alone[res.rid]
             assert np.array_equal(res.xyz, ax) and np.array_equal(res.uvw, av), (budget, res.rid)
+
+
+@pytest.mark.parametrize("name,steps", [("drape32", 30), ("fold24", 60), ("stack16", 20)])
+def test_mixed_precision_tracks_the_reference(sim_mod, oracle_mod, name, steps):
+    """CS_OPT_PRECISION 1 (the packed ILU sweep reads the factor rounded to
+    FP32; everything else FP64) on scenes whose ILU(0)-PCG converges: over
+    trajectories up to the first contact impulse the positions stay within
+    1e-5 m (the tolerance stated for the mixed mode) of the FP64 reference
+    after every step, and every solve converges.  Past the first impulse the
+    reference's own trajectory is chaotic (its contact decisions are
+    thresholds): a 1e-13 m change of drape32's start grows to 2e-3 m by step
+    100 in the oracle itself, so only a bit-identical step can follow it
+    there; single steps are checked along such a trajectory below."""
+    cl = _scenes()[name]
+    o = oracle_mod.Oracle(cl, "port")
+    x, v, t = cl.xyz.copy(), cl.uvw.copy(), 0.0
+    worst = 0.0
+    with sim_mod.DeviceCloth(cl) as d:
+        d.set_option(sim_mod.DeviceCloth.OPT_PRECISION, 1)
+        for k in range(steps):
+            scene.apply_sweep(cl, x, t + cl.dt)
+            x, v, imp, *_ = o.step(x, v)
+            t += cl.dt
+            assert not imp, f"step {k} applies an impulse: past the comparable range"
+            st = d.step(1)
+            gx, _ = d.get_state()
+            worst = max(worst, float(np.abs(gx - x).max()))
+            assert st["pcg_iters"] < cl.pcg_maxit, (k, st)
+    print(f"{name}: max position error of the mixed mode over {steps} steps {worst:.3e} m")
+    assert worst <= 1e-5, worst
+
+
+def test_mixed_precision_single_steps_along_a_contact_trajectory(sim_mod, oracle_mod):
+    """Along drape32's first 100 steps (impulses from step 36 on), each mixed
+    step taken from the reference's own state ends within 1e-5 m of the
+    reference's step."""
+    cl = _scenes()["drape32"]
+    o = oracle_mod.Oracle(cl, "port")
+    x, v = cl.xyz.copy(), cl.uvw.copy()
+    errs = []
+    with sim_mod.DeviceCloth(cl) as d:
+        d.set_option(sim_mod.DeviceCloth.OPT_PRECISION, 1)
+        for k in range(100):
+            d.set_state(x, v)
+            d.step(1)
+            gx, _ = d.get_state()
+            x, v, *_ = o.step(x, v)
+            errs.append(float(np.abs(gx - x).max()))
+    e = np.array(errs)
+    print(f"drape32 single mixed steps: median {np.median(e):.3e} m, max {e.max():.3e} m")
+    assert e.max() <= 1e-5, (int(e.argmax()), e.max())
+
+
+def test_mixed_batch_and_served_equal_mixed_alone(sim_mod):
+    """Mixed-precision cloths stepped together (one FP32-coefficient sweep
+    launch for all) or served end bit-identical to each stepped alone in
+    mixed precision; a mixed request whose solve stops at the PCG cap gets
+    an error record; a batch may not mix precisions."""
+    from cloth_sim_self_collision_amd import _lib, serve
+    cls = [scene.drape_over_sphere(16), scene.fold_sweep(24, size=0.3), scene.drape_over_sphere(32)]
+    steps = 15
+    alone = []
+    for cl in cls:
+        with sim_mod.DeviceCloth(cl) as d:
+            d.set_option(sim_mod.DeviceCloth.OPT_PRECISION, 1)
+            d.step(steps)
+            alone.append(d.get_state())
+    devs = [sim_mod.DeviceCloth(cl) for cl in cls]
+    try:
+        for d in devs:
+            d.set_option(sim_mod.DeviceCloth.OPT_PRECISION, 1)
+        with sim_mod.DeviceBatch(devs) as b:
+            b.step(steps)
+        for d, (ax, av) in zip(devs, alone):
+            gx, gv = d.get_state()
+            assert np.array_equal(gx, ax) and np.array_equal(gv, av)
+        devs[1].set_option(sim_mod.DeviceCloth.OPT_PRECISION, 0)
+        with sim_mod.DeviceBatch(devs) as b:
+            with pytest.raises(_lib.ClothSimError, match="precision"):
+                b.step(1)
+    finally:
+        for d in devs:
+            d.close()
+    stuck = scene.drop_on_obstacle(40, 0.4, scene.sphere((0.0, 0.0, 0.0), 0.3), seed=4)
+    stuck.pcg_maxit = 5
+    reqs = [serve.Request(k, cl, steps=steps, precision="mixed") for k, cl in enumerate(cls)]
+    reqs.append(serve.Request(3, stuck, steps=2, precision="mixed"))
+    out = serve.Server(1 << 36).run(reqs, realtime=False)
+    fails = {f.rid: f.error for f in out["failures"]}
+    assert set(fails) == {3} and "converges" in fails[3]
+    for res in out["results"]:
+        ax, av = alone[res.rid]
+        assert np.array_equal(res.xyz, ax) and np.array_equal(res.uvw, av), res.rid
 
 
 def test_cloth_from_setup_cache_steps_identically(sim_mod, tmp_path):

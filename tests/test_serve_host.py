@@ -42,6 +42,8 @@ class FakeCloth:
 
     def set_option(self, option, value):
         self.option = (option, value)
+        if option == 3:
+            self.precision = value
 
     nnz = 0
 
@@ -62,6 +64,10 @@ class FakeCloth:
 class FakeBatch:
     def __init__(self, devs):
         assert all(not d.closed for d in devs)
+        # as cs_batch_create / cs_batch_step require
+        if len({(d.cloth.pcg_tol, d.cloth.pcg_maxit, getattr(d, "precision", 0))
+                for d in devs}) > 1:
+            raise _lib.ClothSimError(1, "the cloths must share the PCG settings and precision")
         self.devs = list(devs)
 
     def step(self, n):
@@ -136,6 +142,66 @@ def test_solo_requests_step_alone_beside_the_lanes(fake):
     fails = {f.rid: f.error for f in out["failures"]}
     assert set(fails) == {4, 5} and "NaN" in fails[4] and fails[5] == "cancelled"
     assert len(out["solo"]) == 4  # requests 1, 3, 4, 5 (>= 3600 vertices) ran alone
+
+
+def test_mixed_precision_requests_fail_where_pcg_does_not_converge(fake, monkeypatch):
+    """Mixed requests run in lanes of their own; one whose step stops at the
+    PCG cap gets an error record, the others finish."""
+    orig = FakeBatch.step
+    members = []
+
+    def step(self, n):
+        members.append(tuple(sorted(d.cloth.name for d in self.devs)))
+        out = orig(self, n)
+        for d, st in zip(self.devs, out):
+            st["pcg_conv"] = 0.5 if d.cloth.name == "stuck" else 1e-9
+            st["pcg_iters"] = d.cloth.pcg_maxit if d.cloth.name == "stuck" else 7
+        return out
+
+    monkeypatch.setattr(FakeBatch, "step", step)
+    reqs = [_req(0, 8, 3), _req(1, 8, 3, name="stuck"), _req(2, 9, 4)]
+    reqs[1].precision = "mixed"
+    reqs[2].precision = "mixed"
+    out = serve.Server(1 << 40, lanes=()).run(reqs, realtime=False)
+    fails = {f.rid: f.error for f in out["failures"]}
+    assert set(fails) == {1} and "converges" in fails[1]
+    by = {r.rid: r for r in out["results"]}
+    _check(by[0], reqs[0])
+    _check(by[2], reqs[2])
+    # FP64 and mixed cloths never share a batch
+    assert not any("c0" in m and ("stuck" in m or "c2" in m) for m in members)
+    assert [c for c in FakeCloth.created]  # created through the pool
+
+
+def test_requests_with_other_pcg_settings_get_their_own_lane(fake):
+    reqs = [_req(k, 8 + k, 4) for k in range(4)]
+    reqs[1].cloth.pcg_maxit = 50
+    reqs[3].cloth.pcg_tol = 1e-4
+    out = serve.Server(1 << 40, lanes=()).run(reqs, realtime=False)
+    assert not out["failures"]
+    for q in reqs:
+        _check({r.rid: r for r in out["results"]}[q.rid], q)
+    assert len(out["lanes"]) == 3
+
+
+def test_variant_tuner_tries_both_then_keeps_the_faster():
+    t = serve._VariantTuner(explore=4)
+    seq = []
+    for k in range(9):
+        v = t.pick()
+        seq.append(v)
+        t.record(v, 1.0 if v == 0 else 3.0)  # the multi-launch variant is faster here
+    assert seq[:2] == [2, 0]
+    assert seq[2:] == [0, 2, 0, 0, 0, 2, 0]  # every 4th pick explores the other
+
+
+def test_autotuned_lanes_give_the_same_results(fake):
+    reqs = [_req(k, 8 + k, 6) for k in range(4)]
+    out = serve.Server(1 << 40, lanes=(), autotune=True).run(reqs, realtime=False)
+    assert not out["failures"]
+    by = {r.rid: r for r in out["results"]}
+    for q in reqs:
+        _check(by[q.rid], q)
 
 
 def test_budget_and_creation_failures_are_per_request(fake):
