@@ -1058,6 +1058,86 @@ __global__ void k_spmv_dir(Sel sel, int it) {
   spmv_dir_row(c, i, first, c.scal[kBeta], pold, pnew);
 }
 
+// k_spmv_dir with a group of kSpG lanes per row: lane l forms block l's
+// product (the diagonal block on lane deg), reading the row's contiguous
+// blocks side by side; the products are then added in the reference's order
+// (off-diagonal blocks in column order, the diagonal last) as spmv_dir_row
+// adds them, so every result is the same bit for bit.  Rows with more than
+// 2 kSpG - 1 blocks fall back to spmv_dir_row on the group's first lane.
+constexpr int kSpG = 16;
+template <class Sel>
+__global__ __launch_bounds__(kTB) void k_spmv_dir_g(Sel sel, int it) {
+  const DevCloth c = sel.cloth();
+  if (c.flags[kDone]) return;
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kSpG);
+  const int l = (int)(threadIdx.x % kSpG);
+  if (i >= c.np) return;  // a whole group at a time
+  const bool first = it == 0;
+  const double beta = c.scal[kBeta];
+  const auto z = gptr(c.z);
+  const auto pold = gptr(pcg_dir(c, it + 1));
+  const int rb = gptr(c.rowptr)[i], deg = gptr(c.rowptr)[i + 1] - rb;
+  if (deg >= 2 * kSpG) {
+    if (l == 0) spmv_dir_row(c, i, first, beta, pcg_dir(c, it + 1), pcg_dir(c, it));
+    return;
+  }
+  // the product of block k (k == deg: the diagonal) with p_new of its column
+  auto product = [&](int k, double& t0, double& t1, double& t2) {
+    t0 = t1 = t2 = 0.0;
+    if (k > deg) return;
+    const int j = k < deg ? gptr(c.colind)[rb + k] : i;
+    const auto a = k < deg ? gptr(c.crs) + (size_t)(rb + k) * 9 : gptr(c.dia) + (size_t)i * 9;
+    double x0, x1, x2;
+    if (first) {
+      x0 = z[3 * j];
+      x1 = z[3 * j + 1];
+      x2 = z[3 * j + 2];
+    } else {
+      x0 = z[3 * j] + beta * pold[3 * j];
+      x1 = z[3 * j + 1] + beta * pold[3 * j + 1];
+      x2 = z[3 * j + 2] + beta * pold[3 * j + 2];
+    }
+    t0 = a[0] * x0 + a[1] * x1 + a[2] * x2;
+    t1 = a[3] * x0 + a[4] * x1 + a[5] * x2;
+    t2 = a[6] * x0 + a[7] * x1 + a[8] * x2;
+  };
+  double a0, a1, a2, b0, b1, b2;
+  product(l, a0, a1, a2);
+  product(l + kSpG, b0, b1, b2);
+  double y0 = z[3 * i] * 0.0, y1 = z[3 * i + 1] * 0.0, y2 = z[3 * i + 2] * 0.0;
+  for (int k = 0; k <= deg; k++) {  // deg is the same on every lane of the group
+    const bool hi = k >= kSpG;
+    const int src = k - (hi ? kSpG : 0);
+    const double t0 = __shfl(hi ? b0 : a0, src, kSpG);
+    const double t1 = __shfl(hi ? b1 : a1, src, kSpG);
+    const double t2 = __shfl(hi ? b2 : a2, src, kSpG);
+    y0 += 1.0 * t0;
+    y1 += 1.0 * t1;
+    y2 += 1.0 * t2;
+  }
+  if (l != 0) return;
+  double x0, x1, x2;
+  if (first) {
+    x0 = z[3 * i];
+    x1 = z[3 * i + 1];
+    x2 = z[3 * i + 2];
+  } else {
+    x0 = z[3 * i] + beta * pold[3 * i];
+    x1 = z[3 * i + 1] + beta * pold[3 * i + 1];
+    x2 = z[3 * i + 2] + beta * pold[3 * i + 2];
+  }
+  const auto ap = gptr(c.ap), pnew = gptr(pcg_dir(c, it)), prod = gptr(c.prod);
+  ap[3 * i] = y0;
+  ap[3 * i + 1] = y1;
+  ap[3 * i + 2] = y2;
+  pnew[3 * i] = x0;
+  pnew[3 * i + 1] = x1;
+  pnew[3 * i + 2] = x2;
+  prod[3 * i] = x0 * y0;
+  prod[3 * i + 1] = x1 * y1;
+  prod[3 * i + 2] = x2 * y2;
+}
+
 // one entry of k_update_rx: r -= alpha Ap, dx += alpha p, r .* r, and r as
 // the next forward sweep's right-hand side
 __device__ __forceinline__ void update_rx_elem(const DevCloth& c, int e, double alpha,
@@ -1466,7 +1546,8 @@ int run_pcg(const DevCloth& c, double tol, int maxit, int* host_flags, hipStream
   for (; it < maxit && !done; it++) {
     {
       ProfScope ps(prof, kProfSpmv);
-      hipLaunchKernelGGL(k_spmv_dir<One>, dim3(grid_for(c.np)), dim3(kTB), 0, s, One{c, {}}, it);
+      hipLaunchKernelGGL(k_spmv_dir_g<One>, dim3(grid_for((long)kSpG * c.np)), dim3(kTB), 0, s,
+                         One{c, {}}, it);
     }
     {
       ProfScope ps(prof, kProfReduce);
@@ -1627,7 +1708,8 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
   for (int it = 0; it < maxit && !done; it++) {
     {
       ProfScope ps(prof, kProfSpmv);
-      hipLaunchKernelGGL(k_spmv_dir<Many>, dim3(gp, B), dim3(kTB), 0, s, m, it);
+      hipLaunchKernelGGL(k_spmv_dir_g<Many>, dim3(grid_for((long)kSpG * b.max_np), B), dim3(kTB), 0,
+                         s, m, it);
     }
     {
       ProfScope ps(prof, kProfReduce);
