@@ -285,6 +285,34 @@ def fold_leg(steps: int, copy_gbs, maxit: int) -> dict:
                          "frac_of_stream_copy": ach / copy_gbs if (ach and copy_gbs) else None}}
 
 
+def stack_leg(steps: int, maxit: int) -> dict:
+    """Four stacked 512x512 sheets (1,048,576 vertices, clearance 5 mm, over a
+    bar and the ground; configs/stack4x512.json): whole steps from the
+    configuration's initial state, with the per-class device time of each
+    step from HIP events (the contact pass is one class)."""
+    from cloth_sim_self_collision_amd.sim import DeviceCloth
+    cl = scene.load_config(os.path.join(ROOT, "cloth_sim_self_collision_amd", "configs",
+                                        "stack4x512.json"))
+    cl.pcg_maxit = maxit
+    with DeviceCloth(cl) as d:
+        d.profile(True)
+        lat, rec = [], []
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            st = d.step(1)
+            lat.append(time.perf_counter() - t1)
+            rec.append({k: st[k] for k in ("pcg_iters", "n_prox", "n_ccd_pass", "n_ccd",
+                                           "n_riz_pass", "impulse_applied")})
+        prof = d.profile_read()
+        d.profile(False)
+        np_ = d.np_
+    print(f"[bench] stack4x512: {steps} steps in {sum(lat):.2f} s", file=sys.stderr, flush=True)
+    return {"workload": "stack4x512", "np": np_, "nt": cl.nt, "steps": steps, "pcg_maxit": maxit,
+            "ms_per_step": sum(lat) / steps * 1e3, "step_ms": [t * 1e3 for t in lat],
+            "per_step": rec,
+            "class_ms_per_step": {k: v[0] / steps for k, v in prof.items() if v[1]}}
+
+
 def batch_leg(count: int, steps: int, copy_gbs, maxit: int) -> dict:
     """The batch of `count` cloths (sides 24..128, seed 0, each dropped onto
     its own sphere or box): one plain round (each cloth one step alone), then
@@ -388,6 +416,9 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=64, help="cloths of the batch (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--drape-steps", type=int, default=50, help="steps of drape32 (0: skip)")
+    ap.add_argument("--stack-steps", type=int, default=0,
+                    help="whole steps of the four stacked 512^2 sheets (0: skip, the default: "
+                         "about 1 s of setup and 13 s per step at the PCG cap of 1000)")
     ap.add_argument("--pcg-maxit", type=int, default=1000,
                     help="PCG iteration cap of every workload (1000; lower only for profiling "
                          "runs, whose line is then not the benchmark)")
@@ -424,6 +455,7 @@ def main() -> int:
         if args.batch_count > 0 else (None, None)
     drape, drape_cl = drape_leg(args.drape_steps, args.pcg_maxit) if args.drape_steps > 0 \
         else (None, None)
+    stack = stack_leg(args.stack_steps, args.pcg_maxit) if args.stack_steps > 0 else None
 
     cpu, err = None, None
     if not args.no_cpu:
@@ -458,6 +490,8 @@ def main() -> int:
         for leg in ("f64", "mixed"):
             drape[leg].pop("x", None)
     out.update({"cpu_baseline": cpu, "fold256": fold, "batch64": batch, "drape32": drape})
+    if stack is not None:
+        out["stack4x512"] = stack
     # vs_baseline stays null: the reference publishes no number for this metric
     if cpu and out.get("value"):
         out["value_vs_cpu_baseline"] = out["value"] / cpu["value"]

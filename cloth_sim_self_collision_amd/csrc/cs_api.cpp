@@ -146,6 +146,7 @@ int build(Cloth* c, const cs_mesh* m) {
   int *tri, *quad, *bc, *rowptr, *colind, *diaind, *rowof, *lfp, *lfr, *lbp, *lbr;
   int *fup, *fuk, *fuc;
   int *csp, *cs_, *dsp, *ds, *gsp, *gs, *rp, *rg, *node, *pre, *pree, *bypre, *leafof, *rord;
+  int *rend, *lrank;
   double* xyz0;
   A.reserve(&tri, 3 * (size_t)nt);
   A.reserve(&quad, 4 * (size_t)nq);
@@ -175,6 +176,8 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&pree, nn);
   A.reserve(&bypre, nn);
   A.reserve(&leafof, nt);
+  A.reserve(&rend, nn);
+  A.reserve(&lrank, nt);
   A.reserve(&rord, nn);
   A.reserve(&d.x, 3 * (size_t)np);
   A.reserve(&d.v, 3 * (size_t)np);
@@ -212,6 +215,7 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&d.scal, kNumScal);
   A.reserve(&d.flags, kNumFlag);
   A.reserve(&d.bb, 6 * (size_t)nn);
+  A.reserve(&d.rbb, 6 * (size_t)nn);
   A.reserve(&c->ising, np);
   int *frc, *brc, *fsl;
   unsigned short *fmeta, *bmeta;
@@ -270,6 +274,8 @@ int build(Cloth* c, const cs_mesh* m) {
   d.pre_end = pree;
   d.by_pre = bypre;
   d.leaf_of = leafof;
+  d.rend = rend;
+  d.leaf_rank = lrank;
   d.refit_order = rord;
   d.kl = S.kl;
   d.ku = S.ku;
@@ -323,6 +329,20 @@ int build(Cloth* c, const cs_mesh* m) {
   if (int st = upload(pree, S.pre_end, s)) return st;
   if (int st = upload(bypre, S.by_pre, s)) return st;
   if (int st = upload(leafof, S.leaf_of, s)) return st;
+  // by pre-order rank: subtree ends and the leaves (held until the stream
+  // synchronises below, as the copies read them)
+  std::vector<int> rank_end(nn), leaf_ranks;
+  leaf_ranks.reserve(nt);
+  for (int r = 0; r < nn; r++) {
+    rank_end[r] = S.pre_end[S.by_pre[r]];
+    if (rank_end[r] == r + 1) leaf_ranks.push_back(r);
+  }
+  if ((int)leaf_ranks.size() != nt) {
+    set_error("cs_cloth_create: the BVH does not have one leaf per triangle");
+    return kBadArg;
+  }
+  if (int st = upload(rend, rank_end, s)) return st;
+  if (int st = upload(lrank, leaf_ranks, s)) return st;
   if (int st = upload(rord, S.refit_order, s)) return st;
   if (S.kl > 0) {
     if (int st = upload(qfp, S.qf_ptr, s)) return st;

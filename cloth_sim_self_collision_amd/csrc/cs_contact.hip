@@ -26,6 +26,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 
 #include "cs_contact.hpp"
@@ -94,7 +95,9 @@ __global__ void k_refit_leaves(DevCloth c, const double* X, const double* U, dou
     box_grow(b, at(X, i1), eps);
     box_grow(b, at(X, i2), eps);
   }
-  store_box(c.bb, c.leaf_of[t], b);
+  const int L = c.leaf_of[t];
+  store_box(c.bb, L, b);
+  store_box(c.rbb, c.pre[L], b);
 }
 
 // the internal nodes among refit_order[b .. e) (one depth): box = merge of
@@ -104,7 +107,9 @@ __global__ void k_refit_merge(DevCloth c, int b, int e) {
   if (k >= e) return;
   const int n = c.refit_order[k];
   if (c.node[3 * n + 2] == -1) return;  // a leaf: boxed from its triangle
-  store_box(c.bb, n, box_merge(load_box(c.bb, c.node[3 * n + 1]), load_box(c.bb, c.node[3 * n + 2])));
+  const Box m = box_merge(load_box(c.bb, c.node[3 * n + 1]), load_box(c.bb, c.node[3 * n + 2]));
+  store_box(c.bb, n, m);
+  store_box(c.rbb, c.pre[n], m);
 }
 
 // ---------------------------------------------------------------------------
@@ -212,39 +217,88 @@ __device__ void narrow_pair(const DevCloth& c, const double* X, const double* U,
   }
 }
 
-// one thread per leaf: all later leaves (pre-order) with overlapping boxes
-// The traversal records the overlapping leaf pairs and k_narrow tests them,
-// one pair per thread: inside the traversal, a thread's narrow tests (15 per
-// pair, cubic solves in the continuous case) waited for the busiest lane of
-// its wave.  The records are sorted by key afterwards, so their order does
-// not matter.
-__global__ void k_traverse(DevCloth c, int* counters, int cap, int* cand) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= c.nt) return;
-  const int L = c.leaf_of[t];
-  const int rl = c.pre[L];
-  const Box bl = load_box(c.bb, L);
-  int r = 0;
-  while (r < c.nnode) {
-    const int n = c.by_pre[r];
-    const int end = c.pre_end[n];
-    if (end <= rl + 1) {  // no leaf after L below n
-      r = end;
-      continue;
-    }
-    if (!overlap(bl, load_box(c.bb, n))) {
-      r = end;
-      continue;
-    }
-    if (c.node[3 * n + 2] == -1 && r > rl) {
-      const int slot = atomicAdd(&counters[kCand], 1);
-      if (slot < cap) {
-        cand[2 * slot] = L;
-        cand[2 * slot + 1] = n;
-      }
-    }
-    r++;
+// Overlapping leaf pairs (L, M), M after L in pre-order, as the reference's
+// pair recursion finds them (bvh_aabb.cpp:461-474, 544-556): the records are
+// sorted by key afterwards and k_select orders visits by the tree alone, so
+// the order in which pairs are found does not matter.
+//
+// One wave per 64 consecutive leaves in pre-order.  Each lane's own walk is a
+// pre-order scan from the root that skips a subtree (jumps to its end) when
+// it holds no leaf after the lane's leaf or its box does not overlap the
+// leaf's box, and records every leaf it reaches after its own.  The wave walks
+// the union of its lanes' walks: at rank r a lane takes part iff r is at or
+// past the end of the last subtree it skipped, so every lane tests exactly
+// the nodes its own walk tests; the wave goes on to r + 1 if any lane
+// descends, else to the end of r's subtree, the first rank at which any lane
+// takes part again.  The node data are read by wave-uniform rank (scalar
+// loads of rend / rbb / by_pre), the leaf's box stays in registers, and the
+// pairs go through an LDS buffer, one counter atomic per kTravBuf pairs.
+constexpr int kTravBuf = 512;
+
+// overlap() evaluated without short-circuit branches (each comparison as
+// written there, so NaN coordinates behave the same), which keeps all six
+// bounds of a box in one round of loads
+__device__ __forceinline__ bool overlap_all(const Box& a, const Box& b) {
+  return !(a.hx < b.lx) & !(a.lx > b.hx) & !(a.hy < b.ly) & !(a.ly > b.hy) & !(a.hz < b.lz) &
+         !(a.lz > b.hz);
+}
+
+__device__ __forceinline__ void trav_flush(const int2* buf, int fill, int* __restrict__ counters,
+                                           int cap, int2* __restrict__ cand) {
+  __syncthreads();  // the buffer's entries, written by every lane
+  int base = 0;
+  if (threadIdx.x == 0) base = atomicAdd(&counters[kCand], fill);
+  base = __shfl(base, 0);
+  for (int i = (int)threadIdx.x; i < fill; i += 64)
+    if (base + i < cap) cand[base + i] = buf[i];
+  __syncthreads();  // read out before the buffer is reused
+}
+
+__global__ __launch_bounds__(64) void k_traverse(int nnode, int nt,
+                                                 const int* __restrict__ leaf_rank,
+                                                 const int* __restrict__ rend,
+                                                 const double* __restrict__ rbb,
+                                                 const int* __restrict__ by_pre,
+                                                 int* __restrict__ counters, int cap,
+                                                 int2* __restrict__ cand) {
+  __shared__ int2 buf[kTravBuf];
+  const int k = (int)(blockIdx.x * 64 + threadIdx.x);
+  int rl = 0, L = -1, skip = INT_MAX;  // lanes past the last leaf never take part
+  Box bl{};
+  if (k < nt) {
+    rl = leaf_rank[k];
+    L = by_pre[rl];
+    bl = load_box(rbb, rl);
+    skip = 0;
   }
+  int fill = 0;
+  int r = 0;
+  while (r < nnode) {
+    // all three loads issue together (no branch between them and their use)
+    const int end = rend[r];
+    const Box b = load_box(rbb, r);
+    const int m = by_pre[r];
+    const bool alive = r >= skip;
+    const bool down = alive & (end > rl + 1) & overlap_all(bl, b);
+    const bool hit = down & (end == r + 1) & (r > rl);
+    skip = (alive & !down) ? end : skip;
+    const unsigned long long hm = __ballot(hit);
+    if (hm) {
+      const int cnt = __popcll(hm);
+      if (fill + cnt > kTravBuf) {
+        trav_flush(buf, fill, counters, cap, cand);
+        fill = 0;
+      }
+      if (hit) {
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(hm >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)hm, 0u));
+        buf[fill + below] = make_int2(L, m);
+      }
+      fill += cnt;
+    }
+    r = __ballot(down) ? r + 1 : end;
+  }
+  if (fill) trav_flush(buf, fill, counters, cap, cand);
 }
 
 // n overlapping leaf pairs (n <= the buffer's capacity)
@@ -838,8 +892,9 @@ int query(ContactState& cs, const DevCloth& d, hipStream_t s, const double* X, c
   int ncand = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
     CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kQueryCounters, s));
-    hipLaunchKernelGGL(k_traverse, dim3(grid_for(d.nt, 64)), dim3(64), 0, s, d, cs.counters,
-                       cs.cand_cap, cs.cand);
+    hipLaunchKernelGGL(k_traverse, dim3(grid_for(d.nt, 64)), dim3(64), 0, s, d.nnode, d.nt,
+                       d.leaf_rank, d.rend, d.rbb, d.by_pre, cs.counters, cs.cand_cap,
+                       reinterpret_cast<int2*>(cs.cand));
     CS_HIP(hipGetLastError());
     if (int st = read_counters(cs, s)) return st;
     ncand = cs.hcounters[kCand];

@@ -117,6 +117,11 @@ struct DevCloth {
   const int* by_pre;
   const int* leaf_of;
   const int* refit_order;
+  // the traversal's view of the tree by pre-order rank: the subtree end of
+  // the node at each rank (a leaf: rank + 1) and the rank of each leaf in
+  // pre-order (nt entries)
+  const int* rend;
+  const int* leaf_rank;
   // state
   double* x;   // positions (3 np)
   double* v;   // velocities (3 np)
@@ -144,8 +149,9 @@ struct DevCloth {
   SumScratch sum2;
   double* scal;
   int* flags;
-  // BVH boxes (6 per node)
+  // BVH boxes (6 per node), by node id and by pre-order rank
   double* bb;
+  double* rbb;
   // packed sweeps (null when a row has more than kMaxBlocksPerRow blocks)
   int kl, ku, ring_ok;
   int kpack;          // packed blocks per row of both sweeps: max(kl, ku), 0 = not packed

@@ -991,7 +991,7 @@ __global__ void k_matvec(int np, const int* rowptr, const int* colind, const dou
 //           that holds z (ilu_sparse.cpp:670-672), scaling it by beta = 0
 //   prod  = p_new .* ap
 // iteration `it`: p_new = c.p[it & 1], p_old = c.p[(it + 1) & 1]
-// one row of k_spmv_dir: p_new[i] (= z[i] or z[i] + beta p_old[i]), Ap[i]
+// one row on one lane: p_new[i] (= z[i] or z[i] + beta p_old[i]), Ap[i]
 // and p_new[i] .* Ap[i]
 __device__ __forceinline__ void spmv_dir_row(const DevCloth& c, int i, bool first, double beta,
                                              const double* __restrict__ pold_,
@@ -1046,19 +1046,7 @@ __device__ __forceinline__ void spmv_dir_row(const DevCloth& c, int i, bool firs
   prod[3 * i + 2] = x2 * y2;
 }
 
-template <class Sel>
-__global__ void k_spmv_dir(Sel sel, int it) {
-  const DevCloth c = sel.cloth();
-  const int first = it == 0;
-  const double* __restrict__ pold = pcg_dir(c, it + 1);
-  double* __restrict__ pnew = pcg_dir(c, it);
-  if (c.flags[kDone]) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.np) return;
-  spmv_dir_row(c, i, first, c.scal[kBeta], pold, pnew);
-}
-
-// k_spmv_dir with a group of kSpG lanes per row: lane l forms block l's
+// p_new, A p_new and their products with a group of kSpG lanes per row: lane l forms block l's
 // product (the diagonal block on lane deg), reading the row's contiguous
 // blocks side by side; the products are then added in the reference's order
 // (off-diagonal blocks in column order, the diagonal last) as spmv_dir_row
