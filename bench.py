@@ -248,8 +248,18 @@ def serve_leg(args, copy_gbs) -> dict:
                 all(np.array_equal(by_rid[rid].xyz, x) and np.array_equal(by_rid[rid].uvw, v)
                     for rid, x, v in plain["results"]))
     roof = roofline_from_ticks(srv.tick_log, copy_gbs, "served") if srv.tick_log else None
+    # the served run's timeline: per tick, its lane, start (s), duration (ms)
+    # and its members' vertex counts
+    timeline = [[r["lane"], round(r["t0_s"], 3), round(r["ms"], 1), [m[0] for m in r["members"]]]
+                for r in sorted(srv.tick_log, key=lambda r: r["t0_s"])]
+    # per request: id, vertices, arrival, admission and finish (s)
+    arr = {r.rid: r.arrival for r in reqs}
+    req_times = [[x.rid, x.vertices, round(arr[x.rid], 3), round(x.admitted_s, 3),
+                  round(x.finished_s, 3)] for x in sorted(served["results"], key=lambda x: x.rid)]
     return {"reqs": reqs, "drawn_steps": drawn, "served": served, "plain": plain, "same": same,
-            "roofline": roof, "lane_pcg": srv.lane_pcg, "lanes": list(srv.bounds),
+            "roofline": roof, "timeline": timeline, "req_times": req_times,
+            "lane_pcg": srv.lane_pcg,
+            "lanes": list(srv.bounds),
             "solo_from": srv.solo_from}
 
 
@@ -534,6 +544,7 @@ def headline(args, sv, err) -> dict:
         "pos_err_scope": "first step of every request: GPU plain step vs the CPU baseline's step",
         "pcg_iters": {str(r.rid): by_rid[r.rid].pcg_iters for r in reqs if r.rid in by_rid},
         "lane_ticks": served["lanes"], "solo_ticks": served["solo"],
+        "timeline": sv["timeline"], "request_times": sv["req_times"],
         "roofline": sv["roofline"],
     }
 

@@ -92,6 +92,17 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gptr(T* p) {
   return (__attribute__((address_space(1))) T*)p;
 }
 
+// relaxed device-scope accesses: each is coherent across the XCDs by itself
+// (no cache writeback or invalidate around it)
+template <class T>
+__device__ __forceinline__ void store_device(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T load_device(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 constexpr int kSumBlock = 256;                  // threads per block
 constexpr int kSumPer = 4;                      // consecutive elements per thread
 constexpr int kSumTile = kSumBlock * kSumPer;   // elements per tile
@@ -570,21 +581,25 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
     }
     block_minmax64(mn, mx, sh);
     if (tid == 0) {
-      gstore(w.code + b, ok ? code : -1);
-      gstore(w.tot + b, total);
-      gstore(w.pmin + b, mn);
-      gstore(w.pmax + b, mx);
+      store_device(w.code + b, ok ? code : -1);
+      store_device(w.tot + b, total);
+      store_device(w.pmin + b, mn);
+      store_device(w.pmax + b, mx);
+      // The last block to finish chains the tiles, on its first wave.  The
+      // summary is stored and read with device-scope accesses (coherent per
+      // instruction), and its stores have completed before the ticket is
+      // taken, so the last block reads every summary without a device-scope
+      // fence: on this GPU such a fence writes back and invalidates the L2
+      // of the XCD, which slowed every kernel running beside the reduction.
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      sh.last = atomicAdd(w.ticket, 1u) == (unsigned)(nt - 1);
     }
   }
-  // the last block to finish chains the tiles, on its first wave
-  __threadfence();
-  if (tid == 0) sh.last = atomicAdd(w.ticket, 1u) == (unsigned)(nt - 1);
   __syncthreads();
   if (!sh.last || tid >= 64) return;
-  __threadfence();
   const unsigned long long chain_t0 = w.stats ? wall_clock64() : 0;
   const int lane = tid;
-  if (lane == 0) *w.ticket = 0u;
+  if (lane == 0) store_device(w.ticket, 0u);
   double s = 0.0;
   int fallback = 0;
   for (int b0 = 0; b0 < nt && !isnan(s); b0 += kChainWin) {
@@ -597,10 +612,10 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_exact(Src src) {
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int j = min(j0 + r * 64 + lane, wend - 1);
-        cd[r] = gload(w.code + j);
-        tt[r] = gload(w.tot + j);
-        mn[r] = gload(w.pmin + j);
-        mx[r] = gload(w.pmax + j);
+        cd[r] = load_device(w.code + j);
+        tt[r] = load_device(w.tot + j);
+        mn[r] = load_device(w.pmin + j);
+        mx[r] = load_device(w.pmax + j);
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {

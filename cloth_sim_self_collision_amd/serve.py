@@ -230,7 +230,7 @@ class _Lane:
                     # what each member did in it
                     prof = self.batch.profile_read()
                     self.batch.profile(True)
-                    rec = {"lane": self.index, "ms": dt_ms, "prof": prof,
+                    rec = {"lane": self.index, "t0_s": ts - srv.t0, "ms": dt_ms, "prof": prof,
                            "members": [(a.dev.np_, a.dev.nnz, st["pcg_iters"], st["status"])
                                        for a, st in zip(self.active, stats)]}
                     with srv.cv:
@@ -301,7 +301,8 @@ class _Lane:
                 dt_ms = (srv.clock() - ts) * 1e3
                 self.ticks += 1
                 if srv.profile:
-                    rec = {"lane": self.index, "ms": dt_ms, "prof": a.dev.profile_read(),
+                    rec = {"lane": self.index, "t0_s": ts - srv.t0, "ms": dt_ms,
+                           "prof": a.dev.profile_read(),
                            "members": [(a.dev.np_, a.dev.nnz, st["pcg_iters"], st["status"])]}
                     a.dev.profile(True)
                     with srv.cv:
