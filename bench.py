@@ -323,6 +323,36 @@ def stack_leg(steps: int, maxit: int) -> dict:
             "class_ms_per_step": {k: v[0] / steps for k, v in prof.items() if v[1]}}
 
 
+def copies_leg(counts, maxit: int) -> list:
+    """Batches of N copies of the batch workload's largest cloth (128x128):
+    one step each, with the per-launch time of each kernel class, for every
+    N in `counts`.  Every member does the same work, so what grows with N is
+    what batching itself costs."""
+    from cloth_sim_self_collision_amd.sim import DeviceBatch, DeviceCloth
+    cls = scene.batch_drop(64, seed=0)
+    big = max(cls, key=lambda c: c.np_)
+    big.pcg_maxit = maxit
+    out = []
+    for n in counts:
+        devs = [DeviceCloth(big) for _ in range(n)]
+        try:
+            with DeviceBatch(devs) as b:
+                b.step(1)
+                b.profile(True)
+                t0 = time.perf_counter()
+                b.step(1)
+                ms = (time.perf_counter() - t0) * 1e3
+                prof = b.profile_read()
+                b.profile(False)
+        finally:
+            for d in devs:
+                d.close()
+        out.append({"copies": n, "np": big.np_, "ms_per_step": ms,
+                    "per_launch_us": {k: v[0] / v[1] * 1e3 for k, v in prof.items() if v[1]}})
+        print(f"[bench] {n} copies: {ms:.1f} ms per step", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_leg(count: int, steps: int, copy_gbs, maxit: int) -> dict:
     """The batch of `count` cloths (sides 24..128, seed 0, each dropped onto
     its own sphere or box): one plain round (each cloth one step alone), then
@@ -426,6 +456,8 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=64, help="cloths of the batch (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--drape-steps", type=int, default=50, help="steps of drape32 (0: skip)")
+    ap.add_argument("--copies", default="",
+                    help="batches of N copies of the 128x128 cloth, e.g. 1,8,16,32,64 (default: skip)")
     ap.add_argument("--stack-steps", type=int, default=0,
                     help="whole steps of the four stacked 512^2 sheets (0: skip, the default: "
                          "about 1 s of setup and 13 s per step at the PCG cap of 1000)")
@@ -466,6 +498,8 @@ def main() -> int:
     drape, drape_cl = drape_leg(args.drape_steps, args.pcg_maxit) if args.drape_steps > 0 \
         else (None, None)
     stack = stack_leg(args.stack_steps, args.pcg_maxit) if args.stack_steps > 0 else None
+    copies = copies_leg([int(v) for v in args.copies.split(",")], args.pcg_maxit) \
+        if args.copies else None
 
     cpu, err = None, None
     if not args.no_cpu:
@@ -502,6 +536,8 @@ def main() -> int:
     out.update({"cpu_baseline": cpu, "fold256": fold, "batch64": batch, "drape32": drape})
     if stack is not None:
         out["stack4x512"] = stack
+    if copies is not None:
+        out["batch_copies"] = copies
     # vs_baseline stays null: the reference publishes no number for this metric
     if cpu and out.get("value"):
         out["value_vs_cpu_baseline"] = out["value"] / cpu["value"]
