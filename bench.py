@@ -248,9 +248,10 @@ def serve_leg(args, copy_gbs) -> dict:
                 all(np.array_equal(by_rid[rid].xyz, x) and np.array_equal(by_rid[rid].uvw, v)
                     for rid, x, v in plain["results"]))
     roof = roofline_from_ticks(srv.tick_log, copy_gbs, "served") if srv.tick_log else None
-    # the served run's timeline: per tick, its lane, start (s), duration (ms)
-    # and its members' vertex counts
-    timeline = [[r["lane"], round(r["t0_s"], 3), round(r["ms"], 1), [m[0] for m in r["members"]]]
+    # the served run's timeline: per tick, its lane, start (s), duration (ms),
+    # its members' vertex counts and the device time of each kernel class (ms)
+    timeline = [[r["lane"], round(r["t0_s"], 3), round(r["ms"], 1), [m[0] for m in r["members"]],
+                 {k: round(v[0], 1) for k, v in r["prof"].items() if v[1]}]
                 for r in sorted(srv.tick_log, key=lambda r: r["t0_s"])]
     # per request: id, vertices, arrival, admission and finish (s)
     arr = {r.rid: r.arrival for r in reqs}
