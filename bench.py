@@ -227,6 +227,10 @@ def serve_leg(args, copy_gbs) -> dict:
         kw["lane_pcg"] = tuple(int(v) for v in args.lane_pcg.split(","))
     if args.autotune:
         kw["autotune"] = True
+    if args.lane_instances:
+        kw["lane_instances"] = tuple(int(v) for v in args.lane_instances.split(","))
+    if args.lane_window:
+        kw["lane_window"] = tuple(float(v) for v in args.lane_window.split(","))
     # warm-up: the three smallest requests, served and plain, W steps each
     small = sorted(reqs, key=lambda r: r.cloth.np_)[:3]
     warm = [serve.Request(f"warm{k}", r.cloth, steps=max(1, args.warmup)) for k, r in enumerate(small)]
@@ -259,7 +263,8 @@ def serve_leg(args, copy_gbs) -> dict:
                   round(x.finished_s, 3)] for x in sorted(served["results"], key=lambda x: x.rid)]
     return {"reqs": reqs, "drawn_steps": drawn, "served": served, "plain": plain, "same": same,
             "roofline": roof, "timeline": timeline, "req_times": req_times,
-            "lane_pcg": srv.lane_pcg,
+            "lane_pcg": srv.lane_pcg, "lane_instances": srv.lane_instances,
+            "lane_window": srv.lane_window,
             "lanes": list(srv.bounds),
             "solo_from": srv.solo_from}
 
@@ -442,6 +447,10 @@ def main() -> int:
     ap.add_argument("--lanes", default="", help="lane vertex-count bounds, e.g. 2500,21846 "
                                                 "(default: the server's)")
     ap.add_argument("--lane-pcg", default="", help="PCG variant per lane, e.g. 2,2,0")
+    ap.add_argument("--lane-instances", default="",
+                    help="instances per lane that may step at once, e.g. 1,1,1,1,1,2")
+    ap.add_argument("--lane-window", default="",
+                    help="seconds an idle lane instance gathers members, e.g. 0,0,0,0,0,1")
     ap.add_argument("--autotune", action="store_true",
                     help="lanes pick their PCG variant from measured ticks")
     ap.add_argument("--solo-from", type=int, default=-1,
@@ -567,6 +576,8 @@ def headline(args, sv, err) -> dict:
                    "dt": reqs[0].cloth.dt, "pcg_tol": reqs[0].cloth.pcg_tol,
                    "pcg_maxit": reqs[0].cloth.pcg_maxit, "lanes_vertex_bounds": sv["lanes"],
                    "lane_pcg": list(sv["lane_pcg"]), "lane_autotune": bool(args.autotune),
+                   "lane_instances": list(sv["lane_instances"]),
+                   "lane_window_s": list(sv["lane_window"]),
                    "solo_from_vertices": sv["solo_from"],
                    "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                    "precision": "f64, bit-exact"},

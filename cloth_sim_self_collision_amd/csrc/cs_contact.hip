@@ -556,32 +556,52 @@ __global__ void k_impulse_ready(int g, const int* pred, int* npred, int* ready, 
 // written them, so the velocities are those of the key-order Gauss-Seidel
 // sweep.  Every contact enters a list once, so the loop ends after at most
 // g contacts; the count of those run goes to counters[kDoneCnt].
+//
+// A round's chain of memory latencies is what the schedule waits for, so
+// the lists after the first live in LDS (up to kDrainLds entries; the rest
+// of a longer round in the global list), and a contact releases its
+// successors before its impulse: they only run in the next round, after
+// the barrier that ends this one, and their counter atomics overlap the
+// impulse's loads.
+constexpr int kDrainLds = 4096;
+
 __global__ __launch_bounds__(1024) void k_impulse_drain(const int* ce, const int* succ, int* npred,
                                                         int* ready0, int* ready1, const double* X,
                                                         double* U, double delta, double k,
                                                         double dt, double m, int ccd,
                                                         int* counters) {
   __shared__ int n_next;
+  __shared__ int lists[2][kDrainLds];
   int n = counters[kReady];
-  int *cur = ready0, *nxt = ready1;
+  int *gcur = ready0, *gnxt = ready1;  // the first round's list and the spills
+  int cur = 0;                         // the LDS list of this round
+  bool first = true;                   // the first round reads k_impulse_ready's list
   int done = 0;
   while (n > 0) {
     if (threadIdx.x == 0) n_next = 0;
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-      const int i = cur[t];
-      impulse_one(ce + 5 * i, X, U, delta, k, dt, m, ccd);
+      const int i = (first || t >= kDrainLds) ? gcur[t] : lists[cur][t];
       for (int a = 0; a < 4; a++) {
         const int sc = succ[4 * i + a];
-        if (sc >= 0 && atomicSub(&npred[sc], 1) == 1) nxt[atomicAdd(&n_next, 1)] = sc;
+        if (sc >= 0 && atomicSub(&npred[sc], 1) == 1) {
+          const int pos = atomicAdd(&n_next, 1);
+          if (pos < kDrainLds)
+            lists[cur ^ 1][pos] = sc;
+          else
+            gnxt[pos] = sc;
+        }
       }
+      impulse_one(ce + 5 * i, X, U, delta, k, dt, m, ccd);
     }
     done += n;
     __syncthreads();  // the round's velocities and the next list are complete
     n = n_next;
-    int* tmp = cur;
-    cur = nxt;
-    nxt = tmp;
+    int* tmp = gcur;
+    gcur = gnxt;
+    gnxt = tmp;
+    cur ^= 1;
+    first = false;
     __syncthreads();  // every thread has read n_next before it is reset
   }
   if (threadIdx.x == 0) counters[kDoneCnt] = done;

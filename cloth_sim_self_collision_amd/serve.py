@@ -91,6 +91,15 @@ _OPT_PCG = 2
 # requests of at least this many vertices are stepped alone, each by its own
 # thread on its own stream, instead of joining a lane (None: never)
 DEFAULT_SOLO_FROM: Optional[int] = None
+# Per lane: how many instances of the lane may step at once, and how long an
+# idle instance gathers members before its first tick (seconds).  A request
+# that arrives while every instance of its lane is in a tick starts another
+# instance (up to the limit) instead of waiting for the next tick, and an
+# instance that starts from idle waits this window for the requests arriving
+# just after (a tick cannot take members once it runs).  Default: off
+# (one instance, no window) in every lane.
+DEFAULT_LANE_INSTANCES = (1, 1, 1, 1, 1, 1)
+DEFAULT_LANE_WINDOW = (0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
 # cloths created at once on admission (host setup and upload)
 CREATORS = 4
 _NONFINITE = "the solve produced an infinite or NaN displacement (the ILU(0)-PCG iterate overflowed)"
@@ -173,6 +182,8 @@ class _Lane:
         self.ticks = 0
         self.solo = solo
         self.tuner = _VariantTuner() if tune and solo is None else None
+        self.window = 0.0     # seconds an idle instance gathers members before a tick
+        self.in_tick = False  # stepping now (set under the server's lock)
         target = self._run if solo is None else self._run_solo
         self.thread = threading.Thread(target=target, name=f"lane{index}", daemon=True)
 
@@ -191,6 +202,12 @@ class _Lane:
                         srv.cv.wait(0.05)
                     if srv.stopping and not self.inbox and not self.active:
                         return
+                    if not self.active and self.window > 0.0:
+                        # from idle: gather the requests arriving within the
+                        # window into this first tick
+                        until = srv.clock() + self.window
+                        while not srv.stopping and srv.clock() < until:
+                            srv.cv.wait(min(0.02, max(0.0, until - srv.clock())))
                     self.active.extend(self.inbox)
                     self.inbox.clear()
                     going = [a for a in self.active if a.cancel or a.evict]
@@ -210,6 +227,8 @@ class _Lane:
                     if srv.profile:
                         self.batch.profile(True)
                 variant = self._choose_variant()
+                with srv.cv:
+                    self.in_tick = True
                 ts = srv.clock()
                 try:
                     stats = self.batch.step(1)
@@ -221,6 +240,9 @@ class _Lane:
                         self.active.remove(a)
                         self._leave(a, f"step failed: {e}")
                     continue
+                finally:
+                    with srv.cv:
+                        self.in_tick = False
                 dt_ms = (srv.clock() - ts) * 1e3
                 self.ticks += 1
                 if variant is not None:
@@ -392,7 +414,11 @@ class Server:
     (_VariantTuner) instead of lane_pcg;
     solo_from: requests of at least this many vertices are stepped alone by a
     thread of their own (plain steps on the cloth's stream) instead of
-    joining a lane, so a large cloth never paces smaller ones (None: never).
+    joining a lane, so a large cloth never paces smaller ones (None: never);
+    lane_instances / lane_window: per lane, how many instances may step at
+    once (a request arriving while every instance is in a tick starts
+    another) and how long an idle instance gathers members before its first
+    tick (seconds); None: DEFAULT_LANE_INSTANCES / DEFAULT_LANE_WINDOW.
     Set GPU_MAX_HW_QUEUES (HIP's hardware queues per process, 4 by default)
     to at least the number of lanes plus solo requests before HIP starts, or
     streams share queues and their kernels run one after another."""
@@ -402,12 +428,25 @@ class Server:
                  on_error: Optional[Callable[["Failure"], None]] = None,
                  lanes: tuple = DEFAULT_LANES, lane_pcg: Optional[tuple] = None,
                  profile: bool = False, solo_from: Optional[int] = DEFAULT_SOLO_FROM,
-                 autotune: bool = False):
+                 autotune: bool = False, lane_instances: Optional[tuple] = None,
+                 lane_window: Optional[tuple] = None):
         if lane_pcg is None:
             lane_pcg = DEFAULT_LANE_PCG if tuple(lanes) == DEFAULT_LANES else (0,) * (len(lanes) + 1)
+        if lane_instances is None:
+            lane_instances = DEFAULT_LANE_INSTANCES if tuple(lanes) == DEFAULT_LANES \
+                else (1,) * (len(lanes) + 1)
+        if lane_window is None:
+            lane_window = DEFAULT_LANE_WINDOW if tuple(lanes) == DEFAULT_LANES \
+                else (0.0,) * (len(lanes) + 1)
         if len(lane_pcg) != len(lanes) + 1:
             raise ValueError("lane_pcg needs one value per lane")
+        if len(lane_instances) != len(lanes) + 1 or min(lane_instances) < 1:
+            raise ValueError("lane_instances needs one value of at least 1 per lane")
+        if len(lane_window) != len(lanes) + 1 or min(lane_window) < 0.0:
+            raise ValueError("lane_window needs one value of at least 0 per lane")
         self.lane_pcg = tuple(int(v) for v in lane_pcg)
+        self.lane_instances = tuple(int(v) for v in lane_instances)
+        self.lane_window = tuple(float(v) for v in lane_window)
         self.profile = bool(profile)
         self.autotune = bool(autotune)
         self.solo_from = solo_from
@@ -537,7 +576,7 @@ class Server:
             pool.shutdown(wait=True)
             with self.cv:
                 self.stopping = True
-                lanes = list(self.lanes.values())
+                lanes = [ln for group in self.lanes.values() for ln in group]
                 for ln in lanes:
                     for a in ln.active + ln.inbox:
                         a.cancel = True
@@ -605,19 +644,28 @@ class Server:
             if a.admitted_s < 0:
                 a.admitted_s = self.clock() - self.t0
             if self.solo_from is not None and a.req.cloth.np_ >= self.solo_from:
-                ln = _Lane(self, len(self.lanes) + len(self.solo_lanes), solo=a)
+                ln = _Lane(self, sum(len(g) for g in self.lanes.values()) +
+                           len(self.solo_lanes), solo=a)
                 self.solo_lanes.append(ln)
                 ln.thread.start()
             else:
                 cl = a.req.cloth
                 key = (lane, a.req.precision, cl.pcg_tol, cl.pcg_maxit)
-                ln = self.lanes.get(key)
-                if ln is None:
+                group = self.lanes.setdefault(key, [])
+                # an instance not in a tick takes the request into its next
+                # tick; if every one is stepping, another instance starts
+                # (up to the lane's limit), else the first one takes it
+                ln = next((x for x in group if not x.in_tick), None)
+                if ln is None and len(group) < self.lane_instances[lane]:
                     # the one-workgroup solve that the tuner tries is FP64 only
-                    ln = _Lane(self, len(self.lanes) + len(self.solo_lanes),
+                    ln = _Lane(self, sum(len(g) for g in self.lanes.values()) +
+                               len(self.solo_lanes),
                                tune=self.autotune and a.req.precision == "f64")
-                    self.lanes[key] = ln
+                    ln.window = self.lane_window[lane]
+                    group.append(ln)
                     ln.thread.start()
+                elif ln is None:
+                    ln = group[0]
                 ln.inbox.append(a)
             self.cv.notify_all()
 

@@ -70,8 +70,14 @@ class FakeBatch:
             raise _lib.ClothSimError(1, "the cloths must share the PCG settings and precision")
         self.devs = list(devs)
 
+    tick_s = 0.001   # a tick takes time, so that control actions land mid-run
+    ticks = []       # (start, end, member names) of every tick
+
     def step(self, n):
-        time.sleep(0.001)  # a tick takes time, so that control actions land mid-run
+        t0 = time.perf_counter()
+        time.sleep(FakeBatch.tick_s)
+        with FakeCloth.lock:
+            FakeBatch.ticks.append((t0, time.perf_counter(), [d.cloth.name for d in self.devs]))
         out = []
         for d in self.devs:
             assert not d.closed
@@ -98,6 +104,8 @@ def fake(monkeypatch):
     FakeCloth.fail_create = set()
     FakeCloth.oom_once = set()
     FakeCloth.nonfinite_at = {}
+    FakeBatch.tick_s = 0.001
+    FakeBatch.ticks = []
     return FakeCloth
 
 
@@ -278,3 +286,52 @@ def test_poisson_requests_keep_the_long_side_on_the_fast_index():
         assert sc["ny"] >= sc["n"] and sc["side"] == 1.0
         nt = 2 * (sc["n"] - 1) * (sc["ny"] - 1)
         assert 700 <= nt <= 2.2e5
+
+
+def _first_tick(name):
+    return min((t for t in FakeBatch.ticks if name in t[2]), key=lambda t: t[0])
+
+
+@pytest.mark.parametrize("instances", [1, 2])
+def test_lane_instances_start_while_the_lane_steps(fake, instances):
+    """A request arriving while its lane is in a tick waits for the next tick
+    with one instance, and starts a second instance at once with two; every
+    result is still the stand-in's known state."""
+    FakeBatch.tick_s = 0.3
+    reqs = [_req(0, 8, 2, 0.0, "a"), _req(1, 8, 2, 0.1, "b")]
+    out = serve.Server(1 << 40, lanes=(), lane_instances=(instances,),
+                       lane_window=(0.0,)).run(reqs, realtime=True)
+    assert not out["failures"]
+    by = {r.rid: r for r in out["results"]}
+    for r in reqs:
+        _check(by[r.rid], r)
+    a0, b0 = _first_tick("a"), _first_tick("b")
+    if instances == 2:
+        assert b0[0] < a0[1] and b0[2] == ["b"] and a0[2] == ["a"]
+    else:
+        assert b0[0] >= a0[1]
+
+
+@pytest.mark.parametrize("window", [0.0, 0.25])
+def test_lane_window_gathers_the_first_tick(fake, window):
+    """An idle lane with a window waits that long before its first tick, so a
+    request arriving just after the first joins that tick."""
+    FakeBatch.tick_s = 0.05
+    reqs = [_req(0, 8, 1, 0.0, "a"), _req(1, 8, 1, 0.1, "b")]
+    out = serve.Server(1 << 40, lanes=(), lane_instances=(1,),
+                       lane_window=(window,)).run(reqs, realtime=True)
+    assert not out["failures"]
+    by = {r.rid: r for r in out["results"]}
+    for r in reqs:
+        _check(by[r.rid], r)
+    first = min(FakeBatch.ticks, key=lambda t: t[0])
+    assert sorted(first[2]) == (["a", "b"] if window > 0 else ["a"])
+
+
+def test_lane_settings_are_checked(fake):
+    with pytest.raises(ValueError):
+        serve.Server(1 << 40, lanes=(1000,), lane_instances=(1,))
+    with pytest.raises(ValueError):
+        serve.Server(1 << 40, lanes=(), lane_instances=(0,))
+    with pytest.raises(ValueError):
+        serve.Server(1 << 40, lanes=(), lane_window=(-1.0,))
