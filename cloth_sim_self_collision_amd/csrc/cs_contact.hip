@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 
 #include "cs_contact.hpp"
@@ -562,14 +563,16 @@ __global__ void k_impulse_ready(int g, const int* pred, int* npred, int* ready, 
 // of a longer round in the global list), and a contact releases its
 // successors before its impulse: they only run in the next round, after
 // the barrier that ends this one, and their counter atomics overlap the
-// impulse's loads.
+// impulse's loads.  lds_cap (at most kDrainLds) is the entries per LDS list;
+// a smaller value (the CLOTHSIM_DRAIN_LDS environment variable, a test hook)
+// sends more of every round through the global lists.
 constexpr int kDrainLds = 4096;
 
 __global__ __launch_bounds__(1024) void k_impulse_drain(const int* ce, const int* succ, int* npred,
                                                         int* ready0, int* ready1, const double* X,
                                                         double* U, double delta, double k,
                                                         double dt, double m, int ccd,
-                                                        int* counters) {
+                                                        int* counters, int lds_cap) {
   __shared__ int n_next;
   __shared__ int lists[2][kDrainLds];
   int n = counters[kReady];
@@ -581,12 +584,12 @@ __global__ __launch_bounds__(1024) void k_impulse_drain(const int* ce, const int
     if (threadIdx.x == 0) n_next = 0;
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-      const int i = (first || t >= kDrainLds) ? gcur[t] : lists[cur][t];
+      const int i = (first || t >= lds_cap) ? gcur[t] : lists[cur][t];
       for (int a = 0; a < 4; a++) {
         const int sc = succ[4 * i + a];
         if (sc >= 0 && atomicSub(&npred[sc], 1) == 1) {
           const int pos = atomicAdd(&n_next, 1);
-          if (pos < kDrainLds)
+          if (pos < lds_cap)
             lists[cur ^ 1][pos] = sc;
           else
             gnxt[pos] = sc;
@@ -989,9 +992,11 @@ int impulses(ContactState& cs, const DevCloth& d, const Params& P, hipStream_t s
   CS_HIP(hipMemsetAsync(cs.counters + kReady, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_impulse_ready, dim3(grid_for(g)), dim3(kTB), 0, s, g, cs.pred, cs.npred,
                      cs.ready[0], cs.counters);
+  int lds_cap = kDrainLds;
+  if (const char* e = getenv("CLOTHSIM_DRAIN_LDS")) lds_cap = std::min(std::max(atoi(e), 1), kDrainLds);
   hipLaunchKernelGGL(k_impulse_drain, dim3(1), dim3(1024), 0, s, cs.ce, cs.succ, cs.npred,
                      cs.ready[0], cs.ready[1], X, U, P.clearance, P.k_contact, P.dt, P.mass, ccd,
-                     cs.counters);
+                     cs.counters, lds_cap);
   CS_HIP(hipGetLastError());
   if (int st = read_counters(cs, s)) return st;
   if (cs.hcounters[kDoneCnt] != g) {

@@ -387,6 +387,27 @@ def test_trajectory_bitwise(sim_mod, oracle_mod, name, steps, variant, pcg):
             del x_before
 
 
+def test_impulse_rounds_through_the_global_lists(sim_mod, oracle_mod, monkeypatch):
+    """The impulse worklist keeps each round's list in LDS up to a capacity
+    and the rest of a wider round in global memory; with the capacity cut to
+    2 entries (CLOTHSIM_DRAIN_LDS) every round wider than that goes through
+    the global lists, and drape32's impulse steps stay bit-identical."""
+    monkeypatch.setenv("CLOTHSIM_DRAIN_LDS", "2")
+    cl = _scenes()["drape32"]
+    o = oracle_mod.Oracle(cl, "port")
+    x, v = cl.xyz.copy(), cl.uvw.copy()
+    impulses = 0
+    with sim_mod.DeviceCloth(cl) as d:
+        for k in range(70):
+            x, v, imp, it, conv = o.step(x, v)
+            st = d.step(1)
+            gx, gv = d.get_state()
+            impulses += int(imp)
+            assert st["pcg_iters"] == it and st["impulse_applied"] == imp, (k, st)
+            assert np.array_equal(gx, x) and np.array_equal(gv, v), k
+    assert impulses > 0
+
+
 def test_cpp_shim_step_loop_matches_reference(sim_mod, oracle_mod, tmp_path):
     """The reference's StepTime written against include/clothsim_ref.hpp
     (tests/shim/step_loop.cpp, a separate process) reproduces the oracle's
