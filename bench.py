@@ -271,10 +271,14 @@ def serve_leg(args, copy_gbs) -> dict:
 
 def fold_leg(steps: int, copy_gbs, maxit: int) -> dict:
     """The single 256x256 folding cloth, plain step, with the ILU sweep's
-    launch time from HIP events on the cloth's stream."""
-    from cloth_sim_self_collision_amd.sim import DeviceCloth
+    launch time from HIP events on the cloth's stream, priced also against
+    two ceilings of its one-workgroup shape: one CU's read bandwidth
+    (cs_read_probe) and the issue time of the wave-load instructions one
+    launch issues (cs_sweep_shape x cs_issue_probe)."""
+    from cloth_sim_self_collision_amd.sim import DeviceCloth, issue_probe, read_probe
     cl = scene.fold_sweep(256, pcg_maxit=maxit)
     with DeviceCloth(cl) as d:
+        shape = d.sweep_shape()
         d.step(1)
         d.profile(True, sweep_only=True)
         lat, its = [], []
@@ -289,6 +293,10 @@ def fold_leg(steps: int, copy_gbs, maxit: int) -> dict:
         b = algorithmic_bytes("ilu_sweep", d.np_, d.nnz)
         np_ = d.np_
     ach = b / (ms / n * 1e-3) / 1e9 if n else None
+    avg_us = ms / n * 1e3 if n else None
+    one_cu = read_probe(1 << 23, 5, 1)
+    issue_ns = issue_probe(shape["waves"]) if shape["quad"] else None
+    floor_us = shape["wave_loads"] * issue_ns * 1e-3 if issue_ns else None
     print(f"[bench] fold256: {steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
     return {"workload": "fold256", "np": np_, "nt": cl.nt, "steps": steps,
             "vertex_steps_per_s": np_ * steps / wall, "ms_per_step": wall / steps * 1e3,
@@ -296,9 +304,14 @@ def fold_leg(steps: int, copy_gbs, maxit: int) -> dict:
             "p99_step_ms": float(np.percentile(np.array(lat) * 1e3, 99)), "pcg_iters": its,
             "roofline": {"kernel": "ilu_sweep", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS if ach else None, "bytes_per_launch": b,
-                         "avg_launch_us": ms / n * 1e3 if n else None, "launches": n,
+                         "avg_launch_us": avg_us, "launches": n,
                          "traffic": TRAFFIC_RATIO[("fold256", "ilu_sweep")] * b,
-                         "frac_of_stream_copy": ach / copy_gbs if (ach and copy_gbs) else None}}
+                         "frac_of_stream_copy": ach / copy_gbs if (ach and copy_gbs) else None,
+                         "one_cu_read_gbs": one_cu,
+                         "frac_of_one_cu_read": ach / one_cu if (ach and one_cu) else None,
+                         "sweep_shape": shape, "ns_per_wave_load": issue_ns,
+                         "issue_floor_us": floor_us,
+                         "frac_of_issue_floor": floor_us / avg_us if (floor_us and avg_us) else None}}
 
 
 def stack_leg(steps: int, maxit: int) -> dict:
