@@ -1,0 +1,185 @@
+"""CPU emulation of the contact traversal (cloth_sim_self_collision_amd/
+csrc/cs_contact.hip: k_traverse) against the per-leaf walk it replaces.
+
+Per leaf L the walk is a pre-order scan from the root that jumps to the end
+of a subtree when the subtree holds no leaf after L or its box misses L's
+box, and records every leaf it reaches after L.  k_traverse runs one wave
+per 64 consecutive leaves in pre-order and walks the union of its lanes'
+walks: at rank r a lane takes part iff r is at or past the end of the last
+subtree it skipped; the wave moves to r + 1 if any lane descends, else to
+the end of r's subtree.  The emulation checks, lane by lane:
+
+  - every lane tests exactly the ranks its own walk tests, so the pairs are
+    the same set (also with NaN coordinates, for which the overlap test of
+    aabb.h:73-84 answers true);
+  - the rank stays inside the tree and strictly increases (the wave loop
+    ends);
+  - the pairs leave through the LDS buffer as the kernel writes them: never
+    more than its capacity, each flushed once.
+
+Trees are the reference's top-down BVH of small meshes (oracle setup) with
+boxes from the oracle's proximity refit, and random boxes on the same trees
+(nested, as a refit makes them, or not nested at all).
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle"))
+
+from cloth_sim_self_collision_amd import scene  # noqa: E402
+
+K_WAVE = 64
+K_TRAV_BUF = 512  # kTravBuf
+
+
+def overlap(a, b):
+    # aabb.h:73-84 as cs_math.hpp's overlap(): NaN compares false, so it overlaps
+    return not (a[1] < b[0] or a[0] > b[1] or a[3] < b[2] or a[2] > b[3] or a[5] < b[4]
+                or a[4] > b[5])
+
+
+def preorder(nodes):
+    n = len(nodes)
+    pre = np.zeros(n, int)
+    by_pre = np.zeros(n, int)
+    st, r = [0], 0
+    while st:
+        v = st.pop()
+        pre[v] = r
+        by_pre[r] = v
+        r += 1
+        c0, c1 = nodes[v][1], nodes[v][2]
+        if c1 != -1:
+            st.append(c1)
+            st.append(c0)
+    size = np.ones(n, int)
+    for k in range(n - 1, -1, -1):
+        v = by_pre[k]
+        if nodes[v][2] != -1:
+            size[v] = 1 + size[nodes[v][1]] + size[nodes[v][2]]
+    rend = np.array([pre[by_pre[r]] + size[by_pre[r]] for r in range(n)])
+    return pre, by_pre, rend
+
+
+def walk_one(rl, rend, rbb, visits):
+    """The per-leaf walk: the ranks it tests and the leaves it records."""
+    bl = rbb[rl]
+    out, r = [], 0
+    while r < len(rend):
+        visits.append(r)
+        end = rend[r]
+        if end <= rl + 1 or not overlap(bl, rbb[r]):
+            r = end
+            continue
+        if end == r + 1 and r > rl:
+            out.append(r)
+        r += 1
+    return out
+
+
+def walk_wave(rls, rend, rbb, lane_visits):
+    """k_traverse's loop for one wave (lanes = rls)."""
+    n = len(rend)
+    skip = [0] * len(rls)
+    emitted, buf, flushed = [], [], []
+    r, last = 0, -1
+    while r < n:
+        assert 0 <= r < n and r > last  # inside the tree, strictly increasing
+        last = r
+        end = rend[r]
+        down_any = False
+        hits = []
+        for i, rl in enumerate(rls):
+            alive = r >= skip[i]
+            down = alive and end > rl + 1 and overlap(rbb[rl], rbb[r])
+            if alive:
+                lane_visits[i].append(r)
+                if not down:
+                    skip[i] = end
+            if down and end == r + 1 and r > rl:
+                hits.append((rl, r))
+            down_any = down_any or down
+        if hits:
+            if len(buf) + len(hits) > K_TRAV_BUF:
+                flushed.extend(buf)
+                buf = []
+            buf.extend(hits)
+            assert len(buf) <= K_TRAV_BUF
+        r = r + 1 if down_any else end
+    flushed.extend(buf)
+    return flushed
+
+
+def check_tree(nodes, bb):
+    pre, by_pre, rend = preorder(nodes)
+    rbb = [bb[by_pre[r]] for r in range(len(nodes))]
+    leaves = [r for r in range(len(nodes)) if rend[r] == r + 1]
+    want, got = set(), set()
+    for w0 in range(0, len(leaves), K_WAVE):
+        rls = leaves[w0:w0 + K_WAVE]
+        lane_visits = [[] for _ in rls]
+        pairs = walk_wave(rls, rend, rbb, lane_visits)
+        assert len(pairs) == len(set(pairs))
+        got.update(pairs)
+        for i, rl in enumerate(rls):
+            own = []
+            for m in walk_one(rl, rend, rbb, own):
+                want.add((rl, m))
+            assert lane_visits[i] == own  # the lane tests exactly its own walk's ranks
+    assert got == want
+    return len(want)
+
+
+def _tree(n, seed):
+    import pyoracle
+    if not pyoracle.available("port"):
+        pytest.skip("oracle/liboracle.so is not built")
+    cl = scene.drape_over_sphere(n)
+    rng = np.random.default_rng(seed)
+    cl.xyz = cl.xyz + rng.normal(0.0, 0.01, cl.xyz.shape)  # a rumpled sheet
+    o = pyoracle.Oracle(cl, "port")
+    nodes = o.bvh()
+    bb = o.refit_prx(cl.xyz, cl.clearance * 8)
+    o.close()
+    return nodes, np.asarray(bb)
+
+
+@pytest.mark.parametrize("n,seed", [(6, 0), (12, 1), (20, 2)])
+def test_wave_walk_equals_leaf_walks_on_refit_boxes(n, seed):
+    nodes, bb = _tree(n, seed)
+    assert check_tree(nodes, bb) > 0
+
+
+@pytest.mark.parametrize("nested", [True, False])
+def test_wave_walk_equals_leaf_walks_on_random_boxes(nested):
+    nodes, _ = _tree(16, 3)
+    rng = np.random.default_rng(7)
+    n = len(nodes)
+    bb = np.zeros((n, 6))
+    lo = rng.uniform(0, 1, (n, 3))
+    hi = lo + rng.uniform(0, 0.3, (n, 3))
+    bb[:, 0::2], bb[:, 1::2] = lo, hi
+    if nested:  # parents hold their children, as after a refit
+        pre, by_pre, rend = preorder(nodes)
+        for k in range(n - 1, -1, -1):
+            v = by_pre[k]
+            c0, c1 = nodes[v][1], nodes[v][2]
+            if c1 != -1:
+                bb[v, 0::2] = np.minimum(bb[c0, 0::2], bb[c1, 0::2])
+                bb[v, 1::2] = np.maximum(bb[c0, 1::2], bb[c1, 1::2])
+    check_tree(nodes, bb)
+
+
+def test_wave_walk_with_nan_boxes():
+    """Leaves and inner nodes with NaN coordinates: overlap() answers true
+    for them, and the wave still tests each lane's own ranks only."""
+    nodes, bb = _tree(14, 4)
+    bb = bb.copy()
+    rng = np.random.default_rng(11)
+    for v in rng.choice(len(nodes), size=len(nodes) // 10, replace=False):
+        bb[v, rng.integers(0, 6)] = math.nan
+    check_tree(nodes, bb)
