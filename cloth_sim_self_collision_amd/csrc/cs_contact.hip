@@ -255,7 +255,7 @@ __device__ __forceinline__ void trav_flush(const int2* buf, int fill, int* __res
   __syncthreads();  // read out before the buffer is reused
 }
 
-__global__ __launch_bounds__(64) void k_traverse(int nnode, int nt,
+__global__ __launch_bounds__(64) void k_traverse(int nnode, int nt, int per,
                                                  const int* __restrict__ leaf_rank,
                                                  const int* __restrict__ rend,
                                                  const double* __restrict__ rbb,
@@ -263,10 +263,12 @@ __global__ __launch_bounds__(64) void k_traverse(int nnode, int nt,
                                                  int* __restrict__ counters, int cap,
                                                  int2* __restrict__ cand) {
   __shared__ int2 buf[kTravBuf];
-  const int k = (int)(blockIdx.x * 64 + threadIdx.x);
+  // per (1..64) leaves per wave: fewer leaves make shorter union walks and
+  // more waves, which pays where the tree is small and few waves would run
+  const int k = (int)(blockIdx.x * per + threadIdx.x);
   int rl = 0, L = -1, skip = INT_MAX;  // lanes past the last leaf never take part
   Box bl{};
-  if (k < nt) {
+  if ((int)threadIdx.x < per && k < nt) {
     rl = leaf_rank[k];
     L = by_pre[rl];
     bl = load_box(rbb, rl);
@@ -915,7 +917,11 @@ int query(ContactState& cs, const DevCloth& d, hipStream_t s, const double* X, c
   int ncand = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
     CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kQueryCounters, s));
-    hipLaunchKernelGGL(k_traverse, dim3(grid_for(d.nt, 64)), dim3(64), 0, s, d.nnode, d.nt,
+    // leaves per wave: 64, halved (down to 16) while the launch would have
+    // fewer than 2048 waves, so that a small tree's walks spread over more waves
+    int per = 64;
+    while (per > 16 && (d.nt + per - 1) / per < 2048) per /= 2;
+    hipLaunchKernelGGL(k_traverse, dim3((d.nt + per - 1) / per), dim3(64), 0, s, d.nnode, d.nt, per,
                        d.leaf_rank, d.rend, d.rbb, d.by_pre, cs.counters, cs.cand_cap,
                        reinterpret_cast<int2*>(cs.cand));
     CS_HIP(hipGetLastError());

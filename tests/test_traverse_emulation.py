@@ -4,8 +4,8 @@ csrc/cs_contact.hip: k_traverse) against the per-leaf walk it replaces.
 Per leaf L the walk is a pre-order scan from the root that jumps to the end
 of a subtree when the subtree holds no leaf after L or its box misses L's
 box, and records every leaf it reaches after L.  k_traverse runs one wave
-per 64 consecutive leaves in pre-order and walks the union of its lanes'
-walks: at rank r a lane takes part iff r is at or past the end of the last
+per 16, 32 or 64 consecutive leaves in pre-order (fewer for smaller trees)
+and walks the union of its lanes' walks: at rank r a lane takes part iff r is at or past the end of the last
 subtree it skipped; the wave moves to r + 1 if any lane descends, else to
 the end of r's subtree.  The emulation checks, lane by lane:
 
@@ -114,13 +114,13 @@ def walk_wave(rls, rend, rbb, lane_visits):
     return flushed
 
 
-def check_tree(nodes, bb):
+def check_tree(nodes, bb, per=K_WAVE):
     pre, by_pre, rend = preorder(nodes)
     rbb = [bb[by_pre[r]] for r in range(len(nodes))]
     leaves = [r for r in range(len(nodes)) if rend[r] == r + 1]
     want, got = set(), set()
-    for w0 in range(0, len(leaves), K_WAVE):
-        rls = leaves[w0:w0 + K_WAVE]
+    for w0 in range(0, len(leaves), per):
+        rls = leaves[w0:w0 + per]
         lane_visits = [[] for _ in rls]
         pairs = walk_wave(rls, rend, rbb, lane_visits)
         assert len(pairs) == len(set(pairs))
@@ -148,10 +148,11 @@ def _tree(n, seed):
     return nodes, np.asarray(bb)
 
 
+@pytest.mark.parametrize("per", [16, 32, 64])
 @pytest.mark.parametrize("n,seed", [(6, 0), (12, 1), (20, 2)])
-def test_wave_walk_equals_leaf_walks_on_refit_boxes(n, seed):
+def test_wave_walk_equals_leaf_walks_on_refit_boxes(n, seed, per):
     nodes, bb = _tree(n, seed)
-    assert check_tree(nodes, bb) > 0
+    assert check_tree(nodes, bb, per) > 0
 
 
 @pytest.mark.parametrize("nested", [True, False])
