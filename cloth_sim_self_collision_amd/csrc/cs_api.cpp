@@ -5,7 +5,10 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -24,6 +27,77 @@ thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
 
+namespace {
+std::mutex g_mem_mu;
+std::vector<signed char> g_pool;  // per device: 0 not yet asked, 1 pool, -1 plain
+std::map<size_t, std::vector<void*>> g_pinned_free;  // free pinned blocks by size
+std::map<void*, size_t> g_pinned_size;               // every pinned block's size
+
+bool use_pool() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return false;
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  if ((int)g_pool.size() <= dev) g_pool.resize(dev + 1, 0);
+  if (g_pool[dev] == 0) {
+    g_pool[dev] = -1;
+    int ok = 0;
+    hipMemPool_t pool = nullptr;
+    uint64_t keep = UINT64_MAX;  // freed memory stays in the pool
+    if (hipDeviceGetAttribute(&ok, hipDeviceAttributeMemoryPoolsSupported, dev) == hipSuccess &&
+        ok && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess &&
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess)
+      g_pool[dev] = 1;
+  }
+  return g_pool[dev] == 1;
+}
+}  // namespace
+
+int dev_alloc(void** p, size_t bytes, hipStream_t s) {
+  *p = nullptr;
+  if (use_pool())
+    CS_HIP(hipMallocAsync(p, bytes, s));
+  else
+    CS_HIP(hipMalloc(p, bytes));
+  return kOk;
+}
+
+void dev_free(void* p, hipStream_t s) {
+  if (!p) return;
+  if (use_pool())
+    (void)hipFreeAsync(p, s);
+  else
+    (void)hipFree(p);
+}
+
+int pinned_alloc(void** p, size_t bytes) {
+  size_t sz = 256;
+  while (sz < bytes) sz <<= 1;
+  void* q = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto& fl = g_pinned_free[sz];
+    if (!fl.empty()) {
+      q = fl.back();
+      fl.pop_back();
+    }
+  }
+  if (!q) {
+    CS_HIP(hipHostMalloc(&q, sz));
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    g_pinned_size[q] = sz;
+  }
+  std::memset(q, 0, sz);
+  *p = q;
+  return kOk;
+}
+
+void pinned_free(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  auto it = g_pinned_size.find(p);
+  if (it != g_pinned_size.end()) g_pinned_free[it->second].push_back(p);
+}
+
 // One device allocation carved into aligned sub-arrays.
 class Arena {
  public:
@@ -36,14 +110,14 @@ class Arena {
     total_ += ((count * sizeof(T) + 255) / 256) * 256;
     if (count == 0) total_ += 256;
   }
-  int commit() {
+  int commit(hipStream_t s) {
     if (total_ == 0) return kOk;
-    CS_HIP(hipMalloc(&base_, total_));
+    if (int st = dev_alloc(&base_, total_, s)) return st;
     for (auto& it : items_) *it.slot = (char*)base_ + it.offset;
     return kOk;
   }
-  void release() {
-    if (base_) (void)hipFree(base_);
+  void release(hipStream_t s) {
+    dev_free(base_, s);
     base_ = nullptr;
   }
   size_t bytes() const { return total_; }
@@ -240,7 +314,7 @@ int build(Cloth* c, const cs_mesh* m) {
   A.reserve(&qbc, S.qb_comp.size());
   double* host_pd;
   A.reserve(&host_pd, 4 * (size_t)np);
-  if (int st = A.commit()) return st;
+  if (int st = A.commit(c->stream)) return st;
   d.tri = tri;
   d.quad = quad;
   d.bc = bc;
@@ -450,8 +524,8 @@ int cs_cloth_create(const cs_mesh* mesh, const cs_params* params, cs_cloth** out
       hipStreamCreateWithFlags(&c->aux.stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->aux.updated, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->aux.tested, hipEventDisableTiming) != hipSuccess ||
-      hipHostMalloc((void**)&c->hflags, sizeof(int) * kNumFlag) != hipSuccess ||
-      hipHostMalloc((void**)&c->hscal, sizeof(double) * kNumScal) != hipSuccess) {
+      pinned_alloc((void**)&c->hflags, sizeof(int) * kNumFlag) != kOk ||
+      pinned_alloc((void**)&c->hscal, sizeof(double) * kNumScal) != kOk) {
     set_error("stream or pinned buffer creation failed");
     st = kHipError;
   }
@@ -467,18 +541,18 @@ int cs_cloth_create(const cs_mesh* mesh, const cs_params* params, cs_cloth** out
 int cs_cloth_destroy(cs_cloth* p) {
   if (!p) return kOk;
   Cloth* c = H(p);
+  // every stream that used the cloth's memory is idle before it is released
+  // (the device memory after the work queued on the cloth's stream)
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  contact_release(c->contact);
+  if (c->aux.stream) (void)hipStreamSynchronize(c->aux.stream);
+  contact_release(c->contact, c->stream);
   c->prof.release();
-  if (c->sweep_verts) (void)hipFree(c->sweep_verts);
-  c->arena.release();
-  if (c->hflags) (void)hipHostFree(c->hflags);
-  if (c->hscal) (void)hipHostFree(c->hscal);
+  dev_free(c->sweep_verts, c->stream);
+  c->arena.release(c->stream);
+  pinned_free(c->hflags);
+  pinned_free(c->hscal);
   if (c->stream) (void)hipStreamDestroy(c->stream);
-  if (c->aux.stream) {
-    (void)hipStreamSynchronize(c->aux.stream);
-    (void)hipStreamDestroy(c->aux.stream);
-  }
+  if (c->aux.stream) (void)hipStreamDestroy(c->aux.stream);
   if (c->aux.updated) (void)hipEventDestroy(c->aux.updated);
   if (c->aux.tested) (void)hipEventDestroy(c->aux.tested);
   delete c;
@@ -659,12 +733,15 @@ int cs_set_sweep(cs_cloth* p, int n, const int* verts, const double* delta, doub
       set_error("sweep vertex out of range");
       return kBadArg;
     }
-  if (c->sweep_verts) (void)hipFree(c->sweep_verts);
+  CS_HIP(hipStreamSynchronize(c->stream));  // no step still reads the old list
+  dev_free(c->sweep_verts, c->stream);
   c->sweep_verts = nullptr;
   c->n_sweep = 0;
   if (n > 0) {
-    CS_HIP(hipMalloc((void**)&c->sweep_verts, sizeof(int) * n));
-    CS_HIP(hipMemcpy(c->sweep_verts, verts, sizeof(int) * n, hipMemcpyHostToDevice));
+    if (int st = dev_alloc((void**)&c->sweep_verts, sizeof(int) * n, c->stream)) return st;
+    CS_HIP(hipMemcpyAsync(c->sweep_verts, verts, sizeof(int) * n, hipMemcpyHostToDevice,
+                          c->stream));
+    CS_HIP(hipStreamSynchronize(c->stream));
   }
   c->n_sweep = n;
   for (int k = 0; k < 3; k++) c->sweep_delta[k] = n > 0 ? delta[k] : 0.0;
@@ -1041,10 +1118,10 @@ void batch_free(Batch* b) {
   }
   if (b->aux.updated) (void)hipEventDestroy(b->aux.updated);
   if (b->aux.tested) (void)hipEventDestroy(b->aux.tested);
-  if (b->dtab) (void)hipFree(b->dtab);
-  if (b->ptab) (void)hipFree(b->ptab);
-  if (b->dgather) (void)hipFree(b->dgather);
-  if (b->hgather) (void)hipHostFree(b->hgather);
+  dev_free(b->dtab, b->stream);
+  dev_free(b->ptab, b->stream);
+  dev_free(b->dgather, b->stream);
+  pinned_free(b->hgather);
   if (b->internal_done) (void)hipEventDestroy(b->internal_done);
   for (hipEvent_t e : b->cloth_done) (void)hipEventDestroy(e);
   b->prof.release();
@@ -1122,10 +1199,11 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
       hipStreamCreateWithFlags(&b->aux.stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->aux.updated, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->aux.tested, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc((void**)&b->dtab, sizeof(DevCloth) * n) != hipSuccess ||
-      hipMalloc((void**)&b->ptab, sizeof(Params) * n) != hipSuccess ||
-      hipMalloc((void**)&b->dgather, sizeof(int) * n) != hipSuccess ||
-      hipHostMalloc((void**)&b->hgather, sizeof(int) * n) != hipSuccess ||
+      dev_alloc((void**)&b->dtab, sizeof(DevCloth) * n, b->stream) != kOk ||
+      dev_alloc((void**)&b->ptab, sizeof(Params) * n, b->stream) != kOk ||
+      dev_alloc((void**)&b->dgather, sizeof(int) * n, b->stream) != kOk ||
+      pinned_alloc((void**)&b->hgather, sizeof(int) * n) != kOk ||
+      hipStreamSynchronize(b->stream) != hipSuccess ||
       hipEventCreateWithFlags(&b->internal_done, hipEventDisableTiming) != hipSuccess) {
     batch_free(b);
     set_error("cs_batch_create: device allocation failed");

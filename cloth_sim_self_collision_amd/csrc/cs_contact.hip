@@ -816,14 +816,15 @@ __global__ void k_velocity_update(DevCloth c, Params P) {
 
 namespace {
 
+// device buffers of the contact state: allocated and released in the order
+// of the cloth's stream s, without synchronising the device
 template <typename T>
-int dalloc(T** p, size_t n) {
-  CS_HIP(hipMalloc((void**)p, sizeof(T) * (n ? n : 1)));
-  return kOk;
+int dalloc(T** p, size_t n, hipStream_t s) {
+  return dev_alloc((void**)p, sizeof(T) * (n ? n : 1), s);
 }
 template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree(p);
+void dfree(T*& p, hipStream_t s) {
+  dev_free(p, s);
   p = nullptr;
 }
 
@@ -831,41 +832,41 @@ int grow_records(ContactState& cs, int need, hipStream_t s) {
   if (need <= cs.cap) return kOk;
   CS_HIP(hipStreamSynchronize(s));
   int cap = std::max(need + need / 2, 1 << 16);
-  dfree(cs.rkey);
-  dfree(cs.rinfo);
-  dfree(cs.rpair);
-  dfree(cs.skey[0]);
-  dfree(cs.skey[1]);
-  dfree(cs.sidx[0]);
-  dfree(cs.sidx[1]);
-  dfree(cs.head);
-  dfree(cs.gpos);
-  dfree(cs.ce);
-  dfree(cs.pred);
-  dfree(cs.succ);
-  dfree(cs.npred);
-  dfree(cs.ready[0]);
-  dfree(cs.ready[1]);
-  dfree(cs.inc);
-  if (cs.tmp) (void)hipFree(cs.tmp);
+  dfree(cs.rkey, s);
+  dfree(cs.rinfo, s);
+  dfree(cs.rpair, s);
+  dfree(cs.skey[0], s);
+  dfree(cs.skey[1], s);
+  dfree(cs.sidx[0], s);
+  dfree(cs.sidx[1], s);
+  dfree(cs.head, s);
+  dfree(cs.gpos, s);
+  dfree(cs.ce, s);
+  dfree(cs.pred, s);
+  dfree(cs.succ, s);
+  dfree(cs.npred, s);
+  dfree(cs.ready[0], s);
+  dfree(cs.ready[1], s);
+  dfree(cs.inc, s);
+  dev_free(cs.tmp, s);
   cs.tmp = nullptr;
   cs.cap = 0;
-  if (int st = dalloc(&cs.rkey, 4L * cap)) return st;
-  if (int st = dalloc(&cs.rinfo, cap)) return st;
-  if (int st = dalloc(&cs.rpair, 2L * cap)) return st;
-  if (int st = dalloc(&cs.skey[0], cap)) return st;
-  if (int st = dalloc(&cs.skey[1], cap)) return st;
-  if (int st = dalloc(&cs.sidx[0], cap)) return st;
-  if (int st = dalloc(&cs.sidx[1], cap)) return st;
-  if (int st = dalloc(&cs.head, cap)) return st;
-  if (int st = dalloc(&cs.gpos, cap)) return st;
-  if (int st = dalloc(&cs.ce, 5L * cap)) return st;
-  if (int st = dalloc(&cs.pred, 4L * cap)) return st;
-  if (int st = dalloc(&cs.succ, 4L * cap)) return st;
-  if (int st = dalloc(&cs.npred, cap)) return st;
-  if (int st = dalloc(&cs.ready[0], cap)) return st;
-  if (int st = dalloc(&cs.ready[1], cap)) return st;
-  if (int st = dalloc(&cs.inc, 4L * cap)) return st;
+  if (int st = dalloc(&cs.rkey, 4L * cap, s)) return st;
+  if (int st = dalloc(&cs.rinfo, cap, s)) return st;
+  if (int st = dalloc(&cs.rpair, 2L * cap, s)) return st;
+  if (int st = dalloc(&cs.skey[0], cap, s)) return st;
+  if (int st = dalloc(&cs.skey[1], cap, s)) return st;
+  if (int st = dalloc(&cs.sidx[0], cap, s)) return st;
+  if (int st = dalloc(&cs.sidx[1], cap, s)) return st;
+  if (int st = dalloc(&cs.head, cap, s)) return st;
+  if (int st = dalloc(&cs.gpos, cap, s)) return st;
+  if (int st = dalloc(&cs.ce, 5L * cap, s)) return st;
+  if (int st = dalloc(&cs.pred, 4L * cap, s)) return st;
+  if (int st = dalloc(&cs.succ, 4L * cap, s)) return st;
+  if (int st = dalloc(&cs.npred, cap, s)) return st;
+  if (int st = dalloc(&cs.ready[0], cap, s)) return st;
+  if (int st = dalloc(&cs.ready[1], cap, s)) return st;
+  if (int st = dalloc(&cs.inc, 4L * cap, s)) return st;
   // temporary storage for the largest sort / scan
   size_t a = 0, b = 0, c2 = 0;
   CS_HIP(rocprim::radix_sort_pairs(nullptr, a, cs.skey[0], cs.skey[1], cs.sidx[0], cs.sidx[1],
@@ -875,7 +876,7 @@ int grow_records(ContactState& cs, int need, hipStream_t s) {
   CS_HIP(rocprim::exclusive_scan(nullptr, c2, cs.vcount, cs.vstart, 0, (size_t)cs.np + 1,
                                  rocprim::plus<int>(), s));
   cs.tmp_bytes = std::max(a, std::max(b, c2)) + 256;
-  CS_HIP(hipMalloc(&cs.tmp, cs.tmp_bytes));
+  if (int st = dev_alloc(&cs.tmp, cs.tmp_bytes, s)) return st;
   cs.cap = cap;
   return kOk;
 }
@@ -884,9 +885,9 @@ int grow_cand(ContactState& cs, int need, hipStream_t s) {
   if (need <= cs.cand_cap) return kOk;
   CS_HIP(hipStreamSynchronize(s));
   const int cap = std::max(need + need / 2, 1 << 16);
-  dfree(cs.cand);
+  dfree(cs.cand, s);
   cs.cand_cap = 0;
-  if (int st = dalloc(&cs.cand, 2L * cap)) return st;
+  if (int st = dalloc(&cs.cand, 2L * cap, s)) return st;
   cs.cand_cap = cap;
   return kOk;
 }
@@ -1025,57 +1026,57 @@ int contact_init(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStr
     set_error("mesh too large for the contact keys");
     return kBadArg;
   }
-  if (int st = dalloc(&cs.vcount, S.np + 1)) return st;
-  if (int st = dalloc(&cs.vstart, S.np + 1)) return st;
-  if (int st = dalloc(&cs.vfill, S.np + 1)) return st;
-  if (int st = dalloc(&cs.counters, kNumCounters)) return st;
+  if (int st = dalloc(&cs.vcount, S.np + 1, s)) return st;
+  if (int st = dalloc(&cs.vstart, S.np + 1, s)) return st;
+  if (int st = dalloc(&cs.vfill, S.np + 1, s)) return st;
+  if (int st = dalloc(&cs.counters, kNumCounters, s)) return st;
   cs.refit_lvl = S.refit_lvl_ptr;
-  if (int st = dalloc(&cs.zpar, S.np)) return st;
-  if (int st = dalloc(&cs.zroot, S.np)) return st;
-  if (int st = dalloc(&cs.zmemb, S.np)) return st;
-  if (int st = dalloc(&cs.zroots, S.np)) return st;
-  if (int st = dalloc(&cs.u0, 3L * S.np)) return st;
-  if (int st = dalloc(&cs.qx, 3L * S.np)) return st;
-  if (int st = dalloc(&cs.qv, 3L * S.np)) return st;
-  CS_HIP(hipHostMalloc((void**)&cs.hcounters, sizeof(int) * kNumCounters));
+  if (int st = dalloc(&cs.zpar, S.np, s)) return st;
+  if (int st = dalloc(&cs.zroot, S.np, s)) return st;
+  if (int st = dalloc(&cs.zmemb, S.np, s)) return st;
+  if (int st = dalloc(&cs.zroots, S.np, s)) return st;
+  if (int st = dalloc(&cs.u0, 3L * S.np, s)) return st;
+  if (int st = dalloc(&cs.qx, 3L * S.np, s)) return st;
+  if (int st = dalloc(&cs.qv, 3L * S.np, s)) return st;
+  if (int st = pinned_alloc((void**)&cs.hcounters, sizeof(int) * kNumCounters)) return st;
   CS_HIP(hipMemsetAsync(cs.counters, 0, sizeof(int) * kNumCounters, s));
   if (int st = grow_cand(cs, std::max(8 * S.nt, 1 << 16), s)) return st;
   return grow_records(cs, std::max(8 * S.nt, 1 << 16), s);
 }
 
-void contact_release(ContactState& cs) {
-  dfree(cs.cand);
+void contact_release(ContactState& cs, hipStream_t s) {
+  dfree(cs.cand, s);
   cs.cand_cap = 0;
-  dfree(cs.rkey);
-  dfree(cs.rinfo);
-  dfree(cs.rpair);
-  dfree(cs.skey[0]);
-  dfree(cs.skey[1]);
-  dfree(cs.sidx[0]);
-  dfree(cs.sidx[1]);
-  dfree(cs.head);
-  dfree(cs.gpos);
-  dfree(cs.ce);
-  dfree(cs.pred);
-  dfree(cs.succ);
-  dfree(cs.npred);
-  dfree(cs.ready[0]);
-  dfree(cs.ready[1]);
-  dfree(cs.inc);
-  dfree(cs.vcount);
-  dfree(cs.vstart);
-  dfree(cs.vfill);
-  dfree(cs.counters);
-  dfree(cs.zpar);
-  dfree(cs.zroot);
-  dfree(cs.zmemb);
-  dfree(cs.zroots);
-  dfree(cs.u0);
-  dfree(cs.qx);
-  dfree(cs.qv);
-  if (cs.tmp) (void)hipFree(cs.tmp);
+  dfree(cs.rkey, s);
+  dfree(cs.rinfo, s);
+  dfree(cs.rpair, s);
+  dfree(cs.skey[0], s);
+  dfree(cs.skey[1], s);
+  dfree(cs.sidx[0], s);
+  dfree(cs.sidx[1], s);
+  dfree(cs.head, s);
+  dfree(cs.gpos, s);
+  dfree(cs.ce, s);
+  dfree(cs.pred, s);
+  dfree(cs.succ, s);
+  dfree(cs.npred, s);
+  dfree(cs.ready[0], s);
+  dfree(cs.ready[1], s);
+  dfree(cs.inc, s);
+  dfree(cs.vcount, s);
+  dfree(cs.vstart, s);
+  dfree(cs.vfill, s);
+  dfree(cs.counters, s);
+  dfree(cs.zpar, s);
+  dfree(cs.zroot, s);
+  dfree(cs.zmemb, s);
+  dfree(cs.zroots, s);
+  dfree(cs.u0, s);
+  dfree(cs.qx, s);
+  dfree(cs.qv, s);
+  dev_free(cs.tmp, s);
   cs.tmp = nullptr;
-  if (cs.hcounters) (void)hipHostFree(cs.hcounters);
+  pinned_free(cs.hcounters);
   cs.hcounters = nullptr;
 }
 

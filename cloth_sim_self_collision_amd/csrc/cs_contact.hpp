@@ -62,7 +62,8 @@ struct ContactState {
 };
 
 int contact_init(ContactState& cs, const MeshSetup& S, const DevCloth& d, hipStream_t s);
-void contact_release(ContactState& cs);
+// after every stream that used the buffers is idle (they are freed on s)
+void contact_release(ContactState& cs, hipStream_t s);
 // full contact pass on (d.x1, d.v); stats may be NULL
 int contact_resolve(ContactState& cs, const MeshSetup& S, const DevCloth& d, const Params& P,
                     hipStream_t s, int* applied, cs_step_stats* stats, Prof* prof = nullptr);

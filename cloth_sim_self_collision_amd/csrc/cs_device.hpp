@@ -28,6 +28,22 @@ enum Status : int {
 void set_error(const std::string& msg);
 const char* last_error();
 
+// Memory that is released without synchronising the device.  hipFree and
+// hipHostFree wait for every stream of the device, so releasing a cloth, a
+// batch or a contact buffer while another stream runs a long kernel (another
+// batch's solve) would wait for that kernel.  Device buffers come from the
+// device's stream-ordered pool, which keeps freed memory for later
+// allocations (plain hipMalloc/hipFree where the device has no pools); small
+// pinned host buffers come from a free list that is never handed back.
+// dev_alloc: the memory is ready for work on s, and for any stream after s
+// has been synchronised.  dev_free: after the work queued on s; work on
+// other streams that used the memory must have completed.
+int dev_alloc(void** p, size_t bytes, hipStream_t s);
+void dev_free(void* p, hipStream_t s);
+// zero-filled pinned host memory; pinned_free after its last use completed
+int pinned_alloc(void** p, size_t bytes);
+void pinned_free(void* p);
+
 #define CS_HIP(call)                                                                      \
   do {                                                                                    \
     hipError_t e_ = (call);                                                               \
