@@ -1061,6 +1061,7 @@ struct Batch {
   std::vector<Cloth*> cl;
   DevCloth* dtab = nullptr;  // device copies of the cloths' DevCloth / Params
   Params* ptab = nullptr;
+  int2* rowmap = nullptr;  // the batched SpMV's (cloth, first row) per block
   int* dgather = nullptr;
   int* hgather = nullptr;  // pinned
   hipStream_t stream = nullptr;
@@ -1120,6 +1121,7 @@ void batch_free(Batch* b) {
   if (b->aux.tested) (void)hipEventDestroy(b->aux.tested);
   dev_free(b->dtab, b->stream);
   dev_free(b->ptab, b->stream);
+  dev_free(b->rowmap, b->stream);
   dev_free(b->dgather, b->stream);
   pinned_free(b->hgather);
   if (b->internal_done) (void)hipEventDestroy(b->internal_done);
@@ -1216,8 +1218,22 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
       set_error("cs_batch_create: event creation failed");
       return kHipError;
     }
+  // the SpMV's blocks over the rows that exist, kSpmvRows rows each
+  std::vector<int2> rowmap;
+  for (int i = 0; i < n; i++)
+    for (int r0 = 0; r0 < b->cl[i]->d.np; r0 += kSpmvRows) rowmap.push_back(make_int2(i, r0));
+  if (dev_alloc((void**)&b->rowmap, sizeof(int2) * rowmap.size(), b->stream) != kOk ||
+      hipMemcpyAsync(b->rowmap, rowmap.data(), sizeof(int2) * rowmap.size(),
+                     hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess) {
+    batch_free(b);
+    set_error("cs_batch_create: device allocation failed");
+    return kOutOfMemory;
+  }
   v.tab = b->dtab;
   v.ptab = b->ptab;
+  v.rowmap = b->rowmap;
+  v.nrowblk = (int)rowmap.size();
   *out = reinterpret_cast<cs_batch*>(b);
   return kOk;
 }

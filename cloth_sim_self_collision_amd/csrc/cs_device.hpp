@@ -209,6 +209,8 @@ struct One {
   Params P;
   __device__ __forceinline__ const DevCloth& cloth() const { return c; }
   __device__ __forceinline__ const Params& params() const { return P; }
+  // first row of this block in a kernel that takes `rows` rows per block
+  __device__ __forceinline__ int block_row0(int rows) const { return (int)blockIdx.x * rows; }
 };
 
 struct Many {
@@ -216,6 +218,19 @@ struct Many {
   const Params* ptab;
   __device__ __forceinline__ DevCloth cloth() const { return tab[blockIdx.y]; }
   __device__ __forceinline__ Params params() const { return ptab[blockIdx.y]; }
+  __device__ __forceinline__ int block_row0(int rows) const { return (int)blockIdx.x * rows; }
+};
+
+// A batch's row kernel launched over the rows that exist rather than over
+// (rows of the largest cloth) x (cloths): block k takes the rows from
+// rowmap[k].y of cloth rowmap[k].x (one-dimensional grid of rowmap's length)
+struct ManyRows {
+  const DevCloth* tab;
+  const Params* ptab;
+  const int2* rowmap;
+  __device__ __forceinline__ DevCloth cloth() const { return tab[rowmap[blockIdx.x].x]; }
+  __device__ __forceinline__ Params params() const { return ptab[rowmap[blockIdx.x].x]; }
+  __device__ __forceinline__ int block_row0(int) const { return rowmap[blockIdx.x].y; }
 };
 
 

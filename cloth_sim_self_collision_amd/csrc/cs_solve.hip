@@ -1053,11 +1053,12 @@ __device__ __forceinline__ void spmv_dir_row(const DevCloth& c, int i, bool firs
 // adds them, so every result is the same bit for bit.  Rows with more than
 // 2 kSpG - 1 blocks fall back to spmv_dir_row on the group's first lane.
 constexpr int kSpG = 16;
+static_assert(kTB % kSpG == 0 && kTB / kSpG == kSpmvRows, "SpMV rows per block");
 template <class Sel>
 __global__ __launch_bounds__(kTB) void k_spmv_dir_g(Sel sel, int it) {
   const DevCloth c = sel.cloth();
   if (c.flags[kDone]) return;
-  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kSpG);
+  const int i = sel.block_row0(kTB / kSpG) + (int)(threadIdx.x / kSpG);
   const int l = (int)(threadIdx.x % kSpG);
   if (i >= c.np) return;  // a whole group at a time
   const bool first = it == 0;
@@ -1696,8 +1697,8 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
   for (int it = 0; it < maxit && !done; it++) {
     {
       ProfScope ps(prof, kProfSpmv);
-      hipLaunchKernelGGL(k_spmv_dir_g<Many>, dim3(grid_for((long)kSpG * b.max_np), B), dim3(kTB), 0,
-                         s, m, it);
+      hipLaunchKernelGGL(k_spmv_dir_g<ManyRows>, dim3(b.nrowblk), dim3(kTB), 0, s,
+                         ManyRows{b.tab, b.ptab, b.rowmap}, it);
     }
     {
       ProfScope ps(prof, kProfReduce);

@@ -56,7 +56,11 @@ struct BatchView {
   int kpack;   // ... with this many packed blocks per row
   bool wg;     // every cloth takes the one-workgroup PCG solve (one workgroup per cloth)
   bool f32;    // every cloth's packed sweep coefficients are FP32 (mixed precision)
+  const int2* rowmap;  // the SpMV's blocks: (cloth, first row), kSpmvRows rows each
+  int nrowblk;         // entries of rowmap
 };
+// rows per block of the batched SpMV (its rowmap is built with this step)
+constexpr int kSpmvRows = 16;
 // StepTime_InternalDynamicsILU for every cloth of the batch (start positions
 // saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); dgather/hgather hold n
 // ints (device / pinned host) for the convergence poll; with aux, the (r, r)
