@@ -1062,6 +1062,7 @@ struct Batch {
   DevCloth* dtab = nullptr;  // device copies of the cloths' DevCloth / Params
   Params* ptab = nullptr;
   int2* rowmap = nullptr;  // the batched SpMV's (cloth, first row) per block
+  int2* vecmap = nullptr;  // the batched vector update's (cloth, first entry) per block
   int* dgather = nullptr;
   int* hgather = nullptr;  // pinned
   hipStream_t stream = nullptr;
@@ -1122,6 +1123,7 @@ void batch_free(Batch* b) {
   dev_free(b->dtab, b->stream);
   dev_free(b->ptab, b->stream);
   dev_free(b->rowmap, b->stream);
+  dev_free(b->vecmap, b->stream);
   dev_free(b->dgather, b->stream);
   pinned_free(b->hgather);
   if (b->internal_done) (void)hipEventDestroy(b->internal_done);
@@ -1218,12 +1220,18 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
       set_error("cs_batch_create: event creation failed");
       return kHipError;
     }
-  // the SpMV's blocks over the rows that exist, kSpmvRows rows each
-  std::vector<int2> rowmap;
-  for (int i = 0; i < n; i++)
+  // the SpMV's blocks over the rows that exist (kSpmvRows rows each) and the
+  // vector update's over the entries that exist (kVecBlock each)
+  std::vector<int2> rowmap, vecmap;
+  for (int i = 0; i < n; i++) {
     for (int r0 = 0; r0 < b->cl[i]->d.np; r0 += kSpmvRows) rowmap.push_back(make_int2(i, r0));
+    for (int e0 = 0; e0 < 3 * b->cl[i]->d.np; e0 += kVecBlock) vecmap.push_back(make_int2(i, e0));
+  }
   if (dev_alloc((void**)&b->rowmap, sizeof(int2) * rowmap.size(), b->stream) != kOk ||
+      dev_alloc((void**)&b->vecmap, sizeof(int2) * vecmap.size(), b->stream) != kOk ||
       hipMemcpyAsync(b->rowmap, rowmap.data(), sizeof(int2) * rowmap.size(),
+                     hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(b->vecmap, vecmap.data(), sizeof(int2) * vecmap.size(),
                      hipMemcpyHostToDevice, b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess) {
     batch_free(b);
@@ -1234,6 +1242,8 @@ extern "C" int cs_batch_create(int n, cs_cloth* const* cloths, cs_batch** out) {
   v.ptab = b->ptab;
   v.rowmap = b->rowmap;
   v.nrowblk = (int)rowmap.size();
+  v.vecmap = b->vecmap;
+  v.nvecblk = (int)vecmap.size();
   *out = reinterpret_cast<cs_batch*>(b);
   return kOk;
 }

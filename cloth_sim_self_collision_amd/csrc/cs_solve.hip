@@ -1054,6 +1054,7 @@ __device__ __forceinline__ void spmv_dir_row(const DevCloth& c, int i, bool firs
 // 2 kSpG - 1 blocks fall back to spmv_dir_row on the group's first lane.
 constexpr int kSpG = 16;
 static_assert(kTB % kSpG == 0 && kTB / kSpG == kSpmvRows, "SpMV rows per block");
+static_assert(kTB == kVecBlock, "vector-update entries per block");
 template <class Sel>
 __global__ __launch_bounds__(kTB) void k_spmv_dir_g(Sel sel, int it) {
   const DevCloth c = sel.cloth();
@@ -1151,9 +1152,10 @@ __global__ void k_update_rx(Sel sel, int it) {
   const DevCloth c = sel.cloth();
   const double* __restrict__ p = pcg_dir(c, it);
   const int done = c.flags[kDone];
-  if (blockIdx.x == 0 && threadIdx.x == 0) c.flags[kSkip] = done;
+  const int e0 = sel.block_row0(kTB);  // first entry of this block
+  if (e0 == 0 && threadIdx.x == 0) c.flags[kSkip] = done;
   if (done) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = e0 + (int)threadIdx.x;
   if (e >= 3 * c.np) return;
   update_rx_elem(c, e, c.scal[kAlpha], p);
 }
@@ -1706,7 +1708,8 @@ int batch_internal_step(const BatchView& b, double tol, int maxit, int poll, int
     }
     {
       ProfScope ps(prof, kProfAxpy);
-      hipLaunchKernelGGL(k_update_rx<Many>, dim3(gv, B), dim3(kTB), 0, s, m, it);
+      hipLaunchKernelGGL(k_update_rx<ManyRows>, dim3(b.nvecblk), dim3(kTB), 0, s,
+                         ManyRows{b.tab, b.ptab, b.vecmap}, it);
     }
     if (aux) {
       // as in run_pcg: the convergence test of every cloth beside the sweep

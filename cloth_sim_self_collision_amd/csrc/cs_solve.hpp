@@ -58,9 +58,13 @@ struct BatchView {
   bool f32;    // every cloth's packed sweep coefficients are FP32 (mixed precision)
   const int2* rowmap;  // the SpMV's blocks: (cloth, first row), kSpmvRows rows each
   int nrowblk;         // entries of rowmap
+  const int2* vecmap;  // the vector update's blocks: (cloth, first of 3 np entries)
+  int nvecblk;         // entries of vecmap
 };
-// rows per block of the batched SpMV (its rowmap is built with this step)
+// rows per block of the batched SpMV and entries per block of the batched
+// vector update (their block maps are built with these steps)
 constexpr int kSpmvRows = 16;
+constexpr int kVecBlock = 256;
 // StepTime_InternalDynamicsILU for every cloth of the batch (start positions
 // saved, assembly, ILU(0), PCG, x += dx, v = dx/dt); dgather/hgather hold n
 // ints (device / pinned host) for the convergence poll; with aux, the (r, r)
